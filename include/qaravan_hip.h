@@ -1,0 +1,180 @@
+/*
+ * qaravan_hip.h — C ABI of the MI355X (gfx950) statevector backend.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and returns
+ * an int status: 0 = success, < 0 = argument error (message in svh_last_error()),
+ * > 0 = the hipError_t that a HIP call returned.  All device buffers (state
+ * vectors, programs, scratch) are allocated by the caller; the library keeps no
+ * device memory of its own.  Work is enqueued on the given stream (NULL = the
+ * default stream) and entry points return without synchronising unless stated.
+ *
+ * Qubit/bit convention: the state is a flat vector of 2^n complex amplitudes,
+ * interleaved (re, im) in float (dtype SVH_C64) or double (SVH_C128).  The
+ * library works in ADDRESS BITS: amplitude index i has bit b = (i >> b) & 1.
+ * qaravan's big-endian qubit q is address bit n-1-q; the host side converts.
+ *
+ * Reference interfaces these replace (qaravan 0.1.57, src/qaravan/...):
+ *   svh_init_basis        tensorQ/statevector_sim.py:31-48  StatevectorSim.initialize_state
+ *   svh_apply_dense       tensorQ/statevector_sim.py:50-52,124-140  apply_gate -> op_action
+ *   svh_run_program       core/base_sim.py:30-53  BaseSim.run (the per-layer/per-gate loop)
+ *   svh_norm2             tensorQ/statevector_sim.py:107-109  normalize_state (norm)
+ *   svh_scale             tensorQ/statevector_sim.py:107-109  normalize_state (divide)
+ *   svh_pauli_expect      tensorQ/statevector_sim.py:217-234  pauli_expectation_sv, summed
+ *                         over Hamiltonian terms (core/hamiltonians.py:19-88)
+ *   svh_pauli_apply       .dev/test_hamiltonians.ipynb:709-712 ham_action (H·ψ via op_action)
+ *   svh_inner             tensorQ/statevector_sim.py:155-180 partial_overlap with no kept
+ *                         sites, i.e. ⟨a|b⟩ (applications/compilation.py:86 torch.dot)
+ *   svh_run_adjoint       applications/compilation.py:73-133 (cost.backward() through the
+ *                         torch backend); same gradient, computed by the adjoint method
+ *   svh_apply_blocks_mfma no counterpart: op_action applied to 5-qubit gates one by one
+ *   svh_batch_run         tensorQ/monte_carlo_sim.py:14-35 (many small StatevectorSim runs)
+ */
+#ifndef QARAVAN_HIP_H
+#define QARAVAN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVH_ABI_VERSION 1
+
+enum { SVH_C64 = 0, SVH_C128 = 1 };
+
+/* ---- gate program ---------------------------------------------------------
+ * A program is one contiguous byte buffer: svh_prog_header_t, then the pass,
+ * stage, op, grad-slot (int32) and matrix-pool (double) arrays at the byte
+ * offsets the header gives.  The caller builds it on the host, copies it to
+ * device memory, and passes both copies to svh_run_program / svh_run_adjoint:
+ * the host copy drives the launches, the device copy is read by the kernels.
+ */
+#define SVH_PROG_MAGIC 0x53564850u /* 'SVHP' */
+#define SVH_MAX_TILE_BITS 16
+#define SVH_MAX_REG_BITS 5
+
+enum {
+    SVH_PASS_TILE = 1,      /* fused pass over tiles of 2^L amplitudes            */
+    SVH_PASS_DENSE = 2,     /* one dense k-qubit gate (k <= 6) from the pool        */
+    SVH_PASS_NORMALIZE = 3  /* ψ /= ‖ψ‖₂ (FP64 norm), used after Reset projectors   */
+};
+
+enum {
+    SVH_OP_MAT1 = 1, /* dense 2x2 on register bit a                               */
+    SVH_OP_MAT2 = 2, /* dense 4x4 on register bits (a, b), a > b, a = row MSB       */
+    SVH_OP_CX = 3,   /* controlled-X: a = target bit, b = control bit              */
+    SVH_OP_CZ = 4,   /* diag(1,1,1,-1) on (a, b)                                   */
+    SVH_OP_ROT1 = 5, /* exp(-i·s·θ[param]·P), P = pauli (1 X, 2 Y, 3 Z), on a       */
+    SVH_OP_ROT2 = 6  /* exp(-i·s·θ[param]·P⊗P) on (a, b)                           */
+};
+
+enum {
+    SVH_OPF_NEG = 1, /* s = -1 (the adjoint sweep applies U†)                     */
+    SVH_OPF_GRAD = 2 /* before applying: grad_slot += 2·Im⟨λ|P|ψ⟩ (adjoint only)   */
+};
+
+typedef struct {
+    uint32_t magic;
+    int32_t version;
+    int32_t n_qubits;
+    int32_t n_passes;
+    int32_t n_stages;
+    int32_t n_ops;
+    int32_t n_grad_slots;
+    int32_t n_pool; /* doubles in the matrix pool */
+    int64_t off_passes, off_stages, off_ops, off_grad, off_pool;
+    int64_t reserved[3];
+} svh_prog_header_t; /* 96 bytes */
+
+typedef struct {
+    int32_t kind;        /* SVH_PASS_*                                              */
+    int32_t L;           /* tile bits (TILE) or gate qubits k (DENSE)              */
+    int32_t tb;          /* log2 threads per workgroup (TILE)                       */
+    int32_t rb;          /* register bits per thread (TILE); L = tb + rb            */
+    int32_t stage_begin; /* first stage (TILE)                                       */
+    int32_t n_stages;
+    int32_t grad_begin;  /* first grad slot of this pass (adjoint)                  */
+    int32_t n_grad;
+    int64_t pool_off;    /* DENSE: offset (doubles) of the 2^k x 2^k complex matrix  */
+    int32_t bits[SVH_MAX_TILE_BITS]; /* TILE: global bit of tile-local bit i (i < L);
+                                        DENSE: global bit of matrix qubit i (i = 0 is
+                                        the most significant row bit)                */
+} svh_pass_t; /* 104 bytes */
+
+typedef struct {
+    int32_t rbits[8];    /* tile-local bit held in register-index bit k (k < rb)   */
+    int32_t op_begin, op_end;
+} svh_stage_t; /* 40 bytes */
+
+typedef struct {
+    int32_t kind;        /* SVH_OP_*                                                */
+    int32_t a, b;        /* register-index bits                                      */
+    int32_t param;       /* θ index (ROT*)                                           */
+    int32_t pauli;       /* 1 X, 2 Y, 3 Z (ROT*)                                     */
+    int32_t flags;       /* SVH_OPF_*                                                */
+    int32_t grad_slot;   /* pass-local gradient slot (SVH_OPF_GRAD)                  */
+    int32_t pad;
+    double m[32];        /* complex row-major matrix (re, im pairs)                  */
+} svh_op_t; /* 288 bytes */
+
+/* ---- status ---------------------------------------------------------------- */
+const char *svh_last_error(void);
+int svh_abi_version(void);
+
+/* ψ = |idx⟩ (all zeros, amplitude idx = 1). */
+int svh_init_basis(void *psi, int n, int dtype, uint64_t idx, void *stream);
+
+/* Apply a dense k-qubit gate (1 <= k <= 6) in place.  bits[a] = address bit of
+ * matrix qubit a (a = 0 most significant); mat_dev = device 2^k x 2^k complex
+ * matrix, row-major, in the state's precision. */
+int svh_apply_dense(void *psi, int n, int dtype, const int *bits, int k,
+                    const void *mat_dev, void *stream);
+
+/* Bytes of device scratch svh_run_program / svh_run_adjoint need for `prog`. */
+int svh_program_scratch_bytes(const void *prog_host, int dtype, size_t *bytes);
+
+/* Run every pass of a program on ψ in place.  theta_dev: device float64 angles
+ * for ROT ops (may be NULL if the program has none). */
+int svh_run_program(void *psi, int n, int dtype, const void *prog_host,
+                    const void *prog_dev, const double *theta_dev, void *scratch,
+                    size_t scratch_bytes, void *stream);
+
+/* Adjoint sweep: the program holds the reversed, daggered gate list with
+ * SVH_OPF_GRAD on rotations.  Walks ψ and λ back together and ADDS
+ * 2·Im⟨λ_k|P_k|ψ_k⟩ into grad_dev[param] (device float64, caller zeroes it). */
+int svh_run_adjoint(void *psi, void *lam, int n, int dtype, const void *prog_host,
+                    const void *prog_dev, const double *theta_dev, double *grad_dev,
+                    void *scratch, size_t scratch_bytes, void *stream);
+
+/* out_dev[0] = Σ|ψ_i|² accumulated in FP64.  scratch >= 8·1024 bytes. */
+int svh_norm2(const void *psi, int n, int dtype, double *out_dev, void *scratch,
+              size_t scratch_bytes, void *stream);
+
+/* ψ *= 1/sqrt(norm2_dev[0]) (norm2_dev: device double written by svh_norm2). */
+int svh_scale_by_inv_norm(void *psi, int n, int dtype, const double *norm2_dev,
+                          void *stream);
+
+/* Pauli sums: term t is (xmask[t], zmask[t], coeff[t]) with P_t|i⟩ =
+ * i^{ny_t}·(-1)^{popcount(i & zmask)}·|i ^ xmask⟩, ny_t = popcount(x & z).
+ * Term arrays are HOST arrays (n_terms <= 4096), coeff real.
+ * svh_pauli_expect: out_dev[0] = Σ_t c_t Re⟨ψ|P_t|ψ⟩ (FP64 accumulation). */
+int svh_pauli_expect(const void *psi, int n, int dtype, int n_terms,
+                     const uint64_t *xmask, const uint64_t *zmask, const double *coeff,
+                     double *out_dev, void *scratch, size_t scratch_bytes, void *stream);
+
+/* out = Σ_t c_t P_t ψ  (out must not alias psi); the term table is staged in
+ * scratch (>= 32·n_terms bytes). */
+int svh_pauli_apply(const void *psi, void *out, int n, int dtype, int n_terms,
+                    const uint64_t *xmask, const uint64_t *zmask, const double *coeff,
+                    void *scratch, size_t scratch_bytes, void *stream);
+
+/* out_dev[0..1] = ⟨a|b⟩ (re, im) accumulated in FP64. */
+int svh_inner(const void *a, const void *b, int n, int dtype, double *out_dev,
+              void *scratch, size_t scratch_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QARAVAN_HIP_H */

@@ -1,0 +1,276 @@
+"""Numpy complex128 restatement of qaravan's dense statevector path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU
+baseline of the benchmark, never by the product package.
+
+What each function restates (paths relative to the qaravan source tree):
+
+* ``op_action``      src/qaravan/tensorQ/statevector_sim.py:111-140
+                     (sort indices, permute the gate tensor to match, contract the
+                     gate's input legs with the state's legs, outputs in place).
+                     ncon_torch==0.3 (absent here) is replaced by np.tensordot
+                     followed by np.moveaxis; the contraction is the same.
+* ``construct_layers`` src/qaravan/core/circuits.py:42-63 (greedy ASAP layers).
+* ``run``            src/qaravan/core/base_sim.py:30-53 and
+                     statevector_sim.py:31-52,102-109 (init |0…0⟩ / bitstring /
+                     array, per-gate loop over layers, Reset = projector then
+                     renormalise).
+* ``circ_to_mat``    src/qaravan/core/circuits.py:177-192 (Kronecker brute force).
+* ``embed_full``     independent brute force: the 2^n×2^n matrix of a gate on
+                     arbitrary indices built entry by entry from the big-endian
+                     convention (qubit q ↔ bit n-1-q).
+* ``pauli_expectation`` statevector_sim.py:217-234.
+* ``tfi_terms``      src/qaravan/core/hamiltonians.py:19-88,261-272.
+
+Pinning (each is exercised in tests/test_oracle.py):
+* reference known answers: tests/test_core.py:8-41,127-137 (GHZ amplitudes,
+  ⟨ZZ⟩, ⟨Z⟩) and tests/test_resets.py:22-122 (seven exact Reset vectors);
+* op_action vs embed_full on random gates/indices (n<=8) and circ_to_mat vs
+  embed_full on contiguous-index circuits;
+* rotation closed forms vs scipy.linalg.expm;
+* TFI: ground energy of TFI(4, 1, 1)-type chains vs numpy.linalg.eigh of the
+  embedded matrix, ⟨+…+|H|+…+⟩ = h·n, ⟨0…0|H|0…0⟩ = jz·(n-1);
+* gradients: adjoint (this file) vs parameter shift (exact for exp(-iθP)) vs
+  central finite differences.
+"""
+import numpy as np
+
+PAULI = {
+    'i': np.eye(2, dtype=np.complex128),
+    'x': np.array([[0, 1], [1, 0]], dtype=np.complex128),
+    'y': np.array([[0, -1j], [1j, 0]], dtype=np.complex128),
+    'z': np.array([[1, 0], [0, -1]], dtype=np.complex128),
+}
+
+
+# ---------------------------------------------------------------- gate action
+def op_action(op, indices, sv, local_dim=2):
+    """ψ'[..i_L..] = Σ_j M[i_L, j_L] ψ[..j_L..] with L = ``indices`` (first index
+    is the most significant row bit).  Returns a new array with the input's
+    shape (flat in → flat out, rank-n in → rank-n out)."""
+    op = np.asarray(op, dtype=np.complex128)
+    k = len(indices)
+    flat = sv.ndim == 1
+    n = int(round(np.log(len(sv)) / np.log(local_dim))) if flat else sv.ndim
+    state = np.asarray(sv, dtype=np.complex128).reshape([local_dim] * n)
+    t = op.reshape([local_dim] * (2 * k))
+    order = sorted(range(k), key=lambda a: indices[a])
+    t = np.transpose(t, order + [k + a for a in order])
+    locs = [indices[a] for a in order]
+    out = np.tensordot(t, state, axes=(list(range(k, 2 * k)), locs))
+    out = np.moveaxis(out, list(range(k)), locs)
+    return out.reshape(local_dim ** n) if flat else out
+
+
+def construct_layers(gate_list):
+    layers, busy = [], []
+    for g in gate_list:
+        s = set(g.indices)
+        li = len(layers) - 1
+        while li >= 0 and not (s & busy[li]):
+            li -= 1
+        if li < len(layers) - 1:
+            layers[li + 1].append(g)
+            busy[li + 1] |= s
+        else:
+            layers.append([g])
+            busy.append(set(s))
+    return layers
+
+
+def init_state(n, init=None, local_dim=2):
+    if init is None:
+        sv = np.zeros(local_dim ** n, dtype=np.complex128)
+        sv[0] = 1.0
+    elif isinstance(init, str):
+        sv = np.zeros(local_dim ** len(init), dtype=np.complex128)
+        sv[int(init, local_dim)] = 1.0
+    else:
+        sv = np.asarray(init, dtype=np.complex128).reshape(-1).copy()
+    return sv
+
+
+def run(gate_list, n, init=None, layered=True):
+    """Final flat statevector (complex128)."""
+    sv = init_state(n, init)
+    seq = [g for layer in construct_layers(gate_list) for g in layer] if layered else gate_list
+    for g in seq:
+        sv = op_action(np.asarray(g.matrix), g.indices, sv)
+        if g.name == "Reset" and getattr(g, "normalize", True):
+            sv = sv / np.linalg.norm(sv)
+    return sv
+
+
+# ------------------------------------------------------------- brute forces
+def embed_full(mat, indices, n):
+    """Full 2^n×2^n matrix of ``mat`` on ``indices`` (big-endian), built entry by
+    entry: row/col bit of gate index a is qubit indices[a] = address bit n-1-q."""
+    k = len(indices)
+    mat = np.asarray(mat, dtype=np.complex128)
+    N = 2 ** n
+    full = np.zeros((N, N), dtype=np.complex128)
+    shifts = [n - 1 - q for q in indices]
+    clear = 0
+    for s in shifts:
+        clear |= 1 << s
+    for col in range(N):
+        jc = 0
+        for a, s in enumerate(shifts):
+            jc |= ((col >> s) & 1) << (k - 1 - a)
+        base = col & ~clear
+        for ir in range(2 ** k):
+            amp = mat[ir, jc]
+            if amp == 0:
+                continue
+            row = base
+            for a, s in enumerate(shifts):
+                row |= ((ir >> (k - 1 - a)) & 1) << s
+            full[row, col] += amp
+    return full
+
+
+def circ_to_mat(gate_list, n):
+    """Kronecker brute force: per layer pad idle qubits with identity, sort
+    gates by first index, Kronecker them, multiply layers right to left."""
+    mats = []
+    for layer in construct_layers(gate_list):
+        used = {q for g in layer for q in g.indices}
+        items = [(g.indices[0], np.asarray(g.matrix, dtype=np.complex128)) for g in layer]
+        items += [(q, np.eye(2, dtype=np.complex128)) for q in range(n) if q not in used]
+        items.sort(key=lambda t: t[0])
+        m = items[0][1]
+        for _, x in items[1:]:
+            m = np.kron(m, x)
+        mats.append(m)
+    out = mats[0]
+    for m in mats[1:]:
+        out = m @ out
+    return out
+
+
+# ------------------------------------------------------------- rotations
+def rotation(word, theta):
+    """exp(-iθP) for a Pauli word P (closed form, P² = I)."""
+    p = np.array([[1.0 + 0j]])
+    for ch in word:
+        p = np.kron(p, PAULI[ch])
+    return np.cos(theta) * np.eye(p.shape[0]) - 1j * np.sin(theta) * p
+
+
+# ------------------------------------------------------------- observables
+def pauli_string_matrix(ps):
+    m = np.array([[1.0 + 0j]])
+    for ch in ps:
+        m = np.kron(m, PAULI[ch])
+    return m
+
+
+def apply_pauli_string(sv, ps):
+    n = len(ps)
+    out = np.asarray(sv, dtype=np.complex128)
+    for q, ch in enumerate(ps):
+        if ch != 'i':
+            out = op_action(PAULI[ch], [q], out)
+    return out
+
+
+def pauli_expectation(sv, ps):
+    """Re⟨ψ|P|ψ⟩ (statevector_sim.py:217-234)."""
+    return float(np.real(np.vdot(sv, apply_pauli_string(sv, ps))))
+
+
+def tfi_terms(n, jz=1.0, h=0.0, periodic=False):
+    """(coeff, pauli string) list: jz Σ Z_iZ_{i+1} then h Σ X_i."""
+    terms = []
+    pairs = [(i, i + 1) for i in range(n - 1)] + ([(n - 1, 0)] if periodic else [])
+    for a, b in pairs:
+        s = ['i'] * n
+        s[a] = s[b] = 'z'
+        terms.append((jz, ''.join(s)))
+    if abs(h) > 1e-10:
+        for i in range(n):
+            s = ['i'] * n
+            s[i] = 'x'
+            terms.append((h, ''.join(s)))
+    return terms
+
+
+def apply_pauli_sum(sv, terms):
+    out = np.zeros_like(np.asarray(sv, dtype=np.complex128))
+    for c, ps in terms:
+        out += c * apply_pauli_string(sv, ps)
+    return out
+
+
+def energy(sv, terms):
+    return float(np.real(np.vdot(sv, apply_pauli_sum(sv, terms))))
+
+
+def pauli_sum_matrix(terms, n):
+    H = np.zeros((2 ** n, 2 ** n), dtype=np.complex128)
+    for c, ps in terms:
+        H += c * pauli_string_matrix(ps)
+    return H
+
+
+# ------------------------------------------------------------- gradients
+# A parametrised circuit is a list of (kind, qubits, param_index_or_None, matrix)
+#   kind = Pauli word ('y', 'z', 'xx', ...) for exp(-iθP) with θ = theta[param]
+#   kind = None for a fixed gate given by ``matrix``.
+def param_run(spec, theta, n, init=None):
+    sv = init_state(n, init)
+    for kind, qs, p, mat in spec:
+        m = rotation(kind, theta[p]) if kind is not None else mat
+        sv = op_action(m, qs, sv)
+    return sv
+
+
+def param_energy(spec, theta, n, terms):
+    return energy(param_run(spec, theta, n), terms)
+
+
+def adjoint_gradient(spec, theta, n, terms):
+    """dE/dθ by the adjoint method: ψ and λ = Hψ are walked back through U_k†;
+    for a rotation exp(-iθP), dE/dθ_k = 2 Im⟨λ_k|P|ψ_k⟩."""
+    psi = param_run(spec, theta, n)
+    lam = apply_pauli_sum(psi, terms)
+    e = float(np.real(np.vdot(psi, lam)))
+    grad = np.zeros(len(theta))
+    for kind, qs, p, mat in reversed(spec):
+        if kind is not None:
+            pm = pauli_string_matrix(kind)
+            grad[p] += 2.0 * float(np.imag(np.vdot(lam, op_action(pm, qs, psi))))
+            u = rotation(kind, theta[p])
+        else:
+            u = np.asarray(mat, dtype=np.complex128)
+        ud = u.conj().T
+        psi = op_action(ud, qs, psi)
+        lam = op_action(ud, qs, lam)
+    return e, grad
+
+
+def param_shift_gradient(spec, theta, n, terms, idx=None):
+    """Exact for exp(-iθP): E(θ) = a + b cos 2θ + c sin 2θ, so
+    dE/dθ = E(θ+π/4) − E(θ−π/4)."""
+    idx = range(len(theta)) if idx is None else idx
+    g = np.zeros(len(idx))
+    for j, i in enumerate(idx):
+        tp = np.array(theta, dtype=np.float64)
+        tm = tp.copy()
+        tp[i] += np.pi / 4
+        tm[i] -= np.pi / 4
+        g[j] = param_energy(spec, tp, n, terms) - param_energy(spec, tm, n, terms)
+    return g
+
+
+def fd_gradient(spec, theta, n, terms, eps=1e-3, idx=None):
+    """Central differences; for this gate family equals E'(θ)·sin(2ε)/(2ε)."""
+    idx = range(len(theta)) if idx is None else idx
+    g = np.zeros(len(idx))
+    for j, i in enumerate(idx):
+        tp = np.array(theta, dtype=np.float64)
+        tm = tp.copy()
+        tp[i] += eps
+        tm[i] -= eps
+        g[j] = (param_energy(spec, tp, n, terms) - param_energy(spec, tm, n, terms)) / (2 * eps)
+    return g

@@ -1,0 +1,137 @@
+"""Adjoint differentiation of ⟨ψ(θ)|H|ψ(θ)⟩ behind ``torch.autograd.Function``.
+
+The reference differentiates by tape autograd through its torch backend
+(src/qaravan/applications/compilation.py:73-133: matrix_exp gates, tensordot per
+gate, ``cost.backward()``), which stores every intermediate state: O(G·2^n)
+memory.  Here the backward is the adjoint method in O(2^n):
+
+  forward   ψ_G = U_G…U_1|0…0⟩ (fused tile passes, θ read on the device),
+            λ_G = Hψ_G (Pauli-sum kernel), E = Re⟨ψ_G|λ_G⟩ (FP64 reduction)
+  backward  for k = G…1:  dE/dθ_k += 2·Im⟨λ_k|P_k|ψ_k⟩   (gate k = exp(−iθ_k P_k))
+                          ψ_{k-1} = U_k† ψ_k,  λ_{k-1} = U_k† λ_k
+            done by one program of reversed, daggered ops with SVH_OPF_GRAD on
+            the rotations; the ⟨λ|Pψ⟩ sums are accumulated in FP64 per
+            workgroup and reduced in a fixed order.
+
+Rotations are qaravan's RX/RY/RZ/RXX/RYY/RZZ (U = exp(−iθP), no ½,
+core/param_gates.py:51-103); a gate takes its angle from θ[g.param_index].
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .program import OPF_GRAD, TileConfig, compile_ops, lower_circuit
+from .statevector_sim import DeviceScratch, inner, pauli_sum_apply, run_program
+
+
+def _adjoint_cfg(n, dtype, cfg):
+    """The adjoint pass holds ψ and λ: cap the workgroup size so both fit in
+    registers (c64: ≤ 512 threads, c128: ≤ 256)."""
+    tb_max = 9 if dtype == torch.complex64 else 8
+    tb = min(cfg.tb, tb_max, n - 4)
+    return TileConfig(tb=tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost)
+
+
+class ParamProgram:
+    """Forward and adjoint programs of a parametrised circuit plus its loss."""
+
+    def __init__(self, circ, terms, n_params, cfg=None, adj_cfg=None, dtype=torch.complex64):
+        self.n = circ.num_sites
+        if self.n < 8:
+            raise NotImplementedError("the adjoint path needs n >= 8 qubits")
+        self.terms = list(terms)
+        self.n_params = int(n_params)
+        self.dtype = dtype
+        cfg = cfg or TileConfig()
+        adj_cfg = adj_cfg or cfg
+        ops = lower_circuit(circ if circ.built else circ.build(), self.n, param_mode=True)
+        for op in ops:
+            if op.kind in ('dense', 'norm'):
+                raise NotImplementedError("gates on > 2 qubits and resets are not differentiable here")
+            if op.kind in ('rot1', 'rot2') and not 0 <= op.param < self.n_params:
+                raise ValueError(f"param index {op.param} outside [0, {self.n_params})")
+        self.fwd = compile_ops(ops, self.n, TileConfig(tb=min(cfg.tb, self.n - 4), low_bits=cfg.low_bits,
+                                                       max_cost=cfg.max_cost), n_params=self.n_params)
+        back = []
+        for op in reversed(ops):
+            d = op.dagger()
+            if d.kind in ('rot1', 'rot2'):
+                d.flags |= OPF_GRAD
+            back.append(d)
+        self.adj = compile_ops(back, self.n, _adjoint_cfg(self.n, dtype, adj_cfg), grad=True,
+                               n_params=self.n_params)
+        self.n_gates = sum(1 for _ in ops)
+
+
+def _theta64(theta, device):
+    return theta.detach().to(device=device, dtype=torch.float64).contiguous()
+
+
+def forward_state(pp, theta, device, scratch):
+    """ψ(θ) and λ = Hψ (device tensors) for a ParamProgram."""
+    lib = _lib.load()
+    n = pp.n
+    th = _theta64(theta, device)
+    psi = torch.empty(2 ** n, dtype=pp.dtype, device=device)
+    _lib.check(lib.svh_init_basis(_lib.ptr(psi), n, _lib.dtype_code(pp.dtype), 0, _lib.stream_ptr(device)),
+               "svh_init_basis")
+    run_program(psi, pp.fwd, scratch, th)
+    lam = torch.empty_like(psi)
+    pauli_sum_apply(psi, lam, n, pp.terms, scratch)
+    return psi, lam, th
+
+
+def adjoint_sweep(pp, psi, lam, th, scratch):
+    """dE/dθ (float64 device tensor); consumes ψ and λ (walked back in place)."""
+    import ctypes
+    lib = _lib.load()
+    n = pp.n
+    grad = torch.zeros(pp.n_params, dtype=torch.float64, device=psi.device)
+    code = _lib.dtype_code(psi.dtype)
+    need = ctypes.c_size_t()
+    _lib.check(lib.svh_program_scratch_bytes(pp.adj.blob.ctypes.data, code, ctypes.byref(need)),
+               "svh_program_scratch_bytes")
+    buf = scratch.get(need.value)
+    dev = pp.adj.device_copy(psi.device)
+    _lib.check(lib.svh_run_adjoint(_lib.ptr(psi), _lib.ptr(lam), n, code, pp.adj.blob.ctypes.data,
+                                   _lib.ptr(dev), _lib.ptr(th), _lib.ptr(grad), _lib.ptr(buf), buf.numel(),
+                                   _lib.stream_ptr(psi.device)), "svh_run_adjoint")
+    return grad
+
+
+class AdjointEnergy(torch.autograd.Function):
+    """E(θ) = ⟨ψ(θ)|H|ψ(θ)⟩ with an adjoint backward; saves only ψ and λ."""
+
+    @staticmethod
+    def forward(ctx, theta, pp, scratch):
+        psi, lam, th = forward_state(pp, theta, theta.device, scratch)
+        e = inner(psi, lam, pp.n, scratch).real
+        ctx.pp, ctx.psi, ctx.lam, ctx.th, ctx.scratch = pp, psi, lam, th, scratch
+        ctx.theta_dtype = theta.dtype
+        return e.to(torch.float64)
+
+    @staticmethod
+    def backward(ctx, g_e):
+        grad = adjoint_sweep(ctx.pp, ctx.psi, ctx.lam, ctx.th, ctx.scratch)
+        ctx.psi = ctx.lam = None
+        return (g_e * grad).to(ctx.theta_dtype), None, None
+
+
+class AdjointModel:
+    """Callable loss E(θ) for one parametrised circuit, differentiable by torch."""
+
+    def __init__(self, circ, terms, n_params, device="cuda", dtype=torch.complex64, cfg=None):
+        _lib.require_gpu()
+        self.device = torch.device(device)
+        self.pp = ParamProgram(circ, terms, n_params, cfg=cfg, dtype=dtype)
+        self.scratch = DeviceScratch(self.device)
+
+    def __call__(self, theta):
+        return AdjointEnergy.apply(theta, self.pp, self.scratch)
+
+    def energy_and_grad(self, theta):
+        """(E, dE/dθ) without building an autograd graph."""
+        psi, lam, th = forward_state(self.pp, theta, self.device, self.scratch)
+        e = inner(psi, lam, self.pp.n, self.scratch).real
+        g = adjoint_sweep(self.pp, psi, lam, th, self.scratch)
+        return e, g

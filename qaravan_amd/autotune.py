@@ -1,0 +1,93 @@
+"""Autotuner for the fused tile passes.
+
+Picks (tb, low_bits, max_cost) — workgroup size (tile = tb + 4 bits), the
+number of low address bits every tile keeps (contiguous run length), and the
+per-pass ALU cap — by timing the compiled program of a representative circuit
+on the GPU.  Results are cached per (n, dtype) in tune_cache.json next to this
+file; ``best_tile_config`` falls back to the built-in defaults when no entry
+exists.
+"""
+import itertools
+import json
+import os
+import time
+
+from .program import TileConfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE = os.path.join(HERE, "tune_cache.json")
+
+DEFAULTS = {"complex64": TileConfig(tb=8, low_bits=6, max_cost=96.0),
+            "complex128": TileConfig(tb=8, low_bits=6, max_cost=96.0)}
+
+
+def _key(n, dtype):
+    return f"{str(dtype).replace('torch.', '')}:{n}"
+
+
+def _load():
+    if os.path.exists(CACHE):
+        with open(CACHE) as f:
+            return json.load(f)
+    return {}
+
+
+def best_tile_config(n, dtype):
+    cache = _load()
+    name = str(dtype).replace('torch.', '')
+    e = cache.get(_key(n, dtype))
+    if e is None:
+        # nearest tuned size for this dtype
+        cands = [(abs(int(k.split(':')[1]) - n), v) for k, v in cache.items() if k.startswith(name + ':')]
+        if cands:
+            e = min(cands, key=lambda t: t[0])[1]
+    base = DEFAULTS.get(name, TileConfig())
+    if e is None:
+        cfg = base
+    else:
+        cfg = TileConfig(tb=e["tb"], low_bits=e["low_bits"], max_cost=e["max_cost"])
+    return TileConfig(tb=min(cfg.tb, n - 4), low_bits=cfg.low_bits, max_cost=cfg.max_cost)
+
+
+def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print):
+    """Time every configuration in ``space`` on ``circ`` and cache the fastest."""
+    import torch
+
+    from . import _lib
+    from .program import compile_ops, lower_circuit
+    from .statevector_sim import DeviceScratch, run_program
+    n = circ.num_sites
+    space = space or dict(tb=[6, 7, 8, 9], low_bits=[4, 5, 6], max_cost=[48.0, 72.0, 96.0, 144.0, 1e9])
+    lib = _lib.load()
+    ops = lower_circuit(circ.build(), n)
+    psi = torch.empty(2 ** n, dtype=dtype, device=device)
+    scratch = DeviceScratch(device)
+    results = []
+    for tb, lb, mc in itertools.product(space["tb"], space["low_bits"], space["max_cost"]):
+        if tb + 4 > n or (dtype == torch.complex128 and tb > 9):
+            continue
+        cfg = TileConfig(tb=tb, low_bits=lb, max_cost=mc)
+        prog = compile_ops(ops, n, cfg)
+        times = []
+        for r in range(repeats + 1):
+            _lib.check(lib.svh_init_basis(_lib.ptr(psi), n, _lib.dtype_code(dtype), 0, _lib.stream_ptr(device)),
+                       "init")
+            torch.cuda.synchronize(device)
+            t0 = time.perf_counter()
+            run_program(psi, prog, scratch)
+            torch.cuda.synchronize(device)
+            if r:
+                times.append(time.perf_counter() - t0)
+        t = min(times)
+        results.append((t, cfg, prog.n_tile_passes()))
+        log(f"[tune] n={n} {dtype} tb={tb} low={lb} cap={mc:g}: {t * 1e3:.3f} ms, "
+            f"{prog.n_tile_passes()} passes")
+    results.sort(key=lambda x: x[0])
+    t, cfg, npass = results[0]
+    if write:
+        cache = _load()
+        cache[_key(n, dtype)] = dict(tb=cfg.tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost,
+                                     ms=t * 1e3, passes=npass)
+        with open(CACHE, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+    return cfg, results

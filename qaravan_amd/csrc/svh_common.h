@@ -1,0 +1,63 @@
+// Shared device/host helpers for the gfx950 statevector kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <string>
+#include <type_traits>
+
+#include "../../include/qaravan_hip.h"
+
+namespace svh {
+
+// Interleaved complex amplitude; alignment makes every load/store one
+// 8-byte (float) or 16-byte (double) vector access.
+template <typename R>
+struct alignas(2 * sizeof(R)) cpx {
+    R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> cadd(cpx<R> a, cpx<R> b) {
+    return {a.x + b.x, a.y + b.y};
+}
+// a*b + c
+template <typename R>
+__device__ __forceinline__ cpx<R> cfma(cpx<R> a, cpx<R> b, cpx<R> c) {
+    return {a.x * b.x - a.y * b.y + c.x, a.x * b.y + a.y * b.x + c.y};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> cneg(cpx<R> a) {
+    return {-a.x, -a.y};
+}
+// Im(conj(l) * z)
+template <typename R>
+__device__ __forceinline__ double im_conj_mul(cpx<R> l, cpx<R> z) {
+    return (double)l.x * (double)z.y - (double)l.y * (double)z.x;
+}
+
+// Insert a zero bit at position B of p.
+template <int B>
+__device__ __forceinline__ constexpr int ins0(int p) {
+    return ((p >> B) << (B + 1)) | (p & ((1 << B) - 1));
+}
+
+// Status plumbing (thread-local last error string).
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_status(hipError_t e, const char *where);
+
+inline int checked_launch(const char *where) {
+    return hip_status(hipGetLastError(), where);
+}
+
+constexpr int kMaxQubits = 36;
+constexpr size_t kReduceBlocks = 1024;  // partials per FP64 reduction
+
+}  // namespace svh

@@ -1,0 +1,706 @@
+// Fused tile passes: one launch applies a run of gates to every tile of 2^L
+// amplitudes (L = TB + RB).  A workgroup of 2^TB threads owns one tile; each
+// thread holds 2^RB amplitudes in registers.  A "stage" fixes which RB
+// tile-local bits live in the register index; every gate of the stage acts on
+// those bits only, so it is applied in registers with compile-time indexing.
+// Between stages the tile is re-distributed through LDS.  The first stage's
+// layout is used for the HBM load and the last one's for the store; the host
+// scheduler keeps the low tile bits on lanes there so each wave moves
+// contiguous 512-byte (c64) / 1 KiB (c128) runs.
+//
+// DUAL = true is the adjoint sweep: the same walk over ψ and λ together, with
+// 2·Im⟨λ|P|ψ⟩ accumulated in FP64 for every gate flagged SVH_OPF_GRAD before
+// U† is applied to both vectors.
+#include "svh_common.h"
+
+namespace svh {
+
+template <int RB>
+__device__ __forceinline__ int lds_swz(int li) {
+    // XOR the low 5 bits with the next 5: spreads lane groups whose tile-local
+    // bits skip the register bits across the LDS banks.  A bijection on [0, 2^L).
+    return li ^ ((li >> 5) & 31);
+}
+
+// ------------------------------------------------ in-register gate kernels
+template <typename R, int RB, int A>
+__device__ __forceinline__ void ap_mat1(cpx<R> *v, cpx<R> m0, cpx<R> m1, cpx<R> m2, cpx<R> m3) {
+#pragma unroll
+    for (int p = 0; p < (1 << (RB - 1)); ++p) {
+        const int i0 = ins0<A>(p), i1 = i0 | (1 << A);
+        const cpx<R> x0 = v[i0], x1 = v[i1];
+        v[i0] = cfma(m0, x0, cmul(m1, x1));
+        v[i1] = cfma(m2, x0, cmul(m3, x1));
+    }
+}
+
+// exp(-iθP) = c·I - i s·P
+template <typename R, int RB, int A>
+__device__ __forceinline__ void ap_rot1(cpx<R> *v, int pauli, R c, R s) {
+#pragma unroll
+    for (int p = 0; p < (1 << (RB - 1)); ++p) {
+        const int i0 = ins0<A>(p), i1 = i0 | (1 << A);
+        const cpx<R> x0 = v[i0], x1 = v[i1];
+        if (pauli == 1) {  // X: y0 = c x0 - i s x1
+            v[i0] = cpx<R>{c * x0.x + s * x1.y, c * x0.y - s * x1.x};
+            v[i1] = cpx<R>{c * x1.x + s * x0.y, c * x1.y - s * x0.x};
+        } else if (pauli == 2) {  // Y: y0 = c x0 - s x1, y1 = s x0 + c x1
+            v[i0] = cpx<R>{c * x0.x - s * x1.x, c * x0.y - s * x1.y};
+            v[i1] = cpx<R>{s * x0.x + c * x1.x, s * x0.y + c * x1.y};
+        } else {  // Z: y0 = (c - i s) x0, y1 = (c + i s) x1
+            v[i0] = cpx<R>{c * x0.x + s * x0.y, c * x0.y - s * x0.x};
+            v[i1] = cpx<R>{c * x1.x - s * x1.y, c * x1.y + s * x1.x};
+        }
+    }
+}
+
+// Σ Im(conj(λ)·(Pψ)) over this thread's pairs.
+template <typename R, int RB, int A>
+__device__ __forceinline__ double gr_rot1(const cpx<R> *v, const cpx<R> *w, int pauli) {
+    double acc = 0.0;
+#pragma unroll
+    for (int p = 0; p < (1 << (RB - 1)); ++p) {
+        const int i0 = ins0<A>(p), i1 = i0 | (1 << A);
+        const cpx<R> x0 = v[i0], x1 = v[i1], l0 = w[i0], l1 = w[i1];
+        if (pauli == 1) {
+            acc += im_conj_mul(l0, x1) + im_conj_mul(l1, x0);
+        } else if (pauli == 2) {  // (Pψ)0 = -i x1, (Pψ)1 = i x0
+            acc += im_conj_mul(l0, cpx<R>{x1.y, -x1.x}) + im_conj_mul(l1, cpx<R>{-x0.y, x0.x});
+        } else {
+            acc += im_conj_mul(l0, x0) - im_conj_mul(l1, x1);
+        }
+    }
+    return acc;
+}
+
+template <typename R, int HI, int LO>
+struct Quad {
+    static __device__ __forceinline__ int base(int q) { return ins0<HI>(ins0<LO>(q)); }
+};
+
+template <typename R, int RB, int HI, int LO>
+__device__ __forceinline__ void ap_mat2(cpx<R> *v, const cpx<R> (&m)[16]) {
+#pragma unroll
+    for (int q = 0; q < (1 << (RB - 2)); ++q) {
+        const int i00 = Quad<R, HI, LO>::base(q);
+        const int id[4] = {i00, i00 | (1 << LO), i00 | (1 << HI), i00 | (1 << HI) | (1 << LO)};
+        const cpx<R> x[4] = {v[id[0]], v[id[1]], v[id[2]], v[id[3]]};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cpx<R> acc = cmul(m[4 * r], x[0]);
+            acc = cfma(m[4 * r + 1], x[1], acc);
+            acc = cfma(m[4 * r + 2], x[2], acc);
+            acc = cfma(m[4 * r + 3], x[3], acc);
+            v[id[r]] = acc;
+        }
+    }
+}
+
+// CX: TGT_HI -> target HI, control LO; otherwise target LO, control HI.
+// (Both orientations are compile-time so every register index is constant.)
+template <typename R, int RB, int HI, int LO, bool TGT_HI>
+__device__ __forceinline__ void ap_cx_t(cpx<R> *v) {
+#pragma unroll
+    for (int q = 0; q < (1 << (RB - 2)); ++q) {
+        const int i00 = Quad<R, HI, LO>::base(q);
+        const int i11 = i00 | (1 << HI) | (1 << LO);
+        constexpr int flip = TGT_HI ? (1 << LO) : (1 << HI);
+        const int ia = i00 | flip;
+        const cpx<R> t = v[ia];
+        v[ia] = v[i11];
+        v[i11] = t;
+    }
+}
+
+template <typename R, int RB, int HI, int LO>
+__device__ __forceinline__ void ap_cz(cpx<R> *v) {
+#pragma unroll
+    for (int q = 0; q < (1 << (RB - 2)); ++q) {
+        const int i11 = Quad<R, HI, LO>::base(q) | (1 << HI) | (1 << LO);
+        v[i11] = cneg(v[i11]);
+    }
+}
+
+// P⊗P acting on a quad: returns (P⊗P x) in y.
+template <typename R>
+__device__ __forceinline__ void pp_apply(int pauli, const cpx<R> (&x)[4], cpx<R> (&y)[4]) {
+    if (pauli == 1) {  // XX: |b⟩ -> |~b⟩
+        y[0] = x[3]; y[1] = x[2]; y[2] = x[1]; y[3] = x[0];
+    } else if (pauli == 2) {  // YY: (YYx)_00 = -x_11, _11 = -x_00, _01 = x_10, _10 = x_01
+        y[0] = cneg(x[3]); y[1] = x[2]; y[2] = x[1]; y[3] = cneg(x[0]);
+    } else {  // ZZ: diag(1,-1,-1,1)
+        y[0] = x[0]; y[1] = cneg(x[1]); y[2] = cneg(x[2]); y[3] = x[3];
+    }
+}
+
+template <typename R, int RB, int HI, int LO>
+__device__ __forceinline__ void ap_rot2(cpx<R> *v, int pauli, R c, R s) {
+#pragma unroll
+    for (int q = 0; q < (1 << (RB - 2)); ++q) {
+        const int i00 = Quad<R, HI, LO>::base(q);
+        const int id[4] = {i00, i00 | (1 << LO), i00 | (1 << HI), i00 | (1 << HI) | (1 << LO)};
+        const cpx<R> x[4] = {v[id[0]], v[id[1]], v[id[2]], v[id[3]]};
+        cpx<R> y[4];
+        pp_apply<R>(pauli, x, y);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)  // c x - i s y
+            v[id[r]] = cpx<R>{c * x[r].x + s * y[r].y, c * x[r].y - s * y[r].x};
+    }
+}
+
+template <typename R, int RB, int HI, int LO>
+__device__ __forceinline__ double gr_rot2(const cpx<R> *v, const cpx<R> *w, int pauli) {
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < (1 << (RB - 2)); ++q) {
+        const int i00 = Quad<R, HI, LO>::base(q);
+        const int id[4] = {i00, i00 | (1 << LO), i00 | (1 << HI), i00 | (1 << HI) | (1 << LO)};
+        const cpx<R> x[4] = {v[id[0]], v[id[1]], v[id[2]], v[id[3]]};
+        cpx<R> y[4];
+        pp_apply<R>(pauli, x, y);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc += im_conj_mul(w[id[r]], y[r]);
+    }
+    return acc;
+}
+
+// ------------------------------------------------ compile-time dispatch
+// Register-bit indices arrive at run time (uniform across the workgroup); each
+// switch arm instantiates the gate with constant indices so v[] stays in VGPRs.
+#define SVH_SWITCH1(a, CALL)                                   \
+    switch (a) {                                               \
+        case 0: CALL(0); break;                                \
+        case 1: if constexpr (RB > 1) { CALL(1); } break;      \
+        case 2: if constexpr (RB > 2) { CALL(2); } break;      \
+        case 3: if constexpr (RB > 3) { CALL(3); } break;      \
+        case 4: if constexpr (RB > 4) { CALL(4); } break;      \
+        default: break;                                        \
+    }
+#define SVH_C2(H, L, CALL) \
+    case (H)*8 + (L): if constexpr (RB > (H)) { CALL(H, L); } break;
+#define SVH_SWITCH2(hi, lo, CALL)                                                   \
+    switch ((hi)*8 + (lo)) {                                                        \
+        SVH_C2(1, 0, CALL)                                                          \
+        SVH_C2(2, 0, CALL) SVH_C2(2, 1, CALL)                                       \
+        SVH_C2(3, 0, CALL) SVH_C2(3, 1, CALL) SVH_C2(3, 2, CALL)                    \
+        SVH_C2(4, 0, CALL) SVH_C2(4, 1, CALL) SVH_C2(4, 2, CALL) SVH_C2(4, 3, CALL) \
+        default: break;                                                             \
+    }
+
+// Sum over the lanes of one wave; W = active lanes (64, or the whole
+// workgroup when it is smaller than a wave).
+template <int W>
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, W);
+    return v;
+}
+
+template <typename R>
+__device__ __forceinline__ cpx<R> ld_m(const double *m, int i) {
+    return cpx<R>{(R)m[2 * i], (R)m[2 * i + 1]};
+}
+
+template <typename R>
+__device__ __forceinline__ void op_sincos(const double *theta, const svh_op_t &op, R &cs, R &sn) {
+    const R th = (R)theta[op.param];
+    if constexpr (std::is_same<R, float>::value) sincosf(th, &sn, &cs);
+    else sincos(th, &sn, &cs);
+    if (op.flags & SVH_OPF_NEG) sn = -sn;
+}
+
+// Apply one op to the registers (v: ψ, w: λ when DUAL).  Returns the gradient
+// partial (this thread's Σ Im(conj λ · Pψ)) through *g when the op asks for it.
+template <typename R, int RB, bool DUAL>
+__device__ __forceinline__ void apply_op(cpx<R> (&v)[1 << RB], cpx<R> (&w)[DUAL ? (1 << RB) : 1],
+                                         const svh_op_t &op, const double *theta, double *g,
+                                         bool *has_g) {
+    const int kind = op.kind;
+    if (kind == SVH_OP_MAT1) {
+        const cpx<R> m0 = ld_m<R>(op.m, 0), m1 = ld_m<R>(op.m, 1), m2 = ld_m<R>(op.m, 2),
+                     m3 = ld_m<R>(op.m, 3);
+#define CALL_M1(A)                                      \
+    ap_mat1<R, RB, A>(v, m0, m1, m2, m3);               \
+    if constexpr (DUAL) ap_mat1<R, RB, A>(w, m0, m1, m2, m3);
+        SVH_SWITCH1(op.a, CALL_M1)
+#undef CALL_M1
+    } else if (kind == SVH_OP_ROT1) {
+        R cs, sn;
+        op_sincos<R>(theta, op, cs, sn);
+        const int pauli = op.pauli;
+        if constexpr (DUAL) {
+            if (op.flags & SVH_OPF_GRAD) {
+                *has_g = true;
+#define CALL_G1(A) *g = gr_rot1<R, RB, A>(v, w, pauli);
+                SVH_SWITCH1(op.a, CALL_G1)
+#undef CALL_G1
+            }
+        }
+#define CALL_R1(A)                                   \
+    ap_rot1<R, RB, A>(v, pauli, cs, sn);             \
+    if constexpr (DUAL) ap_rot1<R, RB, A>(w, pauli, cs, sn);
+        SVH_SWITCH1(op.a, CALL_R1)
+#undef CALL_R1
+    } else if (kind == SVH_OP_CX) {
+        const int hi = op.a > op.b ? op.a : op.b, lo = op.a > op.b ? op.b : op.a;
+        if (op.a > op.b) {
+#define CALL_CXH(H, L)                               \
+    ap_cx_t<R, RB, H, L, true>(v);                   \
+    if constexpr (DUAL) ap_cx_t<R, RB, H, L, true>(w);
+            SVH_SWITCH2(hi, lo, CALL_CXH)
+#undef CALL_CXH
+        } else {
+#define CALL_CXL(H, L)                               \
+    ap_cx_t<R, RB, H, L, false>(v);                  \
+    if constexpr (DUAL) ap_cx_t<R, RB, H, L, false>(w);
+            SVH_SWITCH2(hi, lo, CALL_CXL)
+#undef CALL_CXL
+        }
+    } else if (kind == SVH_OP_CZ) {
+        const int hi = op.a > op.b ? op.a : op.b, lo = op.a > op.b ? op.b : op.a;
+#define CALL_CZ(H, L)                 \
+    ap_cz<R, RB, H, L>(v);            \
+    if constexpr (DUAL) ap_cz<R, RB, H, L>(w);
+        SVH_SWITCH2(hi, lo, CALL_CZ)
+#undef CALL_CZ
+    } else if (kind == SVH_OP_MAT2) {
+        cpx<R> m[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) m[i] = ld_m<R>(op.m, i);
+#define CALL_M2(H, L)                    \
+    ap_mat2<R, RB, H, L>(v, m);          \
+    if constexpr (DUAL) ap_mat2<R, RB, H, L>(w, m);
+        SVH_SWITCH2(op.a, op.b, CALL_M2)
+#undef CALL_M2
+    } else if (kind == SVH_OP_ROT2) {
+        R cs, sn;
+        op_sincos<R>(theta, op, cs, sn);
+        const int pauli = op.pauli;
+        if constexpr (DUAL) {
+            if (op.flags & SVH_OPF_GRAD) {
+                *has_g = true;
+#define CALL_G2(H, L) *g = gr_rot2<R, RB, H, L>(v, w, pauli);
+                SVH_SWITCH2(op.a, op.b, CALL_G2)
+#undef CALL_G2
+            }
+        }
+#define CALL_R2(H, L)                             \
+    ap_rot2<R, RB, H, L>(v, pauli, cs, sn);       \
+    if constexpr (DUAL) ap_rot2<R, RB, H, L>(w, pauli, cs, sn);
+        SVH_SWITCH2(op.a, op.b, CALL_R2)
+#undef CALL_R2
+    }
+}
+
+// Tile-local index of this thread's register 0 in a layout whose register
+// bits are rb[0..RB): thread bits fill the remaining tile bits in order.
+template <int L, int RB>
+__device__ __forceinline__ int thr_local(const svh_stage_t &S, int tid) {
+    int rmask = 0;
+#pragma unroll
+    for (int k = 0; k < RB; ++k) rmask |= 1 << S.rbits[k];
+    int li = 0, t = tid;
+#pragma unroll
+    for (int p = 0; p < L; ++p) {
+        if (!((rmask >> p) & 1)) {
+            li |= (t & 1) << p;
+            t >>= 1;
+        }
+    }
+    return li;
+}
+
+template <int RB>
+__device__ __forceinline__ int reg_local(const svh_stage_t &S, int j) {
+    int rl = 0;
+#pragma unroll
+    for (int k = 0; k < RB; ++k) rl |= ((j >> k) & 1) << S.rbits[k];
+    return rl;
+}
+
+template <int L>
+__device__ __forceinline__ uint64_t to_global(const int (&gbit)[L], int li) {
+    uint64_t a = 0;
+#pragma unroll
+    for (int p = 0; p < L; ++p) a |= (uint64_t)((li >> p) & 1) << gbit[p];
+    return a;
+}
+
+// ------------------------------------------------ the pass kernel
+template <typename R, int TB, int RB, bool DUAL>
+__global__ void __launch_bounds__(1 << TB)
+k_tile(cpx<R> *__restrict__ psi, cpx<R> *__restrict__ lam, int n,
+       const svh_pass_t *__restrict__ pass, const svh_stage_t *__restrict__ stages,
+       const svh_op_t *__restrict__ ops, const double *__restrict__ theta,
+       double *__restrict__ gpart) {
+    constexpr int L = TB + RB;
+    constexpr int NR = 1 << RB;
+    constexpr int NW = ((1 << TB) + 63) / 64;  // waves per workgroup
+    extern __shared__ unsigned char smem_raw[];
+    cpx<R> *lds = reinterpret_cast<cpx<R> *>(smem_raw);
+    cpx<R> *lds2 = lds + (1 << L);
+    double *gacc = reinterpret_cast<double *>(smem_raw + (DUAL ? 2 : 1) * sizeof(cpx<R>) * (1 << L));
+
+    const int tid = threadIdx.x;
+    const int n_stages = pass->n_stages;
+    const int s0 = pass->stage_begin;
+    const int ng = pass->n_grad;
+    int gbit[L];
+#pragma unroll
+    for (int i = 0; i < L; ++i) gbit[i] = pass->bits[i];
+
+    // Tile base address: workgroup id deposited into the non-tile bits.
+    uint64_t tmask = 0;
+#pragma unroll
+    for (int i = 0; i < L; ++i) tmask |= 1ull << gbit[i];
+    uint64_t base = 0;
+    {
+        uint64_t g = blockIdx.x;
+        for (int b = 0; b < n; ++b) {
+            if (!((tmask >> b) & 1)) {
+                base |= (g & 1ull) << b;
+                g >>= 1;
+            }
+        }
+    }
+
+    cpx<R> v[NR];
+    cpx<R> w[DUAL ? NR : 1];
+
+    // ---- load in stage 0's layout
+    {
+        const svh_stage_t &S = stages[s0];
+        const uint64_t tg = base | to_global<L>(gbit, thr_local<L, RB>(S, tid));
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            const uint64_t a = tg | to_global<L>(gbit, reg_local<RB>(S, j));
+            v[j] = psi[a];
+            if constexpr (DUAL) w[j] = lam[a];
+        }
+    }
+
+    for (int s = 0; s < n_stages; ++s) {
+        const svh_stage_t &S = stages[s0 + s];
+        if (s > 0) {
+            // ---- re-distribute the tile from stage s-1's layout to stage s's
+            const svh_stage_t &P = stages[s0 + s - 1];
+            const int tp = thr_local<L, RB>(P, tid);
+            if (s > 1) __syncthreads();  // previous exchange's reads are done
+#pragma unroll
+            for (int j = 0; j < NR; ++j) {
+                const int li = lds_swz<RB>(tp | reg_local<RB>(P, j));
+                lds[li] = v[j];
+                if constexpr (DUAL) lds2[li] = w[j];
+            }
+            __syncthreads();
+            const int tn = thr_local<L, RB>(S, tid);
+#pragma unroll
+            for (int j = 0; j < NR; ++j) {
+                const int li = lds_swz<RB>(tn | reg_local<RB>(S, j));
+                v[j] = lds[li];
+                if constexpr (DUAL) w[j] = lds2[li];
+            }
+        }
+        // ---- gates of this stage, in registers
+        for (int o = S.op_begin; o < S.op_end; ++o) {
+            const svh_op_t &op = ops[o];
+            double g = 0.0;
+            bool has_g = false;
+            apply_op<R, RB, DUAL>(v, w, op, theta, &g, &has_g);
+            if constexpr (DUAL) {
+                if (has_g) {
+                    g = wave_sum<((1 << TB) < 64 ? (1 << TB) : 64)>(g);
+                    if ((tid & 63) == 0) gacc[(tid >> 6) * ng + op.grad_slot] = 2.0 * g;
+                }
+            }
+        }
+    }
+
+    // ---- store in the last stage's layout
+    {
+        const svh_stage_t &S = stages[s0 + n_stages - 1];
+        const uint64_t tg = base | to_global<L>(gbit, thr_local<L, RB>(S, tid));
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            const uint64_t a = tg | to_global<L>(gbit, reg_local<RB>(S, j));
+            psi[a] = v[j];
+            if constexpr (DUAL) lam[a] = w[j];
+        }
+    }
+
+    if constexpr (DUAL) {
+        if (ng > 0) {
+            __syncthreads();
+            for (int sl = tid; sl < ng; sl += (1 << TB)) {
+                double acc = 0.0;
+                for (int wv = 0; wv < NW; ++wv) acc += gacc[wv * ng + sl];
+                gpart[(uint64_t)blockIdx.x * ng + sl] = acc;
+            }
+        }
+    }
+}
+
+// Sum the per-workgroup partials of one pass (fixed order) into grad[param].
+__global__ void __launch_bounds__(256)
+k_grad_reduce(const double *__restrict__ gpart, int nblocks, int ng, const int32_t *__restrict__ slot_param,
+              double *__restrict__ grad) {
+    const int sl = blockIdx.x;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) acc += gpart[(uint64_t)b * ng + sl];
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(&grad[slot_param[sl]], red[0]);
+}
+
+// ------------------------------------------------ host launch
+constexpr size_t kMaxLds = 160 * 1024;
+
+template <typename R, int TB, bool DUAL>
+static int launch_tb(cpx<R> *psi, cpx<R> *lam, int n, const svh_pass_t &hp, const svh_pass_t *dp,
+                     const svh_stage_t *st, const svh_op_t *op, const double *theta, double *gpart,
+                     hipStream_t stream) {
+    constexpr int RB = 4;
+    constexpr int L = TB + RB;
+    constexpr int NW = ((1 << TB) + 63) / 64;
+    const size_t lds = (DUAL ? 2 : 1) * sizeof(cpx<R>) * ((size_t)1 << L) +
+                       (DUAL ? sizeof(double) * NW * (size_t)hp.n_grad : 0);
+    if (lds > kMaxLds) return fail(-7, "tile pass needs " + std::to_string(lds) + " B of LDS (> 160 KiB)");
+    auto kern = k_tile<R, TB, RB, DUAL>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)kMaxLds);
+        if (e != hipSuccess) return hip_status(e, "hipFuncSetAttribute(k_tile)");
+        attr_set = true;
+    }
+    const uint64_t grid = 1ull << (n - L);
+    if (grid > 0x7fffffffull) return fail(-7, "grid too large");
+    kern<<<(unsigned)grid, 1 << TB, lds, stream>>>(psi, lam, n, dp, st, op, theta, gpart);
+    return checked_launch("k_tile");
+}
+
+// Instantiated (TB, dtype, DUAL) combinations: the ones that fit in registers
+// without spilling (c64: TB <= 10 forward, <= 9 adjoint; c128: TB <= 9 forward,
+// <= 8 adjoint).
+template <typename R, bool DUAL>
+static int launch_tile(cpx<R> *psi, cpx<R> *lam, int n, const svh_pass_t &hp, const svh_pass_t *dp,
+                       const svh_stage_t *st, const svh_op_t *op, const double *theta, double *gpart,
+                       hipStream_t stream) {
+    if (hp.rb != 4) return fail(-7, "only 4 register bits are built");
+    constexpr bool C64 = std::is_same<R, float>::value;
+    constexpr int tb_max = DUAL ? (C64 ? 9 : 8) : (C64 ? 10 : 9);
+    constexpr int tb_min = 4;
+    if (hp.tb < tb_min || hp.tb > tb_max)
+        return fail(-7, "tile pass tb=" + std::to_string(hp.tb) + " outside [" + std::to_string(tb_min) + ", " +
+                            std::to_string(tb_max) + "] for this dtype/mode");
+    switch (hp.tb) {
+        case 4: return launch_tb<R, 4, DUAL>(psi, lam, n, hp, dp, st, op, theta, gpart, stream);
+        case 5: return launch_tb<R, 5, DUAL>(psi, lam, n, hp, dp, st, op, theta, gpart, stream);
+        case 6: return launch_tb<R, 6, DUAL>(psi, lam, n, hp, dp, st, op, theta, gpart, stream);
+        case 7: return launch_tb<R, 7, DUAL>(psi, lam, n, hp, dp, st, op, theta, gpart, stream);
+        case 8: return launch_tb<R, 8, DUAL>(psi, lam, n, hp, dp, st, op, theta, gpart, stream);
+        case 9: if constexpr (tb_max >= 9) return launch_tb<R, 9, DUAL>(psi, lam, n, hp, dp, st, op, theta, gpart, stream); break;
+        case 10: if constexpr (tb_max >= 10) return launch_tb<R, 10, DUAL>(psi, lam, n, hp, dp, st, op, theta, gpart, stream); break;
+        default: break;
+    }
+    return fail(-7, "tile pass tb not built");
+}
+
+// Validate a program's host copy against n; returns 0 or an error.
+static int check_program(const void *prog_host, int n, const svh_prog_header_t **hdr) {
+    if (!prog_host) return fail(-1, "program pointer is NULL");
+    const auto *h = static_cast<const svh_prog_header_t *>(prog_host);
+    if (h->magic != SVH_PROG_MAGIC) return fail(-8, "bad program magic");
+    if (h->version != SVH_ABI_VERSION) return fail(-8, "program version mismatch");
+    if (n >= 0 && h->n_qubits != n) return fail(-8, "program built for a different n");
+    const char *b = static_cast<const char *>(prog_host);
+    const auto *passes = reinterpret_cast<const svh_pass_t *>(b + h->off_passes);
+    const auto *stages = reinterpret_cast<const svh_stage_t *>(b + h->off_stages);
+    const auto *ops = reinterpret_cast<const svh_op_t *>(b + h->off_ops);
+    for (int p = 0; p < h->n_passes; ++p) {
+        const svh_pass_t &P = passes[p];
+        if (P.kind == SVH_PASS_TILE) {
+            if (P.L != P.tb + P.rb || P.L > h->n_qubits || P.L > SVH_MAX_TILE_BITS)
+                return fail(-8, "tile pass " + std::to_string(p) + ": bad L");
+            if (P.n_stages < 1 || P.stage_begin < 0 || P.stage_begin + P.n_stages > h->n_stages)
+                return fail(-8, "tile pass " + std::to_string(p) + ": bad stage range");
+            uint64_t m = 0;
+            for (int i = 0; i < P.L; ++i) {
+                if (P.bits[i] < 0 || P.bits[i] >= h->n_qubits || ((m >> P.bits[i]) & 1))
+                    return fail(-8, "tile pass " + std::to_string(p) + ": bad tile bits");
+                m |= 1ull << P.bits[i];
+            }
+            for (int s = P.stage_begin; s < P.stage_begin + P.n_stages; ++s) {
+                const svh_stage_t &S = stages[s];
+                int rm = 0;
+                for (int k = 0; k < P.rb; ++k) {
+                    if (S.rbits[k] < 0 || S.rbits[k] >= P.L || ((rm >> S.rbits[k]) & 1))
+                        return fail(-8, "stage " + std::to_string(s) + ": bad register bits");
+                    rm |= 1 << S.rbits[k];
+                }
+                if (S.op_begin < 0 || S.op_end < S.op_begin || S.op_end > h->n_ops)
+                    return fail(-8, "stage " + std::to_string(s) + ": bad op range");
+                for (int o = S.op_begin; o < S.op_end; ++o) {
+                    const svh_op_t &O = ops[o];
+                    const bool two = O.kind == SVH_OP_MAT2 || O.kind == SVH_OP_CX ||
+                                     O.kind == SVH_OP_CZ || O.kind == SVH_OP_ROT2;
+                    if (O.kind < SVH_OP_MAT1 || O.kind > SVH_OP_ROT2)
+                        return fail(-8, "op " + std::to_string(o) + ": bad kind");
+                    if (O.a < 0 || O.a >= P.rb || (two && (O.b < 0 || O.b >= P.rb || O.b == O.a)))
+                        return fail(-8, "op " + std::to_string(o) + ": bad register bit");
+                    if ((O.kind == SVH_OP_MAT2 || O.kind == SVH_OP_ROT2) && O.a < O.b)
+                        return fail(-8, "op " + std::to_string(o) + ": a must exceed b");
+                    if ((O.kind == SVH_OP_ROT1 || O.kind == SVH_OP_ROT2) && (O.pauli < 1 || O.pauli > 3))
+                        return fail(-8, "op " + std::to_string(o) + ": bad pauli");
+                    if ((O.flags & SVH_OPF_GRAD) && (O.grad_slot < 0 || O.grad_slot >= P.n_grad))
+                        return fail(-8, "op " + std::to_string(o) + ": bad grad slot");
+                }
+            }
+        } else if (P.kind == SVH_PASS_DENSE) {
+            if (P.L < 1 || P.L > 6 || P.L > h->n_qubits) return fail(-8, "dense pass: bad k");
+            if (P.pool_off < 0 || P.pool_off + 2 * (1ll << (2 * P.L)) > h->n_pool)
+                return fail(-8, "dense pass: matrix outside pool");
+        } else if (P.kind != SVH_PASS_NORMALIZE) {
+            return fail(-8, "bad pass kind");
+        }
+    }
+    *hdr = h;
+    return 0;
+}
+
+// Dense passes take their matrix from the (double) pool; c64 needs a float copy.
+template <typename R>
+__global__ void k_pool_to(const double *__restrict__ src, cpx<R> *__restrict__ dst, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = cpx<R>{(R)src[2 * i], (R)src[2 * i + 1]};
+}
+
+}  // namespace svh
+
+using namespace svh;
+
+extern "C" int svh_apply_dense(void *psi, int n, int dtype, const int *bits, int k, const void *mat_dev,
+                               void *stream);
+extern "C" int svh_norm2(const void *psi, int n, int dtype, double *out_dev, void *scratch,
+                         size_t scratch_bytes, void *stream);
+extern "C" int svh_scale_by_inv_norm(void *psi, int n, int dtype, const double *norm2_dev, void *stream);
+
+static size_t scratch_need(const svh_prog_header_t *h, int dtype, bool dual) {
+    const char *b = reinterpret_cast<const char *>(h);
+    const auto *passes = reinterpret_cast<const svh_pass_t *>(b + h->off_passes);
+    size_t need = (kReduceBlocks + 8) * sizeof(double);  // norm partials + result
+    const size_t es = dtype == SVH_C64 ? 8 : 16;
+    for (int p = 0; p < h->n_passes; ++p) {
+        const svh_pass_t &P = passes[p];
+        size_t s = (kReduceBlocks + 8) * sizeof(double);
+        if (P.kind == SVH_PASS_DENSE) s += es * ((size_t)1 << (2 * P.L));
+        if (P.kind == SVH_PASS_TILE && dual && P.n_grad > 0)
+            s = sizeof(double) * (((size_t)1 << (h->n_qubits - P.L)) * P.n_grad);
+        if (s > need) need = s;
+    }
+    return (need + 255) & ~(size_t)255;
+}
+
+static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_host, const void *prog_dev,
+                    const double *theta_dev, double *grad_dev, void *scratch, size_t scratch_bytes,
+                    void *stream, bool dual) {
+    if (!psi || (dual && (!lam || !grad_dev))) return fail(-1, "state/lambda/grad pointer is NULL");
+    if (n < 1 || n > kMaxQubits) return fail(-2, "n out of range");
+    if (dtype != SVH_C64 && dtype != SVH_C128) return fail(-3, "bad dtype");
+    if (!prog_dev) return fail(-1, "device program pointer is NULL");
+    const svh_prog_header_t *h = nullptr;
+    if (int e = check_program(prog_host, n, &h)) return e;
+    const size_t need = scratch_need(h, dtype, dual);
+    if (!scratch || scratch_bytes < need)
+        return fail(-5, "scratch too small: need " + std::to_string(need) + " bytes");
+    const char *hb = static_cast<const char *>(prog_host);
+    const char *db = static_cast<const char *>(prog_dev);
+    const auto *hpass = reinterpret_cast<const svh_pass_t *>(hb + h->off_passes);
+    const auto *dpass = reinterpret_cast<const svh_pass_t *>(db + h->off_passes);
+    const auto *dstage = reinterpret_cast<const svh_stage_t *>(db + h->off_stages);
+    const auto *dop = reinterpret_cast<const svh_op_t *>(db + h->off_ops);
+    const auto *dgrad = reinterpret_cast<const int32_t *>(db + h->off_grad);
+    const auto *dpool = reinterpret_cast<const double *>(db + h->off_pool);
+    hipStream_t st = (hipStream_t)stream;
+    double *sd = static_cast<double *>(scratch);
+
+    // theta must be present when any op reads it
+    if (!theta_dev) {
+        const auto *hop = reinterpret_cast<const svh_op_t *>(hb + h->off_ops);
+        for (int o = 0; o < h->n_ops; ++o)
+            if (hop[o].kind == SVH_OP_ROT1 || hop[o].kind == SVH_OP_ROT2)
+                return fail(-1, "program has rotation ops but theta is NULL");
+    }
+
+    for (int p = 0; p < h->n_passes; ++p) {
+        const svh_pass_t &P = hpass[p];
+        int e = 0;
+        if (P.kind == SVH_PASS_TILE) {
+            if (dual && P.n_grad > 0) {
+                // partials sized by check above
+            }
+            if (dtype == SVH_C64)
+                e = dual ? launch_tile<float, true>((cpx<float> *)psi, (cpx<float> *)lam, n, P, dpass + p, dstage,
+                                                    dop, theta_dev, sd, st)
+                         : launch_tile<float, false>((cpx<float> *)psi, nullptr, n, P, dpass + p, dstage, dop,
+                                                     theta_dev, sd, st);
+            else
+                e = dual ? launch_tile<double, true>((cpx<double> *)psi, (cpx<double> *)lam, n, P, dpass + p,
+                                                     dstage, dop, theta_dev, sd, st)
+                         : launch_tile<double, false>((cpx<double> *)psi, nullptr, n, P, dpass + p, dstage, dop,
+                                                      theta_dev, sd, st);
+            if (!e && dual && P.n_grad > 0) {
+                const int nb = (int)(1ull << (n - P.L));
+                k_grad_reduce<<<P.n_grad, 256, 0, st>>>(sd, nb, P.n_grad, dgrad + P.grad_begin, grad_dev);
+                e = checked_launch("k_grad_reduce");
+            }
+        } else if (P.kind == SVH_PASS_DENSE) {
+            if (dual) return fail(-8, "dense passes are not differentiable in the adjoint sweep");
+            const int K = 1 << P.L;
+            void *mat = sd + kReduceBlocks + 8;
+            if (dtype == SVH_C64)
+                k_pool_to<float><<<(K * K + 255) / 256, 256, 0, st>>>(dpool + P.pool_off, (cpx<float> *)mat, K * K);
+            else
+                k_pool_to<double><<<(K * K + 255) / 256, 256, 0, st>>>(dpool + P.pool_off, (cpx<double> *)mat,
+                                                                       K * K);
+            e = checked_launch("k_pool_to");
+            if (!e) e = svh_apply_dense(psi, n, dtype, P.bits, P.L, mat, stream);
+        } else if (P.kind == SVH_PASS_NORMALIZE) {
+            if (dual) return fail(-8, "normalisation is not differentiable in the adjoint sweep");
+            e = svh_norm2(psi, n, dtype, sd + kReduceBlocks, sd, kReduceBlocks * sizeof(double), stream);
+            if (!e) e = svh_scale_by_inv_norm(psi, n, dtype, sd + kReduceBlocks, stream);
+        }
+        if (e) return e;
+    }
+    return 0;
+}
+
+extern "C" {
+
+int svh_program_scratch_bytes(const void *prog_host, int dtype, size_t *bytes) {
+    if (!bytes) return fail(-1, "bytes pointer is NULL");
+    const svh_prog_header_t *h = nullptr;
+    if (int e = check_program(prog_host, -1, &h)) return e;
+    *bytes = scratch_need(h, dtype, true);
+    return 0;
+}
+
+int svh_run_program(void *psi, int n, int dtype, const void *prog_host, const void *prog_dev,
+                    const double *theta_dev, void *scratch, size_t scratch_bytes, void *stream) {
+    return run_impl(psi, nullptr, n, dtype, prog_host, prog_dev, theta_dev, nullptr, scratch, scratch_bytes,
+                    stream, false);
+}
+
+int svh_run_adjoint(void *psi, void *lam, int n, int dtype, const void *prog_host, const void *prog_dev,
+                    const double *theta_dev, double *grad_dev, void *scratch, size_t scratch_bytes,
+                    void *stream) {
+    return run_impl(psi, lam, n, dtype, prog_host, prog_dev, theta_dev, grad_dev, scratch, scratch_bytes,
+                    stream, true);
+}
+
+}  // extern "C"

@@ -1,0 +1,503 @@
+"""Circuit compiler: qaravan gate lists -> gate programs for the HIP kernels.
+
+The reference applies every gate as its own tensor contraction, one Python call
+per gate (src/qaravan/core/base_sim.py:40-47 -> tensorQ/statevector_sim.py:
+124-140).  Here the whole layered circuit is lowered once into a program that
+the library runs as a handful of fused passes:
+
+1. Lowering.  Each gate becomes an elementary op on ADDRESS bits (qaravan qubit
+   q is bit n-1-q, because the state is a C-order rank-n tensor, i.e.
+   big-endian).  2-qubit matrices equal to a CNOT (either orientation) or to CZ
+   become permutation / sign ops; other 1- and 2-qubit matrices stay dense;
+   3..6-qubit gates become stand-alone dense passes; ``Reset`` becomes its
+   projector followed by a renormalisation pass (base_sim.py:25-28).
+2. 1-qubit fusion.  Runs of dense 1-qubit ops on the same bit (with no other op
+   on that bit in between) are multiplied into one 2x2.
+3. Pass partition.  Ops are packed greedily, in order, into tile passes: a pass
+   owns L address bits (always the lowest ``low_bits`` ones, so every tile
+   load is a run of contiguous amplitudes) and takes every op whose bits fit,
+   stopping at an ALU-cost cap.  An op that does not fit blocks its bits for
+   the rest of the pass, so the order of overlapping gates is preserved.
+4. Stages.  Inside a pass, ops are grouped into stages of 4 register bits;
+   each stage change is one LDS re-distribution in the kernel.
+
+The byte layout matches include/qaravan_hip.h (svh_prog_header_t, svh_pass_t,
+svh_stage_t, svh_op_t).
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+MAGIC = 0x53564850
+ABI_VERSION = 1
+PASS_TILE, PASS_DENSE, PASS_NORMALIZE = 1, 2, 3
+OP_MAT1, OP_MAT2, OP_CX, OP_CZ, OP_ROT1, OP_ROT2 = 1, 2, 3, 4, 5, 6
+OPF_NEG, OPF_GRAD = 1, 2
+RB = 4
+MAX_TILE_BITS = 16
+
+HDR_DT = np.dtype([('magic', '<u4'), ('version', '<i4'), ('n_qubits', '<i4'), ('n_passes', '<i4'),
+                   ('n_stages', '<i4'), ('n_ops', '<i4'), ('n_grad_slots', '<i4'), ('n_pool', '<i4'),
+                   ('off_passes', '<i8'), ('off_stages', '<i8'), ('off_ops', '<i8'),
+                   ('off_grad', '<i8'), ('off_pool', '<i8'), ('reserved', '<i8', (3,))])
+PASS_DT = np.dtype([('kind', '<i4'), ('L', '<i4'), ('tb', '<i4'), ('rb', '<i4'),
+                    ('stage_begin', '<i4'), ('n_stages', '<i4'), ('grad_begin', '<i4'),
+                    ('n_grad', '<i4'), ('pool_off', '<i8'), ('bits', '<i4', (MAX_TILE_BITS,))])
+STAGE_DT = np.dtype([('rbits', '<i4', (8,)), ('op_begin', '<i4'), ('op_end', '<i4')])
+OP_DT = np.dtype([('kind', '<i4'), ('a', '<i4'), ('b', '<i4'), ('param', '<i4'), ('pauli', '<i4'),
+                  ('flags', '<i4'), ('grad_slot', '<i4'), ('pad', '<i4'), ('m', '<f8', (32,))])
+assert HDR_DT.itemsize == 96 and PASS_DT.itemsize == 104
+assert STAGE_DT.itemsize == 40 and OP_DT.itemsize == 288
+
+PAULI_CODE = {'x': 1, 'y': 2, 'z': 3}
+
+# per-amplitude VALU cost of an op inside a tile pass (used for the pass cap)
+OP_COST = {'mat1': 8.0, 'rot1': 4.0, 'mat2': 16.0, 'cx': 0.5, 'cz': 0.25, 'rot2': 8.0}
+
+_CX_TC = np.array([[1, 0, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0], [0, 1, 0, 0]])  # target = row MSB
+_CX_CT = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0]])  # control = row MSB
+_CZ = np.diag([1, 1, 1, -1])
+
+
+@dataclass
+class EOp:
+    """Elementary op on address bits.
+
+    kind: 'mat1' | 'mat2' | 'cx' | 'cz' | 'rot1' | 'rot2' | 'dense' | 'norm'
+    bits: address bits; mat2/dense: bits[0] is the row MSB; cx: (target, control)
+    """
+    kind: str
+    bits: tuple
+    mat: np.ndarray = None
+    param: int = -1
+    pauli: int = 0
+    flags: int = 0
+
+    def dagger(self):
+        if self.kind in ('mat1', 'mat2', 'dense'):
+            return EOp(self.kind, self.bits, self.mat.conj().T.copy(), self.param, self.pauli, self.flags)
+        if self.kind in ('rot1', 'rot2'):
+            return EOp(self.kind, self.bits, None, self.param, self.pauli, self.flags ^ OPF_NEG)
+        if self.kind in ('cx', 'cz'):
+            return self
+        raise ValueError(f"{self.kind} has no adjoint")
+
+
+# ----------------------------------------------------------------- lowering
+def _as_np(m):
+    try:
+        import torch
+        if torch.is_tensor(m):
+            m = m.detach().cpu().numpy()
+    except ImportError:  # pragma: no cover
+        pass
+    return np.asarray(m, dtype=np.complex128)
+
+
+def lower_matrix_gate(mat, qubits, n, name=None):
+    """Ops for a gate given as a matrix over ``qubits`` (first = row MSB)."""
+    mat = _as_np(mat)
+    k = len(qubits)
+    if mat.shape != (2 ** k, 2 ** k):
+        raise ValueError(f"gate matrix shape {mat.shape} does not match {k} qubit(s)")
+    if len(set(qubits)) != k:
+        raise ValueError(f"repeated qubit in {qubits}")
+    for q in qubits:
+        if not 0 <= q < n:
+            raise ValueError(f"qubit {q} outside a {n}-qubit register")
+    bits = tuple(n - 1 - q for q in qubits)
+    out = []
+    if k == 1:
+        out.append(EOp('mat1', bits, mat))
+    elif k == 2:
+        if np.array_equal(mat, _CX_TC):
+            out.append(EOp('cx', (bits[0], bits[1])))
+        elif np.array_equal(mat, _CX_CT):
+            out.append(EOp('cx', (bits[1], bits[0])))
+        elif np.array_equal(mat, _CZ):
+            out.append(EOp('cz', bits))
+        else:
+            out.append(EOp('mat2', bits, mat))
+    elif k <= 6:
+        out.append(EOp('dense', bits, mat))
+    else:
+        raise NotImplementedError("gates on more than 6 qubits are not supported")
+    if name == "Reset":
+        pass  # normalisation appended by the caller when requested
+    return out
+
+
+def lower_circuit(circ, n=None, param_mode=False):
+    """Lower a built qaravan ``Circuit`` (greedy layers, circuits.py:42-63) to
+    EOps in layered order.  With ``param_mode``, rotations RX/RY/RZ/RXX/RYY/RZZ
+    whose angle is given by ``gate.param_index`` become device-θ ops."""
+    n = circ.num_sites if n is None else n
+    if circ.local_dim != 2:
+        raise NotImplementedError("the HIP statevector path supports qubits (local_dim = 2) only")
+    layers = circ.layers if circ.layers is not None else circ.build().layers
+    ops = []
+    for layer in layers:
+        for g in layer:
+            ops.extend(lower_gate(g, n, param_mode))
+    return ops
+
+
+def lower_gate(g, n, param_mode=False):
+    if param_mode and getattr(g, "param_index", None) is not None:
+        word = getattr(g, "generator_word", None)
+        bits = tuple(n - 1 - q for q in g.indices)
+        if word in ('x', 'y', 'z'):
+            return [EOp('rot1', bits, param=int(g.param_index), pauli=PAULI_CODE[word])]
+        if word in ('xx', 'yy', 'zz'):
+            return [EOp('rot2', bits, param=int(g.param_index), pauli=PAULI_CODE[word[0]])]
+        raise NotImplementedError(f"{g.name} cannot take a device angle")
+    ops = lower_matrix_gate(g.matrix, list(g.indices), n, g.name)
+    if g.name == "Reset" and getattr(g, "normalize", True):
+        ops.append(EOp('norm', ()))
+    return ops
+
+
+def fuse_1q(ops):
+    """Multiply runs of dense 1-qubit ops per bit; flush a bit's pending matrix
+    before any other op touches it."""
+    pending = {}
+    out = []
+
+    def flush(b):
+        m = pending.pop(b, None)
+        if m is not None:
+            out.append(EOp('mat1', (b,), m))
+
+    for op in ops:
+        if op.kind == 'mat1':
+            b = op.bits[0]
+            pending[b] = op.mat @ pending[b] if b in pending else op.mat.copy()
+            continue
+        if op.kind == 'norm':
+            for b in sorted(pending):
+                flush(b)
+        else:
+            for b in op.bits:
+                flush(b)
+        out.append(op)
+    for b in sorted(pending):
+        flush(b)
+    return out
+
+
+# ----------------------------------------------------------------- scheduling
+@dataclass
+class Stage:
+    rbits: list
+    ops: list = field(default_factory=list)
+
+
+@dataclass
+class Pass:
+    kind: int
+    bits: list                 # global bits (tile: ascending tile-local order; dense: matrix order)
+    ops: list = field(default_factory=list)
+    tb: int = 0
+    stages: list = field(default_factory=list)
+    mat: np.ndarray = None
+
+
+@dataclass
+class TileConfig:
+    """Launch shape of tile passes (picked by the autotuner).
+
+    tb: log2 threads per workgroup (tile L = tb + 4 bits);
+    low_bits: lowest address bits every tile contains (contiguous runs of
+              2^low_bits amplitudes per load);
+    max_cost: cap on Σ per-amplitude VALU cost of the ops in one pass.
+    """
+    tb: int = 8
+    low_bits: int = 6
+    max_cost: float = 96.0
+
+
+def _is_barrier(op):
+    return op.kind in ('dense', 'norm')
+
+
+def partition(ops, n, cfg):
+    """Greedy packing of ops into passes (see module docstring, step 3)."""
+    L = cfg.tb + RB
+    if L > n:
+        raise ValueError(f"tile of {L} bits exceeds n={n}")
+    m = min(cfg.low_bits, L)
+    low = set(range(m))
+    rem = list(ops)
+    passes = []
+    while rem:
+        head = rem[0]
+        if head.kind == 'dense':
+            passes.append(Pass(PASS_DENSE, list(head.bits), [head], mat=head.mat))
+            rem.pop(0)
+            continue
+        if head.kind == 'norm':
+            passes.append(Pass(PASS_NORMALIZE, [], [head]))
+            rem.pop(0)
+            continue
+        T = set(low)
+        blocked = set()
+        taken = []
+        cost = 0.0
+        for i, op in enumerate(rem):
+            if _is_barrier(op):
+                break
+            b = set(op.bits)
+            if b & blocked:
+                blocked |= b
+            else:
+                need = b - T
+                c = OP_COST[op.kind]
+                if len(T) + len(need) <= L and (cost + c <= cfg.max_cost or not taken):
+                    T |= need
+                    taken.append(i)
+                    cost += c
+                else:
+                    blocked |= b
+            if len(blocked) == n or cost + 0.25 > cfg.max_cost:
+                break   # nothing later can join this pass
+        # complete the tile with the lowest unused bits
+        x = 0
+        while len(T) < L:
+            if x not in T:
+                T.add(x)
+            x += 1
+        tset = set(taken)
+        p = Pass(PASS_TILE, sorted(T), [rem[i] for i in taken], tb=cfg.tb)
+        passes.append(p)
+        rem = [op for i, op in enumerate(rem) if i not in tset]
+    return passes
+
+
+def _stage_ops(p):
+    """Split a tile pass's ops into register stages (tile-local bits)."""
+    L = p.tb + RB
+    loc = {g: i for i, g in enumerate(p.bits)}
+    pend = [(op, tuple(loc[b] for b in op.bits)) for op in p.ops]
+    lane_bits = set(range(min(6, L)))
+    io_bits = list(range(L - RB, L))            # register bits of a coalesced layout
+    stages = []
+    cur = None
+    while pend:
+        if cur is not None:
+            took, blocked, keep = [], set(), []
+            for op, lb in pend:
+                s = set(lb)
+                if not (s & blocked) and s <= set(cur.rbits):
+                    took.append((op, lb))
+                else:
+                    blocked |= s
+                    keep.append((op, lb))
+            if took:
+                cur.ops.extend(took)
+                pend = keep
+                continue
+        R, blocked = [], set()
+        for op, lb in pend:
+            s = set(lb)
+            if s & blocked:
+                blocked |= s
+                continue
+            new = [x for x in lb if x not in R]
+            if len(R) + len(new) <= RB:
+                R += new
+            else:
+                blocked |= s
+            if len(R) == RB:
+                break
+        for x in reversed(range(L)):            # fill with high tile bits
+            if len(R) == RB:
+                break
+            if x not in R:
+                R.append(x)
+        cur = Stage(R, [])
+        stages.append(cur)
+    if not stages:
+        stages.append(Stage(list(io_bits), []))
+    # coalesced load/store: the first/last layout must keep lanes on low bits
+    if set(stages[0].rbits) & lane_bits:
+        stages.insert(0, Stage(list(io_bits), []))
+    if set(stages[-1].rbits) & lane_bits:
+        stages.append(Stage(list(io_bits), []))
+    p.stages = stages
+    return p
+
+
+# ----------------------------------------------------------------- encoding
+def _swap_qubits_4x4(m):
+    return m.reshape(2, 2, 2, 2).transpose(1, 0, 3, 2).reshape(4, 4)
+
+
+class Program:
+    """An encoded program plus the metadata the host needs."""
+
+    def __init__(self, n, passes, blob, grad_params, n_params):
+        self.n = n
+        self.passes = passes
+        self.blob = blob                  # np.uint8 array, the svh program
+        self.grad_params = grad_params    # slot -> param index (all passes)
+        self.n_params = n_params
+        self._dev = {}
+
+    @property
+    def n_passes(self):
+        return len(self.passes)
+
+    def n_tile_passes(self):
+        return sum(p.kind == PASS_TILE for p in self.passes)
+
+    def device_copy(self, device):
+        import torch
+        key = str(device)
+        if key not in self._dev:
+            self._dev[key] = torch.from_numpy(self.blob).to(device)
+        return self._dev[key]
+
+    def describe(self):
+        kinds = {PASS_TILE: 0, PASS_DENSE: 0, PASS_NORMALIZE: 0}
+        st = 0
+        for p in self.passes:
+            kinds[p.kind] += 1
+            st += len(p.stages)
+        return dict(passes=len(self.passes), tile=kinds[PASS_TILE], dense=kinds[PASS_DENSE],
+                    normalize=kinds[PASS_NORMALIZE], stages=st,
+                    ops=sum(len(p.ops) for p in self.passes))
+
+
+def encode(n, passes, grad=False, n_params=0):
+    stage_rows, op_rows, grad_params, pool = [], [], [], []
+    pass_rows = np.zeros(len(passes), dtype=PASS_DT)
+    for pi, p in enumerate(passes):
+        pr = pass_rows[pi]
+        pr['kind'] = p.kind
+        pr['rb'] = RB
+        if p.kind == PASS_TILE:
+            L = p.tb + RB
+            pr['L'] = L
+            pr['tb'] = p.tb
+            pr['bits'][:L] = p.bits
+            pr['stage_begin'] = len(stage_rows)
+            pr['n_stages'] = len(p.stages)
+            pr['grad_begin'] = len(grad_params)
+            slot = 0
+            for st in p.stages:
+                s = np.zeros((), dtype=STAGE_DT)
+                s['rbits'][:RB] = st.rbits
+                s['op_begin'] = len(op_rows)
+                ridx = {x: k for k, x in enumerate(st.rbits)}
+                for op, lb in st.ops:
+                    o = np.zeros((), dtype=OP_DT)
+                    regs = [ridx[x] for x in lb]
+                    o['param'] = op.param
+                    o['pauli'] = op.pauli
+                    o['flags'] = op.flags
+                    if op.kind == 'mat1':
+                        o['kind'] = OP_MAT1
+                        o['a'] = regs[0]
+                        o['m'][:8] = op.mat.reshape(-1).view(np.float64)
+                    elif op.kind == 'mat2':
+                        m = op.mat
+                        a, b = regs
+                        if a < b:
+                            a, b = b, a
+                            m = _swap_qubits_4x4(m)
+                        o['kind'], o['a'], o['b'] = OP_MAT2, a, b
+                        o['m'][:] = np.ascontiguousarray(m).reshape(-1).view(np.float64)
+                    elif op.kind == 'cx':
+                        o['kind'], o['a'], o['b'] = OP_CX, regs[0], regs[1]
+                    elif op.kind == 'cz':
+                        o['kind'], o['a'], o['b'] = OP_CZ, max(regs), min(regs)
+                    elif op.kind == 'rot1':
+                        o['kind'], o['a'] = OP_ROT1, regs[0]
+                    elif op.kind == 'rot2':
+                        o['kind'], o['a'], o['b'] = OP_ROT2, max(regs), min(regs)
+                    else:
+                        raise ValueError(op.kind)
+                    if grad and op.kind in ('rot1', 'rot2') and (op.flags & OPF_GRAD):
+                        o['grad_slot'] = slot
+                        grad_params.append(op.param)
+                        slot += 1
+                    op_rows.append(o)
+                s['op_end'] = len(op_rows)
+                stage_rows.append(s)
+            pr['n_grad'] = slot
+        elif p.kind == PASS_DENSE:
+            k = len(p.bits)
+            pr['L'] = k
+            pr['bits'][:k] = p.bits
+            pr['pool_off'] = len(pool)
+            pool.extend(np.ascontiguousarray(p.mat, dtype=np.complex128).reshape(-1).view(np.float64))
+    stages = np.array(stage_rows, dtype=STAGE_DT) if stage_rows else np.zeros(0, STAGE_DT)
+    opsa = np.array(op_rows, dtype=OP_DT) if op_rows else np.zeros(0, OP_DT)
+    grads = np.array(grad_params, dtype=np.int32)
+    poola = np.array(pool, dtype=np.float64)
+
+    def align(x):
+        return (x + 63) & ~63
+    off = align(HDR_DT.itemsize)
+    offs = {}
+    for name, arr in (('passes', pass_rows), ('stages', stages), ('ops', opsa), ('grad', grads),
+                      ('pool', poola)):
+        offs[name] = off
+        off = align(off + arr.nbytes)
+    blob = np.zeros(off, dtype=np.uint8)
+    h = np.zeros((), dtype=HDR_DT)
+    h['magic'], h['version'], h['n_qubits'] = MAGIC, ABI_VERSION, n
+    h['n_passes'], h['n_stages'], h['n_ops'] = len(pass_rows), len(stages), len(opsa)
+    h['n_grad_slots'], h['n_pool'] = len(grads), len(poola)
+    for name in offs:
+        h['off_' + name] = offs[name]
+    blob[:HDR_DT.itemsize] = np.frombuffer(h.tobytes(), dtype=np.uint8)
+    for name, arr in (('passes', pass_rows), ('stages', stages), ('ops', opsa), ('grad', grads),
+                      ('pool', poola)):
+        if arr.nbytes:
+            blob[offs[name]:offs[name] + arr.nbytes] = np.frombuffer(arr.tobytes(), dtype=np.uint8)
+    return Program(n, passes, blob, grads, n_params)
+
+
+def min_tile_n(grad=False):
+    return 8
+
+
+def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
+    """EOps -> Program.  Small registers (n < 8) use one dense pass per op."""
+    cfg = cfg or TileConfig()
+    if fuse:
+        ops = fuse_1q(ops)
+    if n < min_tile_n():
+        if any(op.kind in ('rot1', 'rot2') for op in ops):
+            raise NotImplementedError("device-angle rotations need n >= 8")
+        passes = []
+        for op in ops:
+            if op.kind == 'norm':
+                passes.append(Pass(PASS_NORMALIZE, [], [op]))
+            else:
+                passes.append(Pass(PASS_DENSE, list(_dense_bits(op)), [op], mat=_dense_mat(op)))
+        return encode(n, passes, grad, n_params)
+    tb = min(cfg.tb, n - RB)
+    cfg = TileConfig(tb=tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost)
+    passes = partition(ops, n, cfg)
+    for p in passes:
+        if p.kind == PASS_TILE:
+            _stage_ops(p)
+    return encode(n, passes, grad, n_params)
+
+
+def _dense_bits(op):
+    if op.kind == 'cx':
+        return (op.bits[0], op.bits[1])
+    return op.bits
+
+
+def _dense_mat(op):
+    if op.kind in ('mat1', 'mat2', 'dense'):
+        return op.mat
+    if op.kind == 'cx':
+        return _CX_TC.astype(np.complex128)     # (target, control) order
+    if op.kind == 'cz':
+        return _CZ.astype(np.complex128)
+    raise ValueError(op.kind)

@@ -1,0 +1,90 @@
+"""CPU interpreter of encoded gate programs (test helper).
+
+Decodes the byte layout of include/qaravan_hip.h and applies every pass,
+stage and op in order with the oracle's ``op_action``, translating register
+bits -> tile-local bits -> address bits -> qaravan qubits exactly as the HIP
+kernels do.  It checks the compiler (lowering, fusion, partition, stages and
+encoding) on the CPU; the kernels' own data movement is checked on the GPU.
+"""
+import numpy as np
+
+from oracle import sv_oracle as O
+from qaravan_amd import program as P
+
+_CX_TC = np.array([[1, 0, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0], [0, 1, 0, 0]], dtype=np.complex128)
+_CZ = np.diag([1, 1, 1, -1]).astype(np.complex128)
+_WORD = {1: 'x', 2: 'y', 3: 'z'}
+
+
+def decode(blob):
+    h = np.frombuffer(blob[:P.HDR_DT.itemsize].tobytes(), dtype=P.HDR_DT)[0]
+
+    def arr(name, dt, count):
+        off = int(h['off_' + name])
+        return np.frombuffer(blob[off:off + dt.itemsize * count].tobytes(), dtype=dt)
+    passes = arr('passes', P.PASS_DT, int(h['n_passes']))
+    stages = arr('stages', P.STAGE_DT, int(h['n_stages']))
+    ops = arr('ops', P.OP_DT, int(h['n_ops']))
+    grad = arr('grad', np.dtype('<i4'), int(h['n_grad_slots']))
+    pool = arr('pool', np.dtype('<f8'), int(h['n_pool']))
+    return h, passes, stages, ops, grad, pool
+
+
+def _op_matrix(o, theta):
+    kind = int(o['kind'])
+    m = o['m'].view(np.complex128)
+    if kind == P.OP_MAT1:
+        return m[:4].reshape(2, 2)
+    if kind == P.OP_MAT2:
+        return m[:16].reshape(4, 4)
+    if kind == P.OP_CX:
+        return _CX_TC
+    if kind == P.OP_CZ:
+        return _CZ
+    s = -1.0 if int(o['flags']) & P.OPF_NEG else 1.0
+    w = _WORD[int(o['pauli'])]
+    if kind == P.OP_ROT1:
+        return O.rotation(w, s * theta[int(o['param'])])
+    if kind == P.OP_ROT2:
+        return O.rotation(w + w, s * theta[int(o['param'])])
+    raise ValueError(kind)
+
+
+def run(blob, sv, theta=None, lam=None, grad=None):
+    """Apply the program to flat state ``sv`` (and ``lam`` in adjoint mode);
+    adds 2·Im⟨λ|P|ψ⟩ into ``grad`` for flagged ops.  Returns (sv, lam)."""
+    h, passes, stages, ops, gslots, pool = decode(blob)
+    n = int(h['n_qubits'])
+    sv = np.asarray(sv, dtype=np.complex128).copy()
+    lam = None if lam is None else np.asarray(lam, dtype=np.complex128).copy()
+    for p in passes:
+        kind = int(p['kind'])
+        if kind == P.PASS_NORMALIZE:
+            sv = sv / np.linalg.norm(sv)
+        elif kind == P.PASS_DENSE:
+            k = int(p['L'])
+            m = pool[int(p['pool_off']):int(p['pool_off']) + 2 * 4 ** k].view(np.complex128)
+            qubits = [n - 1 - int(b) for b in p['bits'][:k]]
+            sv = O.op_action(m.reshape(2 ** k, 2 ** k), qubits, sv)
+        else:
+            L = int(p['L'])
+            tile_bits = [int(b) for b in p['bits'][:L]]
+            for s in range(int(p['stage_begin']), int(p['stage_begin']) + int(p['n_stages'])):
+                st = stages[s]
+                rb = [int(x) for x in st['rbits'][:P.RB]]
+                for oi in range(int(st['op_begin']), int(st['op_end'])):
+                    o = ops[oi]
+                    kind_o = int(o['kind'])
+                    two = kind_o in (P.OP_MAT2, P.OP_CX, P.OP_CZ, P.OP_ROT2)
+                    regs = [int(o['a'])] + ([int(o['b'])] if two else [])
+                    qubits = [n - 1 - tile_bits[rb[r]] for r in regs]
+                    if lam is not None and int(o['flags']) & P.OPF_GRAD:
+                        w = _WORD[int(o['pauli'])]
+                        pw = O.pauli_string_matrix(w if kind_o == P.OP_ROT1 else w + w)
+                        slot = int(p['grad_begin']) + int(o['grad_slot'])
+                        grad[gslots[slot]] += 2.0 * np.imag(np.vdot(lam, O.op_action(pw, qubits, sv)))
+                    m = _op_matrix(o, theta)
+                    sv = O.op_action(m, qubits, sv)
+                    if lam is not None:
+                        lam = O.op_action(m, qubits, lam)
+    return sv, lam

@@ -1,0 +1,140 @@
+"""The circuit compiler checked on the CPU: programs interpreted op by op
+(tests/prog_interp.py) must reproduce the oracle's state."""
+import numpy as np
+import pytest
+
+import prog_interp
+from oracle import sv_oracle as O
+from qaravan_amd import CNOT, CZ, Circuit, Gate, H, Reset, X, random_unitary
+from qaravan_amd import program as P
+from qaravan_amd.workloads import brickwork_circuit, hea_circuit, hea_theta, tfim_terms
+
+
+def random_circuit(n, n_gates, rng, kinds=("u1", "u2", "cx", "cz"), k3=False):
+    gates = []
+    for _ in range(n_gates):
+        kind = kinds[int(rng.integers(len(kinds)))]
+        if kind == "u1":
+            gates.append(Gate("U", [int(rng.integers(n))], random_unitary(2, rng)))
+        elif kind == "u2":
+            qs = [int(x) for x in rng.choice(n, size=2, replace=False)]
+            gates.append(Gate("U", qs, random_unitary(4, rng)))
+        elif kind == "cx":
+            i = int(rng.integers(n - 1))
+            gates.append(CNOT([i, i + 1] if rng.random() < 0.5 else [i + 1, i], n))
+        elif kind == "cz":
+            qs = [int(x) for x in rng.choice(n, size=2, replace=False)]
+            gates.append(CZ(qs))
+    if k3:
+        qs = [int(x) for x in rng.choice(n, size=3, replace=False)]
+        gates.insert(len(gates) // 2, Gate("U3", qs, random_unitary(8, rng)))
+    return Circuit(gates, n=n)
+
+
+def compile_circ(circ, cfg):
+    return P.compile_ops(P.lower_circuit(circ.build(), circ.num_sites), circ.num_sites, cfg)
+
+
+def test_lowering_recognises_cnot_orientations_and_cz():
+    n = 4
+    ops = P.lower_circuit(Circuit([CNOT([1, 2], n), CNOT([3, 2], n), CZ([0, 3])], n=n).build(), n)
+    kinds = [(o.kind, o.bits) for o in ops]
+    # CNOT([1,2]): target qubit 1 (bit 2), control qubit 2 (bit 1)
+    assert ('cx', (2, 1)) in kinds
+    # CNOT([3,2]) is stored as indices [2,3] with the swapped matrix: target qubit 3 (bit 0)
+    assert ('cx', (0, 1)) in kinds
+    assert ('cz', (3, 0)) in kinds
+
+
+@pytest.mark.parametrize("tb,low,cost", [(4, 6, 96.0), (4, 4, 24.0), (5, 6, 1e9), (6, 2, 40.0),
+                                          (8, 6, 96.0)])
+@pytest.mark.parametrize("seed", range(3))
+def test_program_matches_oracle(tb, low, cost, seed):
+    rng = np.random.default_rng(100 + seed)
+    n = max(tb + 4, 9) + seed
+    circ = random_circuit(n, 60, rng, k3=(seed == 1))
+    prog = compile_circ(circ, P.TileConfig(tb=tb, low_bits=low, max_cost=cost))
+    got, _ = prog_interp.run(prog.blob, O.init_state(n))
+    ref = O.run(circ.gate_list, n)
+    assert np.linalg.norm(got - ref) < 1e-12
+
+
+def test_small_register_uses_dense_passes():
+    circ = Circuit([H(0), CNOT([1, 0], 3), CNOT([2, 1], 3)], n=3)
+    prog = compile_circ(circ, P.TileConfig())
+    assert all(p.kind == P.PASS_DENSE for p in prog.passes)
+    got, _ = prog_interp.run(prog.blob, O.init_state(3))
+    assert np.allclose(got, O.run(circ.gate_list, 3))
+
+
+def test_reset_lowering_normalises():
+    gates = [H(0), CNOT([1, 0]), Reset(1, "1"), H(1), CNOT([2, 1])]
+    circ = Circuit(gates, n=9)
+    prog = compile_circ(circ, P.TileConfig(tb=4))
+    assert any(p.kind == P.PASS_NORMALIZE for p in prog.passes)
+    got, _ = prog_interp.run(prog.blob, O.init_state(9))
+    assert np.allclose(got, O.run(gates, 9), atol=1e-12)
+
+
+def test_brickwork_12q_program():
+    circ = brickwork_circuit(12, 20)
+    for cfg in (P.TileConfig(tb=8, low_bits=6, max_cost=96), P.TileConfig(tb=6, low_bits=4, max_cost=48)):
+        prog = compile_circ(circ, cfg)
+        got, _ = prog_interp.run(prog.blob, O.init_state(12))
+        assert np.linalg.norm(got - O.run(circ.gate_list, 12)) < 1e-12
+
+
+def test_every_stage_op_fits_its_register_bits():
+    circ = brickwork_circuit(20, 4)
+    prog = compile_circ(circ, P.TileConfig(tb=8, low_bits=6, max_cost=96))
+    h, passes, stages, ops, _, _ = prog_interp.decode(prog.blob)
+    for p in passes:
+        if p['kind'] != P.PASS_TILE:
+            continue
+        bits = list(p['bits'][:p['L']])
+        assert bits == sorted(bits) and len(set(bits)) == p['L']
+        assert set(range(6)) <= set(bits)
+        first = stages[p['stage_begin']]
+        last = stages[p['stage_begin'] + p['n_stages'] - 1]
+        # lanes stay on the six lowest tile bits for the HBM load and store
+        assert not set(first['rbits'][:4]) & set(range(6))
+        assert not set(last['rbits'][:4]) & set(range(6))
+        for s in range(p['stage_begin'], p['stage_begin'] + p['n_stages']):
+            st = stages[s]
+            assert len(set(st['rbits'][:4])) == 4
+            for o in ops[st['op_begin']:st['op_end']]:
+                assert 0 <= o['a'] < 4
+
+
+def test_gate_count_and_fusion_c1():
+    circ = brickwork_circuit(28, 40)
+    ops = P.fuse_1q(P.lower_circuit(circ.build(), 28))
+    n1 = sum(o.kind == 'mat1' for o in ops)
+    ncx = sum(o.kind == 'cx' for o in ops)
+    assert ncx == 540
+    assert n1 <= 28 * 40          # RX·RY·RZ per qubit per layer fused into one 2x2
+
+
+def _param_ops(circ, n):
+    return P.lower_circuit(circ.build(), n, param_mode=True)
+
+
+@pytest.mark.parametrize("tb", [4, 6])
+def test_param_forward_and_adjoint_programs(tb):
+    """Forward program with device-θ rotations, and the adjoint program's
+    gradient, interpreted on the CPU, against the oracle's adjoint."""
+    from qaravan_amd.adjoint import ParamProgram
+    n, layers = 10, 3
+    theta = hea_theta(n, layers)
+    circ = hea_circuit(n, layers, theta)
+    terms = tfim_terms(n)
+    pp = ParamProgram(circ, terms, len(theta), cfg=P.TileConfig(tb=tb, low_bits=6, max_cost=40))
+    psi, _ = prog_interp.run(pp.fwd.blob, O.init_state(n), theta=theta)
+    ref = O.run(circ.gate_list, n)
+    assert np.linalg.norm(psi - ref) < 1e-12
+    lam = O.apply_pauli_sum(psi, terms)
+    grad = np.zeros(len(theta))
+    prog_interp.run(pp.adj.blob, psi, theta=theta, lam=lam, grad=grad)
+    from test_oracle import hea_spec
+    _, gref = O.adjoint_gradient(hea_spec(n, layers), theta, n, terms)
+    assert np.allclose(grad, gref, atol=1e-10)
