@@ -145,10 +145,14 @@ def bench_adjoint(dev, iters=5):
     rel_err = float(np.linalg.norm(g_adj - g128) / np.linalg.norm(g128))
 
     th_cpu = torch.tensor(theta, dtype=torch.float64)
-    tape_energy_and_grad(circ, th_cpu, terms, dev)  # warm-up
+    # warm the torch kernels on a 12-qubit copy of the ansatz, then time one
+    # full gradient at 20 qubits (the tape path takes seconds per gradient)
+    th_small = torch.tensor(hea_theta(12, 2), dtype=torch.float64)
+    tape_energy_and_grad(hea_circuit(12, 2, th_small.numpy()), th_small, tfim_terms(12), dev)
     torch.cuda.synchronize(dev)
+    log("[bench] torch tape: timing one 20-qubit gradient")
     t0 = time.perf_counter()
-    tape_iters = 2
+    tape_iters = 1
     for _ in range(tape_iters):
         e_t, g_t = tape_energy_and_grad(circ, th_cpu, terms, dev)
     torch.cuda.synchronize(dev)

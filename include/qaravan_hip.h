@@ -26,7 +26,8 @@
  *                         sites, i.e. ⟨a|b⟩ (applications/compilation.py:86 torch.dot)
  *   svh_run_adjoint       applications/compilation.py:73-133 (cost.backward() through the
  *                         torch backend); same gradient, computed by the adjoint method
- *   svh_apply_blocks_mfma no counterpart: op_action applied to 5-qubit gates one by one
+ *   svh_apply_block5      no counterpart: op_action applies a 5-qubit gate with one
+ *                         tensordot; here it is an FP32 MFMA GEMM (c64)
  *   svh_batch_run         tensorQ/monte_carlo_sim.py:14-35 (many small StatevectorSim runs)
  */
 #ifndef QARAVAN_HIP_H
@@ -52,7 +53,7 @@ enum { SVH_C64 = 0, SVH_C128 = 1 };
  */
 #define SVH_PROG_MAGIC 0x53564850u /* 'SVHP' */
 #define SVH_MAX_TILE_BITS 16
-#define SVH_MAX_REG_BITS 5
+#define SVH_MAX_REG_BITS 4
 
 enum {
     SVH_PASS_TILE = 1,      /* fused pass over tiles of 2^L amplitudes            */
@@ -84,7 +85,8 @@ typedef struct {
     int32_t n_grad_slots;
     int32_t n_pool; /* doubles in the matrix pool */
     int64_t off_passes, off_stages, off_ops, off_grad, off_pool;
-    int64_t reserved[3];
+    int64_t n_params; /* length of θ read by ROT ops (0 if none)                  */
+    int64_t reserved[2];
 } svh_prog_header_t; /* 96 bytes */
 
 typedef struct {
@@ -102,10 +104,19 @@ typedef struct {
                                         the most significant row bit)                */
 } svh_pass_t; /* 104 bytes */
 
+/* A stage's layout: thread t holds registers j = 0..15 at tile-local index
+ * insert-zeros(t at rsorted) | loc(j), i.e. at global amplitude
+ * base | Σ_i bit_i(t)·2^tgbit[i] | goff[j].  The tables are precomputed by the
+ * host so the kernel does no bit arithmetic per amplitude. */
 typedef struct {
+    int64_t goff[16];    /* global amplitude offset of register j                 */
+    int32_t lsw[16];     /* swizzled tile-local (LDS) offset of register j         */
     int32_t rbits[8];    /* tile-local bit held in register-index bit k (k < rb)   */
+    int32_t rsorted[4];  /* rbits[0..rb) in ascending order                         */
+    int32_t tgbit[12];   /* global bit of thread-index bit i (i < tb)              */
     int32_t op_begin, op_end;
-} svh_stage_t; /* 40 bytes */
+    int32_t pad[2];
+} svh_stage_t; /* 304 bytes */
 
 typedef struct {
     int32_t kind;        /* SVH_OP_*                                                */
@@ -116,7 +127,8 @@ typedef struct {
     int32_t grad_slot;   /* pass-local gradient slot (SVH_OPF_GRAD)                  */
     int32_t pad;
     double m[32];        /* complex row-major matrix (re, im pairs)                  */
-} svh_op_t; /* 288 bytes */
+    float mf[32];        /* the same matrix rounded to float (used by SVH_C64)       */
+} svh_op_t; /* 416 bytes */
 
 /* ---- status ---------------------------------------------------------------- */
 const char *svh_last_error(void);
@@ -131,11 +143,18 @@ int svh_init_basis(void *psi, int n, int dtype, uint64_t idx, void *stream);
 int svh_apply_dense(void *psi, int n, int dtype, const int *bits, int k,
                     const void *mat_dev, void *stream);
 
+/* Apply a dense 5-qubit gate with the FP32 matrix cores (SVH_C64 only, n >= 12).
+ * bits[j] = address bit of matrix row bit j (j = 0 least significant);
+ * u_dev = device 32x32 complex64 matrix, row-major.  Programs route their
+ * 5-qubit dense passes here for complex64 states. */
+int svh_apply_block5(void *psi, int n, int dtype, const int *bits, const void *u_dev, void *stream);
+
 /* Bytes of device scratch svh_run_program / svh_run_adjoint need for `prog`. */
 int svh_program_scratch_bytes(const void *prog_host, int dtype, size_t *bytes);
 
 /* Run every pass of a program on ψ in place.  theta_dev: device float64 angles
- * for ROT ops (may be NULL if the program has none). */
+ * for ROT ops (may be NULL if the program has none); cos/sin of every angle are
+ * evaluated once per call in FP64 and rounded to the state's precision. */
 int svh_run_program(void *psi, int n, int dtype, const void *prog_host,
                     const void *prog_dev, const double *theta_dev, void *scratch,
                     size_t scratch_bytes, void *stream);

@@ -20,17 +20,23 @@ struct alignas(2 * sizeof(R)) cpx {
 };
 
 template <typename R>
+__device__ __forceinline__ R ffma(R a, R b, R c) {
+    if constexpr (std::is_same<R, float>::value) return __builtin_fmaf(a, b, c);
+    else return __builtin_fma(a, b, c);
+}
+// a*b with explicit FMAs: 2 VALU ops per component
+template <typename R>
 __device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
-    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+    return {ffma(a.x, b.x, -a.y * b.y), ffma(a.x, b.y, a.y * b.x)};
 }
 template <typename R>
 __device__ __forceinline__ cpx<R> cadd(cpx<R> a, cpx<R> b) {
     return {a.x + b.x, a.y + b.y};
 }
-// a*b + c
+// a*b + c: 2 FMAs per component
 template <typename R>
 __device__ __forceinline__ cpx<R> cfma(cpx<R> a, cpx<R> b, cpx<R> c) {
-    return {a.x * b.x - a.y * b.y + c.x, a.x * b.y + a.y * b.x + c.y};
+    return {ffma(a.x, b.x, ffma(-a.y, b.y, c.x)), ffma(a.x, b.y, ffma(a.y, b.x, c.y))};
 }
 template <typename R>
 __device__ __forceinline__ cpx<R> cneg(cpx<R> a) {

@@ -1,0 +1,184 @@
+// Dense 5-qubit blocks on the FP32 matrix cores (complex64 states).
+//
+// A 32x32 complex unitary U acting on 5 address bits is a real GEMM
+//   Y(64 x C) = W(64 x 64) · X(64 x C),  W[2r+a][2c+b] = [[Re U, -Im U], [Im U, Re U]]
+// where X's rows interleave (re, im) of the 32 block amplitudes and its C
+// columns run over the other bits.  One workgroup (4 waves) owns a tile of
+// 4096 amplitudes: the 5 block bits and the 7 lowest other bits, so every tile
+// is a set of >= 128-amplitude contiguous runs in HBM.  The tile is staged in
+// LDS with coalesced loads, each wave multiplies a 64 x 32-column slab with
+// v_mfma_f32_32x32x2_f32 (W lives in 64 VGPRs per lane for the whole launch),
+// writes its result back to LDS, and the tile is stored coalesced.  The grid
+// is persistent (workgroups stride over tiles) so W is built once per WG.
+//
+// Operand layouts of v_mfma_f32_32x32x2_f32 (wave64):
+//   A[i][k]: lane i + 32k;  B[k][j]: lane j + 32k;
+//   D[i][j]: lane j + 32·((i/4)%2), register (i%4) + 4·(i/8).
+#include "svh_common.h"
+
+namespace svh {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+struct Block5Args {
+    int64_t gk[16];   // global offset of tile element 256·k (k < 16)
+    int32_t lk[16];   // LDS complex index of tile element 256·k
+    int32_t tg[8];    // global bit of tile-order bit i (i < 8, the thread bits)
+    int32_t tl[8];    // LDS index bit of tile-order bit i
+    int64_t tmask;    // global bits inside the tile
+    int64_t ntiles;   // 2^(n-12)
+    int32_t n;
+    int32_t pad;
+};
+
+// LDS layout of a tile: complex index r·128 + c (r = block row, c = column).
+__global__ void __launch_bounds__(256)
+k_block5(cpx<float> *__restrict__ psi, const cpx<float> *__restrict__ U, Block5Args A) {
+    __shared__ cpx<float> tile[4096];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int j32 = lane & 31;
+    const int hi = lane >> 5;  // 0: real rows / k even, 1: imaginary rows / k odd
+
+    // W fragments: Af[mt][s] = W[32 mt + j32][2 s + hi]
+    //   row i = 32 mt + j32 -> amplitude r = i / 2, part a = i % 2
+    //   col k = 2 s + hi    -> amplitude c = s,     part b = hi
+    float Af[2][32];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+        const int i = 32 * mt + j32;
+        const int r = i >> 1, a = i & 1;
+#pragma unroll
+        for (int s = 0; s < 32; ++s) {
+            const cpx<float> u = U[r * 32 + s];
+            Af[mt][s] = a == 0 ? (hi == 0 ? u.x : -u.y) : (hi == 0 ? u.y : u.x);
+        }
+    }
+
+    // this thread's share of the tile-order index: t = tid (8 bits)
+    uint64_t gt = 0;
+    int lt = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        gt |= (uint64_t)((tid >> i) & 1) << A.tg[i];
+        lt |= ((tid >> i) & 1) << A.tl[i];
+    }
+    const float *tf = reinterpret_cast<const float *>(tile);
+    const int col = 32 * wave + j32;
+
+    for (int64_t t = blockIdx.x; t < A.ntiles; t += gridDim.x) {
+        // tile base: t deposited into the non-tile bits
+        uint64_t base = 0;
+        {
+            uint64_t g = (uint64_t)t;
+            for (int b = 0; b < A.n && g; ++b) {
+                if (!((A.tmask >> b) & 1)) {
+                    base |= (g & 1ull) << b;
+                    g >>= 1;
+                }
+            }
+        }
+        const cpx<float> *src = psi + (base | gt);
+        cpx<float> vals[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) vals[k] = src[A.gk[k]];
+        __syncthreads();  // previous tile's stores from LDS are done
+#pragma unroll
+        for (int k = 0; k < 16; ++k) tile[lt | A.lk[k]] = vals[k];
+        __syncthreads();
+
+        floatx16 acc0 = {0}, acc1 = {0};
+#pragma unroll
+        for (int s = 0; s < 32; ++s) {
+            const float b = tf[(s * 128 + col) * 2 + hi];
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Af[0][s], b, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Af[1][s], b, acc1, 0, 0, 0);
+        }
+        // every wave has read its slab before any wave overwrites LDS
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 16; v += 2) {
+            const int i = (v & 3) + 4 * hi + 8 * (v >> 2);  // even row -> (re, im) in v, v+1
+            tile[(i >> 1) * 128 + col] = cpx<float>{acc0[v], acc0[v + 1]};
+            tile[((32 + i) >> 1) * 128 + col] = cpx<float>{acc1[v], acc1[v + 1]};
+        }
+        __syncthreads();
+        cpx<float> *dst = psi + (base | gt);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) dst[A.gk[k]] = tile[lt | A.lk[k]];
+    }
+}
+
+}  // namespace svh
+
+using namespace svh;
+
+// Tile geometry for a block on address bits bits[0..5) (bits[j] = row bit j).
+static int block5_args(int n, const int *bits, Block5Args *A) {
+    if (n < 12) return fail(-2, "MFMA block path needs n >= 12");
+    uint64_t bm = 0;
+    for (int j = 0; j < 5; ++j) {
+        if (bits[j] < 0 || bits[j] >= n || ((bm >> bits[j]) & 1)) return fail(-4, "bad block bits");
+        bm |= 1ull << bits[j];
+    }
+    // tile = the 5 block bits + the 7 lowest other bits; LDS index bit of each
+    int other[7], no = 0;
+    for (int b = 0; b < n && no < 7; ++b)
+        if (!((bm >> b) & 1)) other[no++] = b;
+    uint64_t tmask = bm;
+    for (int i = 0; i < 7; ++i) tmask |= 1ull << other[i];
+    int tbits[12], nt = 0;
+    for (int b = 0; b < n; ++b)
+        if ((tmask >> b) & 1) tbits[nt++] = b;
+    // LDS index of tile-order bit p: block row bit j -> 7 + j, other bit i -> i
+    int lpos[12];
+    for (int p = 0; p < 12; ++p) {
+        const int g = tbits[p];
+        lpos[p] = -1;
+        for (int j = 0; j < 5; ++j)
+            if (bits[j] == g) lpos[p] = 7 + j;
+        for (int i = 0; i < 7; ++i)
+            if (other[i] == g) lpos[p] = i;
+    }
+    for (int i = 0; i < 8; ++i) {
+        A->tg[i] = tbits[i];
+        A->tl[i] = lpos[i];
+    }
+    for (int k = 0; k < 16; ++k) {
+        int64_t go = 0;
+        int lo = 0;
+        for (int i = 0; i < 4; ++i) {
+            if ((k >> i) & 1) {
+                go |= 1ll << tbits[8 + i];
+                lo |= 1 << lpos[8 + i];
+            }
+        }
+        A->gk[k] = go;
+        A->lk[k] = lo;
+    }
+    A->tmask = (int64_t)tmask;
+    A->ntiles = 1ll << (n - 12);
+    A->n = n;
+    A->pad = 0;
+    return 0;
+}
+
+extern "C" int svh_apply_block5(void *psi, int n, int dtype, const int *bits, const void *u_dev,
+                                void *stream) {
+    if (!psi || !bits || !u_dev) return fail(-1, "NULL pointer");
+    if (dtype != SVH_C64) return fail(-3, "the MFMA block path is complex64 only");
+    if (n < 12 || n > kMaxQubits) return fail(-2, "MFMA block path needs 12 <= n <= 36");
+    Block5Args A;
+    if (int e = block5_args(n, bits, &A)) return e;
+    int dev = 0, cus = 0;
+    if (int e = hip_status(hipGetDevice(&dev), "hipGetDevice")) return e;
+    if (int e = hip_status(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                           "hipDeviceGetAttribute"))
+        return e;
+    // two resident workgroups per CU (150 VGPRs + 32 AGPRs per lane)
+    const int64_t cap = 2 * (int64_t)(cus > 0 ? cus : 256);
+    const int64_t grid = A.ntiles < cap ? A.ntiles : cap;
+    k_block5<<<(unsigned)grid, 256, 0, (hipStream_t)stream>>>((cpx<float> *)psi, (const cpx<float> *)u_dev, A);
+    return checked_launch("k_block5");
+}

@@ -15,12 +15,11 @@
 
 namespace svh {
 
-template <int RB>
-__device__ __forceinline__ int lds_swz(int li) {
-    // XOR the low 5 bits with the next 5: spreads lane groups whose tile-local
-    // bits skip the register bits across the LDS banks.  A bijection on [0, 2^L).
-    return li ^ ((li >> 5) & 31);
-}
+// LDS swizzle: XOR the low 5 bits with the next 5.  Spreads lane groups whose
+// tile-local bits skip the register bits across the LDS banks.  It is linear
+// over GF(2), so swz(a | b) = swz(a) ^ swz(b) for disjoint a, b: the host stores
+// swz(register offset) per stage and the kernel XORs in swz(thread offset).
+__device__ __forceinline__ int lds_swz(int li) { return li ^ ((li >> 5) & 31); }
 
 // ------------------------------------------------ in-register gate kernels
 template <typename R, int RB, int A>
@@ -197,28 +196,29 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 template <typename R>
-__device__ __forceinline__ cpx<R> ld_m(const double *m, int i) {
-    return cpx<R>{(R)m[2 * i], (R)m[2 * i + 1]};
+__device__ __forceinline__ cpx<R> ld_m(const svh_op_t &op, int i) {
+    if constexpr (std::is_same<R, float>::value) return cpx<R>{op.mf[2 * i], op.mf[2 * i + 1]};
+    else return cpx<R>{op.m[2 * i], op.m[2 * i + 1]};
 }
 
+// (cos θ, sin θ) of the op's angle from the per-call table (FP64-evaluated).
 template <typename R>
-__device__ __forceinline__ void op_sincos(const double *theta, const svh_op_t &op, R &cs, R &sn) {
-    const R th = (R)theta[op.param];
-    if constexpr (std::is_same<R, float>::value) sincosf(th, &sn, &cs);
-    else sincos(th, &sn, &cs);
-    if (op.flags & SVH_OPF_NEG) sn = -sn;
+__device__ __forceinline__ void op_sincos(const cpx<R> *trig, const svh_op_t &op, R &cs, R &sn) {
+    const cpx<R> t = trig[op.param];
+    cs = t.x;
+    sn = (op.flags & SVH_OPF_NEG) ? -t.y : t.y;
 }
 
 // Apply one op to the registers (v: ψ, w: λ when DUAL).  Returns the gradient
 // partial (this thread's Σ Im(conj λ · Pψ)) through *g when the op asks for it.
 template <typename R, int RB, bool DUAL>
 __device__ __forceinline__ void apply_op(cpx<R> (&v)[1 << RB], cpx<R> (&w)[DUAL ? (1 << RB) : 1],
-                                         const svh_op_t &op, const double *theta, double *g,
+                                         const svh_op_t &op, const cpx<R> *trig, double *g,
                                          bool *has_g) {
     const int kind = op.kind;
     if (kind == SVH_OP_MAT1) {
-        const cpx<R> m0 = ld_m<R>(op.m, 0), m1 = ld_m<R>(op.m, 1), m2 = ld_m<R>(op.m, 2),
-                     m3 = ld_m<R>(op.m, 3);
+        const cpx<R> m0 = ld_m<R>(op, 0), m1 = ld_m<R>(op, 1), m2 = ld_m<R>(op, 2),
+                     m3 = ld_m<R>(op, 3);
 #define CALL_M1(A)                                      \
     ap_mat1<R, RB, A>(v, m0, m1, m2, m3);               \
     if constexpr (DUAL) ap_mat1<R, RB, A>(w, m0, m1, m2, m3);
@@ -226,7 +226,7 @@ __device__ __forceinline__ void apply_op(cpx<R> (&v)[1 << RB], cpx<R> (&w)[DUAL 
 #undef CALL_M1
     } else if (kind == SVH_OP_ROT1) {
         R cs, sn;
-        op_sincos<R>(theta, op, cs, sn);
+        op_sincos<R>(trig, op, cs, sn);
         const int pauli = op.pauli;
         if constexpr (DUAL) {
             if (op.flags & SVH_OPF_GRAD) {
@@ -266,7 +266,7 @@ __device__ __forceinline__ void apply_op(cpx<R> (&v)[1 << RB], cpx<R> (&w)[DUAL 
     } else if (kind == SVH_OP_MAT2) {
         cpx<R> m[16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) m[i] = ld_m<R>(op.m, i);
+        for (int i = 0; i < 16; ++i) m[i] = ld_m<R>(op, i);
 #define CALL_M2(H, L)                    \
     ap_mat2<R, RB, H, L>(v, m);          \
     if constexpr (DUAL) ap_mat2<R, RB, H, L>(w, m);
@@ -274,7 +274,7 @@ __device__ __forceinline__ void apply_op(cpx<R> (&v)[1 << RB], cpx<R> (&w)[DUAL 
 #undef CALL_M2
     } else if (kind == SVH_OP_ROT2) {
         R cs, sn;
-        op_sincos<R>(theta, op, cs, sn);
+        op_sincos<R>(trig, op, cs, sn);
         const int pauli = op.pauli;
         if constexpr (DUAL) {
             if (op.flags & SVH_OPF_GRAD) {
@@ -292,37 +292,24 @@ __device__ __forceinline__ void apply_op(cpx<R> (&v)[1 << RB], cpx<R> (&w)[DUAL 
     }
 }
 
-// Tile-local index of this thread's register 0 in a layout whose register
-// bits are rb[0..RB): thread bits fill the remaining tile bits in order.
-template <int L, int RB>
+// Tile-local index of this thread's register 0 in a stage's layout: the thread
+// index with zeros inserted at the (ascending) register bits.
 __device__ __forceinline__ int thr_local(const svh_stage_t &S, int tid) {
-    int rmask = 0;
+    int t = tid;
 #pragma unroll
-    for (int k = 0; k < RB; ++k) rmask |= 1 << S.rbits[k];
-    int li = 0, t = tid;
-#pragma unroll
-    for (int p = 0; p < L; ++p) {
-        if (!((rmask >> p) & 1)) {
-            li |= (t & 1) << p;
-            t >>= 1;
-        }
+    for (int k = 0; k < 4; ++k) {
+        const int r = S.rsorted[k];
+        t = ((t >> r) << (r + 1)) | (t & ((1 << r) - 1));
     }
-    return li;
+    return t;
 }
 
-template <int RB>
-__device__ __forceinline__ int reg_local(const svh_stage_t &S, int j) {
-    int rl = 0;
-#pragma unroll
-    for (int k = 0; k < RB; ++k) rl |= ((j >> k) & 1) << S.rbits[k];
-    return rl;
-}
-
-template <int L>
-__device__ __forceinline__ uint64_t to_global(const int (&gbit)[L], int li) {
+// Global amplitude offset of this thread's register 0 (thread bits only).
+template <int TB>
+__device__ __forceinline__ uint64_t thr_global(const svh_stage_t &S, int tid) {
     uint64_t a = 0;
 #pragma unroll
-    for (int p = 0; p < L; ++p) a |= (uint64_t)((li >> p) & 1) << gbit[p];
+    for (int i = 0; i < TB; ++i) a |= (uint64_t)((tid >> i) & 1) << S.tgbit[i];
     return a;
 }
 
@@ -331,8 +318,9 @@ template <typename R, int TB, int RB, bool DUAL>
 __global__ void __launch_bounds__(1 << TB)
 k_tile(cpx<R> *__restrict__ psi, cpx<R> *__restrict__ lam, int n,
        const svh_pass_t *__restrict__ pass, const svh_stage_t *__restrict__ stages,
-       const svh_op_t *__restrict__ ops, const double *__restrict__ theta,
+       const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig,
        double *__restrict__ gpart) {
+    static_assert(RB == 4, "stage tables hold 16 registers");
     constexpr int L = TB + RB;
     constexpr int NR = 1 << RB;
     constexpr int NW = ((1 << TB) + 63) / 64;  // waves per workgroup
@@ -345,18 +333,14 @@ k_tile(cpx<R> *__restrict__ psi, cpx<R> *__restrict__ lam, int n,
     const int n_stages = pass->n_stages;
     const int s0 = pass->stage_begin;
     const int ng = pass->n_grad;
-    int gbit[L];
-#pragma unroll
-    for (int i = 0; i < L; ++i) gbit[i] = pass->bits[i];
-
     // Tile base address: workgroup id deposited into the non-tile bits.
     uint64_t tmask = 0;
 #pragma unroll
-    for (int i = 0; i < L; ++i) tmask |= 1ull << gbit[i];
+    for (int i = 0; i < L; ++i) tmask |= 1ull << pass->bits[i];
     uint64_t base = 0;
     {
         uint64_t g = blockIdx.x;
-        for (int b = 0; b < n; ++b) {
+        for (int b = 0; b < n && g; ++b) {
             if (!((tmask >> b) & 1)) {
                 base |= (g & 1ull) << b;
                 g >>= 1;
@@ -370,12 +354,12 @@ k_tile(cpx<R> *__restrict__ psi, cpx<R> *__restrict__ lam, int n,
     // ---- load in stage 0's layout
     {
         const svh_stage_t &S = stages[s0];
-        const uint64_t tg = base | to_global<L>(gbit, thr_local<L, RB>(S, tid));
+        const cpx<R> *p = psi + (base | thr_global<TB>(S, tid));
+        const cpx<R> *q = DUAL ? lam + (base | thr_global<TB>(S, tid)) : nullptr;
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
-            const uint64_t a = tg | to_global<L>(gbit, reg_local<RB>(S, j));
-            v[j] = psi[a];
-            if constexpr (DUAL) w[j] = lam[a];
+            v[j] = p[S.goff[j]];
+            if constexpr (DUAL) w[j] = q[S.goff[j]];
         }
     }
 
@@ -384,19 +368,19 @@ k_tile(cpx<R> *__restrict__ psi, cpx<R> *__restrict__ lam, int n,
         if (s > 0) {
             // ---- re-distribute the tile from stage s-1's layout to stage s's
             const svh_stage_t &P = stages[s0 + s - 1];
-            const int tp = thr_local<L, RB>(P, tid);
+            const int tp = lds_swz(thr_local(P, tid));
             if (s > 1) __syncthreads();  // previous exchange's reads are done
 #pragma unroll
             for (int j = 0; j < NR; ++j) {
-                const int li = lds_swz<RB>(tp | reg_local<RB>(P, j));
+                const int li = tp ^ P.lsw[j];
                 lds[li] = v[j];
                 if constexpr (DUAL) lds2[li] = w[j];
             }
             __syncthreads();
-            const int tn = thr_local<L, RB>(S, tid);
+            const int tn = lds_swz(thr_local(S, tid));
 #pragma unroll
             for (int j = 0; j < NR; ++j) {
-                const int li = lds_swz<RB>(tn | reg_local<RB>(S, j));
+                const int li = tn ^ S.lsw[j];
                 v[j] = lds[li];
                 if constexpr (DUAL) w[j] = lds2[li];
             }
@@ -406,7 +390,7 @@ k_tile(cpx<R> *__restrict__ psi, cpx<R> *__restrict__ lam, int n,
             const svh_op_t &op = ops[o];
             double g = 0.0;
             bool has_g = false;
-            apply_op<R, RB, DUAL>(v, w, op, theta, &g, &has_g);
+            apply_op<R, RB, DUAL>(v, w, op, trig, &g, &has_g);
             if constexpr (DUAL) {
                 if (has_g) {
                     g = wave_sum<((1 << TB) < 64 ? (1 << TB) : 64)>(g);
@@ -419,12 +403,12 @@ k_tile(cpx<R> *__restrict__ psi, cpx<R> *__restrict__ lam, int n,
     // ---- store in the last stage's layout
     {
         const svh_stage_t &S = stages[s0 + n_stages - 1];
-        const uint64_t tg = base | to_global<L>(gbit, thr_local<L, RB>(S, tid));
+        cpx<R> *p = psi + (base | thr_global<TB>(S, tid));
+        cpx<R> *q = DUAL ? lam + (base | thr_global<TB>(S, tid)) : nullptr;
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
-            const uint64_t a = tg | to_global<L>(gbit, reg_local<RB>(S, j));
-            psi[a] = v[j];
-            if constexpr (DUAL) lam[a] = w[j];
+            p[S.goff[j]] = v[j];
+            if constexpr (DUAL) q[S.goff[j]] = w[j];
         }
     }
 
@@ -437,6 +421,17 @@ k_tile(cpx<R> *__restrict__ psi, cpx<R> *__restrict__ lam, int n,
                 gpart[(uint64_t)blockIdx.x * ng + sl] = acc;
             }
         }
+    }
+}
+
+// (cos θ_p, sin θ_p) for every parameter, evaluated in FP64 and rounded once.
+template <typename R>
+__global__ void k_trig(const double *__restrict__ theta, int64_t np, cpx<R> *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) {
+        double sn, cs;
+        sincos(theta[i], &sn, &cs);
+        out[i] = cpx<R>{(R)cs, (R)sn};
     }
 }
 
@@ -462,7 +457,7 @@ constexpr size_t kMaxLds = 160 * 1024;
 
 template <typename R, int TB, bool DUAL>
 static int launch_tb(cpx<R> *psi, cpx<R> *lam, int n, const svh_pass_t &hp, const svh_pass_t *dp,
-                     const svh_stage_t *st, const svh_op_t *op, const double *theta, double *gpart,
+                     const svh_stage_t *st, const svh_op_t *op, const cpx<R> *trig, double *gpart,
                      hipStream_t stream) {
     constexpr int RB = 4;
     constexpr int L = TB + RB;
@@ -480,7 +475,7 @@ static int launch_tb(cpx<R> *psi, cpx<R> *lam, int n, const svh_pass_t &hp, cons
     }
     const uint64_t grid = 1ull << (n - L);
     if (grid > 0x7fffffffull) return fail(-7, "grid too large");
-    kern<<<(unsigned)grid, 1 << TB, lds, stream>>>(psi, lam, n, dp, st, op, theta, gpart);
+    kern<<<(unsigned)grid, 1 << TB, lds, stream>>>(psi, lam, n, dp, st, op, trig, gpart);
     return checked_launch("k_tile");
 }
 
@@ -489,7 +484,7 @@ static int launch_tb(cpx<R> *psi, cpx<R> *lam, int n, const svh_pass_t &hp, cons
 // <= 8 adjoint).
 template <typename R, bool DUAL>
 static int launch_tile(cpx<R> *psi, cpx<R> *lam, int n, const svh_pass_t &hp, const svh_pass_t *dp,
-                       const svh_stage_t *st, const svh_op_t *op, const double *theta, double *gpart,
+                       const svh_stage_t *st, const svh_op_t *op, const cpx<R> *trig, double *gpart,
                        hipStream_t stream) {
     if (hp.rb != 4) return fail(-7, "only 4 register bits are built");
     constexpr bool C64 = std::is_same<R, float>::value;
@@ -499,13 +494,13 @@ static int launch_tile(cpx<R> *psi, cpx<R> *lam, int n, const svh_pass_t &hp, co
         return fail(-7, "tile pass tb=" + std::to_string(hp.tb) + " outside [" + std::to_string(tb_min) + ", " +
                             std::to_string(tb_max) + "] for this dtype/mode");
     switch (hp.tb) {
-        case 4: return launch_tb<R, 4, DUAL>(psi, lam, n, hp, dp, st, op, theta, gpart, stream);
-        case 5: return launch_tb<R, 5, DUAL>(psi, lam, n, hp, dp, st, op, theta, gpart, stream);
-        case 6: return launch_tb<R, 6, DUAL>(psi, lam, n, hp, dp, st, op, theta, gpart, stream);
-        case 7: return launch_tb<R, 7, DUAL>(psi, lam, n, hp, dp, st, op, theta, gpart, stream);
-        case 8: return launch_tb<R, 8, DUAL>(psi, lam, n, hp, dp, st, op, theta, gpart, stream);
-        case 9: if constexpr (tb_max >= 9) return launch_tb<R, 9, DUAL>(psi, lam, n, hp, dp, st, op, theta, gpart, stream); break;
-        case 10: if constexpr (tb_max >= 10) return launch_tb<R, 10, DUAL>(psi, lam, n, hp, dp, st, op, theta, gpart, stream); break;
+        case 4: return launch_tb<R, 4, DUAL>(psi, lam, n, hp, dp, st, op, trig, gpart, stream);
+        case 5: return launch_tb<R, 5, DUAL>(psi, lam, n, hp, dp, st, op, trig, gpart, stream);
+        case 6: return launch_tb<R, 6, DUAL>(psi, lam, n, hp, dp, st, op, trig, gpart, stream);
+        case 7: return launch_tb<R, 7, DUAL>(psi, lam, n, hp, dp, st, op, trig, gpart, stream);
+        case 8: return launch_tb<R, 8, DUAL>(psi, lam, n, hp, dp, st, op, trig, gpart, stream);
+        case 9: if constexpr (tb_max >= 9) return launch_tb<R, 9, DUAL>(psi, lam, n, hp, dp, st, op, trig, gpart, stream); break;
+        case 10: if constexpr (tb_max >= 10) return launch_tb<R, 10, DUAL>(psi, lam, n, hp, dp, st, op, trig, gpart, stream); break;
         default: break;
     }
     return fail(-7, "tile pass tb not built");
@@ -545,6 +540,27 @@ static int check_program(const void *prog_host, int n, const svh_prog_header_t *
                 }
                 if (S.op_begin < 0 || S.op_end < S.op_begin || S.op_end > h->n_ops)
                     return fail(-8, "stage " + std::to_string(s) + ": bad op range");
+                // the precomputed address tables must describe this layout exactly
+                int rs_prev = -1;
+                for (int k = 0; k < P.rb; ++k) {
+                    if (S.rsorted[k] <= rs_prev || !((rm >> S.rsorted[k]) & 1))
+                        return fail(-8, "stage " + std::to_string(s) + ": bad sorted register bits");
+                    rs_prev = S.rsorted[k];
+                }
+                uint64_t seen = 0;
+                for (int i = 0; i < P.tb; ++i) {
+                    const int g = S.tgbit[i];
+                    if (g < 0 || g >= h->n_qubits || !((m >> g) & 1) || ((seen >> g) & 1))
+                        return fail(-8, "stage " + std::to_string(s) + ": bad thread bit table");
+                    seen |= 1ull << g;
+                }
+                for (int j = 0; j < (1 << P.rb); ++j) {
+                    const uint64_t go = (uint64_t)S.goff[j];
+                    if (S.goff[j] < 0 || (go & ~m) || (go & seen))
+                        return fail(-8, "stage " + std::to_string(s) + ": bad register offset table");
+                    if (S.lsw[j] < 0 || S.lsw[j] >= (1 << P.L))
+                        return fail(-8, "stage " + std::to_string(s) + ": bad LDS offset table");
+                }
                 for (int o = S.op_begin; o < S.op_end; ++o) {
                     const svh_op_t &O = ops[o];
                     const bool two = O.kind == SVH_OP_MAT2 || O.kind == SVH_OP_CX ||
@@ -555,8 +571,9 @@ static int check_program(const void *prog_host, int n, const svh_prog_header_t *
                         return fail(-8, "op " + std::to_string(o) + ": bad register bit");
                     if ((O.kind == SVH_OP_MAT2 || O.kind == SVH_OP_ROT2) && O.a < O.b)
                         return fail(-8, "op " + std::to_string(o) + ": a must exceed b");
-                    if ((O.kind == SVH_OP_ROT1 || O.kind == SVH_OP_ROT2) && (O.pauli < 1 || O.pauli > 3))
-                        return fail(-8, "op " + std::to_string(o) + ": bad pauli");
+                    if ((O.kind == SVH_OP_ROT1 || O.kind == SVH_OP_ROT2) &&
+                        (O.pauli < 1 || O.pauli > 3 || O.param < 0 || O.param >= h->n_params))
+                        return fail(-8, "op " + std::to_string(o) + ": bad pauli or param index");
                     if ((O.flags & SVH_OPF_GRAD) && (O.grad_slot < 0 || O.grad_slot >= P.n_grad))
                         return fail(-8, "op " + std::to_string(o) + ": bad grad slot");
                 }
@@ -589,6 +606,15 @@ extern "C" int svh_apply_dense(void *psi, int n, int dtype, const int *bits, int
 extern "C" int svh_norm2(const void *psi, int n, int dtype, double *out_dev, void *scratch,
                          size_t scratch_bytes, void *stream);
 extern "C" int svh_scale_by_inv_norm(void *psi, int n, int dtype, const double *norm2_dev, void *stream);
+extern "C" int svh_apply_block5(void *psi, int n, int dtype, const int *bits, const void *u_dev, void *stream);
+
+// Scratch layout: [trig table: n_params (cos, sin) pairs][work area], the work
+// area holding FP64 reduction partials, a dense pass's matrix, or the adjoint's
+// per-workgroup gradient partials.
+static size_t trig_bytes(const svh_prog_header_t *h, int dtype) {
+    const size_t es = dtype == SVH_C64 ? 8 : 16;
+    return ((size_t)h->n_params * es + 255) & ~(size_t)255;
+}
 
 static size_t scratch_need(const svh_prog_header_t *h, int dtype, bool dual) {
     const char *b = reinterpret_cast<const char *>(h);
@@ -603,7 +629,7 @@ static size_t scratch_need(const svh_prog_header_t *h, int dtype, bool dual) {
             s = sizeof(double) * (((size_t)1 << (h->n_qubits - P.L)) * P.n_grad);
         if (s > need) need = s;
     }
-    return (need + 255) & ~(size_t)255;
+    return trig_bytes(h, dtype) + ((need + 255) & ~(size_t)255);
 }
 
 static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_host, const void *prog_dev,
@@ -618,6 +644,7 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
     const size_t need = scratch_need(h, dtype, dual);
     if (!scratch || scratch_bytes < need)
         return fail(-5, "scratch too small: need " + std::to_string(need) + " bytes");
+    if (h->n_params > 0 && !theta_dev) return fail(-1, "program has rotation ops but theta is NULL");
     const char *hb = static_cast<const char *>(prog_host);
     const char *db = static_cast<const char *>(prog_dev);
     const auto *hpass = reinterpret_cast<const svh_pass_t *>(hb + h->off_passes);
@@ -627,33 +654,32 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
     const auto *dgrad = reinterpret_cast<const int32_t *>(db + h->off_grad);
     const auto *dpool = reinterpret_cast<const double *>(db + h->off_pool);
     hipStream_t st = (hipStream_t)stream;
-    double *sd = static_cast<double *>(scratch);
+    void *trig = scratch;
+    double *sd = reinterpret_cast<double *>(static_cast<char *>(scratch) + trig_bytes(h, dtype));
 
-    // theta must be present when any op reads it
-    if (!theta_dev) {
-        const auto *hop = reinterpret_cast<const svh_op_t *>(hb + h->off_ops);
-        for (int o = 0; o < h->n_ops; ++o)
-            if (hop[o].kind == SVH_OP_ROT1 || hop[o].kind == SVH_OP_ROT2)
-                return fail(-1, "program has rotation ops but theta is NULL");
+    if (h->n_params > 0) {
+        const unsigned g = (unsigned)((h->n_params + 255) / 256);
+        if (dtype == SVH_C64)
+            k_trig<float><<<g, 256, 0, st>>>(theta_dev, h->n_params, (cpx<float> *)trig);
+        else
+            k_trig<double><<<g, 256, 0, st>>>(theta_dev, h->n_params, (cpx<double> *)trig);
+        if (int e = checked_launch("k_trig")) return e;
     }
 
     for (int p = 0; p < h->n_passes; ++p) {
         const svh_pass_t &P = hpass[p];
         int e = 0;
         if (P.kind == SVH_PASS_TILE) {
-            if (dual && P.n_grad > 0) {
-                // partials sized by check above
-            }
             if (dtype == SVH_C64)
                 e = dual ? launch_tile<float, true>((cpx<float> *)psi, (cpx<float> *)lam, n, P, dpass + p, dstage,
-                                                    dop, theta_dev, sd, st)
+                                                    dop, (const cpx<float> *)trig, sd, st)
                          : launch_tile<float, false>((cpx<float> *)psi, nullptr, n, P, dpass + p, dstage, dop,
-                                                     theta_dev, sd, st);
+                                                     (const cpx<float> *)trig, sd, st);
             else
                 e = dual ? launch_tile<double, true>((cpx<double> *)psi, (cpx<double> *)lam, n, P, dpass + p,
-                                                     dstage, dop, theta_dev, sd, st)
+                                                     dstage, dop, (const cpx<double> *)trig, sd, st)
                          : launch_tile<double, false>((cpx<double> *)psi, nullptr, n, P, dpass + p, dstage, dop,
-                                                      theta_dev, sd, st);
+                                                      (const cpx<double> *)trig, sd, st);
             if (!e && dual && P.n_grad > 0) {
                 const int nb = (int)(1ull << (n - P.L));
                 k_grad_reduce<<<P.n_grad, 256, 0, st>>>(sd, nb, P.n_grad, dgrad + P.grad_begin, grad_dev);
@@ -669,7 +695,14 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
                 k_pool_to<double><<<(K * K + 255) / 256, 256, 0, st>>>(dpool + P.pool_off, (cpx<double> *)mat,
                                                                        K * K);
             e = checked_launch("k_pool_to");
-            if (!e) e = svh_apply_dense(psi, n, dtype, P.bits, P.L, mat, stream);
+            if (!e && P.L == 5 && dtype == SVH_C64 && n >= 12) {
+                // complex64 5-qubit gates go to the matrix cores
+                int rb[5];
+                for (int j = 0; j < 5; ++j) rb[j] = P.bits[4 - j];
+                e = svh_apply_block5(psi, n, dtype, rb, mat, stream);
+            } else if (!e) {
+                e = svh_apply_dense(psi, n, dtype, P.bits, P.L, mat, stream);
+            }
         } else if (P.kind == SVH_PASS_NORMALIZE) {
             if (dual) return fail(-8, "normalisation is not differentiable in the adjoint sweep");
             e = svh_norm2(psi, n, dtype, sd + kReduceBlocks, sd, kReduceBlocks * sizeof(double), stream);

@@ -39,15 +39,35 @@ MAX_TILE_BITS = 16
 HDR_DT = np.dtype([('magic', '<u4'), ('version', '<i4'), ('n_qubits', '<i4'), ('n_passes', '<i4'),
                    ('n_stages', '<i4'), ('n_ops', '<i4'), ('n_grad_slots', '<i4'), ('n_pool', '<i4'),
                    ('off_passes', '<i8'), ('off_stages', '<i8'), ('off_ops', '<i8'),
-                   ('off_grad', '<i8'), ('off_pool', '<i8'), ('reserved', '<i8', (3,))])
+                   ('off_grad', '<i8'), ('off_pool', '<i8'), ('n_params', '<i8'),
+                   ('reserved', '<i8', (2,))])
 PASS_DT = np.dtype([('kind', '<i4'), ('L', '<i4'), ('tb', '<i4'), ('rb', '<i4'),
                     ('stage_begin', '<i4'), ('n_stages', '<i4'), ('grad_begin', '<i4'),
                     ('n_grad', '<i4'), ('pool_off', '<i8'), ('bits', '<i4', (MAX_TILE_BITS,))])
-STAGE_DT = np.dtype([('rbits', '<i4', (8,)), ('op_begin', '<i4'), ('op_end', '<i4')])
+STAGE_DT = np.dtype([('goff', '<i8', (16,)), ('lsw', '<i4', (16,)), ('rbits', '<i4', (8,)),
+                     ('rsorted', '<i4', (4,)), ('tgbit', '<i4', (12,)), ('op_begin', '<i4'),
+                     ('op_end', '<i4'), ('pad', '<i4', (2,))])
 OP_DT = np.dtype([('kind', '<i4'), ('a', '<i4'), ('b', '<i4'), ('param', '<i4'), ('pauli', '<i4'),
-                  ('flags', '<i4'), ('grad_slot', '<i4'), ('pad', '<i4'), ('m', '<f8', (32,))])
+                  ('flags', '<i4'), ('grad_slot', '<i4'), ('pad', '<i4'), ('m', '<f8', (32,)),
+                  ('mf', '<f4', (32,))])
 assert HDR_DT.itemsize == 96 and PASS_DT.itemsize == 104
-assert STAGE_DT.itemsize == 40 and OP_DT.itemsize == 288
+assert STAGE_DT.itemsize == 304 and OP_DT.itemsize == 416
+
+
+def lds_swz(li):
+    """The kernel's LDS swizzle (linear over GF(2)): low 5 bits ^= next 5."""
+    return li ^ ((li >> 5) & 31)
+
+
+def stage_tables(tile_bits, tb, rbits):
+    """Address tables of one stage layout (see svh_stage_t): thread bits fill
+    the non-register tile-local bits in ascending order."""
+    L = tb + RB
+    tpos = [x for x in range(L) if x not in rbits]
+    tgbit = [tile_bits[x] for x in tpos]
+    loc = [sum(((j >> k) & 1) << rbits[k] for k in range(RB)) for j in range(1 << RB)]
+    goff = [sum(((j >> k) & 1) << tile_bits[rbits[k]] for k in range(RB)) for j in range(1 << RB)]
+    return dict(rsorted=sorted(rbits), tgbit=tgbit, lsw=[lds_swz(x) for x in loc], goff=goff, loc=loc)
 
 PAULI_CODE = {'x': 1, 'y': 2, 'z': 3}
 
@@ -387,6 +407,11 @@ def encode(n, passes, grad=False, n_params=0):
             for st in p.stages:
                 s = np.zeros((), dtype=STAGE_DT)
                 s['rbits'][:RB] = st.rbits
+                tab = stage_tables(p.bits, p.tb, st.rbits)
+                s['rsorted'][:] = tab['rsorted']
+                s['tgbit'][:p.tb] = tab['tgbit']
+                s['lsw'][:] = tab['lsw']
+                s['goff'][:] = tab['goff']
                 s['op_begin'] = len(op_rows)
                 ridx = {x: k for k, x in enumerate(st.rbits)}
                 for op, lb in st.ops:
@@ -398,7 +423,7 @@ def encode(n, passes, grad=False, n_params=0):
                     if op.kind == 'mat1':
                         o['kind'] = OP_MAT1
                         o['a'] = regs[0]
-                        o['m'][:8] = op.mat.reshape(-1).view(np.float64)
+                        o['m'][:8] = np.ascontiguousarray(op.mat, dtype=np.complex128).reshape(-1).view(np.float64)
                     elif op.kind == 'mat2':
                         m = op.mat
                         a, b = regs
@@ -417,6 +442,7 @@ def encode(n, passes, grad=False, n_params=0):
                         o['kind'], o['a'], o['b'] = OP_ROT2, max(regs), min(regs)
                     else:
                         raise ValueError(op.kind)
+                    o['mf'][:] = o['m'].astype(np.float32)
                     if grad and op.kind in ('rot1', 'rot2') and (op.flags & OPF_GRAD):
                         o['grad_slot'] = slot
                         grad_params.append(op.param)
@@ -449,6 +475,8 @@ def encode(n, passes, grad=False, n_params=0):
     h['magic'], h['version'], h['n_qubits'] = MAGIC, ABI_VERSION, n
     h['n_passes'], h['n_stages'], h['n_ops'] = len(pass_rows), len(stages), len(opsa)
     h['n_grad_slots'], h['n_pool'] = len(grads), len(poola)
+    used = [int(o['param']) + 1 for o in op_rows if int(o['kind']) in (OP_ROT1, OP_ROT2)]
+    h['n_params'] = max([int(n_params)] + used) if used else 0
     for name in offs:
         h['off_' + name] = offs[name]
     blob[:HDR_DT.itemsize] = np.frombuffer(h.tobytes(), dtype=np.uint8)

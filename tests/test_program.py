@@ -138,3 +138,39 @@ def test_param_forward_and_adjoint_programs(tb):
     from test_oracle import hea_spec
     _, gref = O.adjoint_gradient(hea_spec(n, layers), theta, n, terms)
     assert np.allclose(grad, gref, atol=1e-10)
+
+
+def test_stage_tables_cover_each_tile_once():
+    """Emulates the kernel's address arithmetic for every stage: the global
+    offsets (thread part | register part) hit every amplitude of the tile once,
+    and the swizzled LDS indices are a permutation of [0, 2^L)."""
+    circ = brickwork_circuit(16, 3)
+    for cfg in (P.TileConfig(tb=4, low_bits=4, max_cost=40), P.TileConfig(tb=8, low_bits=6, max_cost=96)):
+        prog = compile_circ(circ, cfg)
+        h, passes, stages, ops, _, _ = prog_interp.decode(prog.blob)
+        for p in passes:
+            if p['kind'] != P.PASS_TILE:
+                continue
+            L, tb = int(p['L']), int(p['tb'])
+            tile = set()
+            for i in range(1 << L):
+                a = 0
+                for k in range(L):
+                    a |= ((i >> k) & 1) << int(p['bits'][k])
+                tile.add(a)
+            for s in range(p['stage_begin'], p['stage_begin'] + p['n_stages']):
+                st = stages[s]
+                seen_g, seen_l = set(), set()
+                for t in range(1 << tb):
+                    tg = 0
+                    for i in range(tb):
+                        tg |= ((t >> i) & 1) << int(st['tgbit'][i])
+                    tl = t
+                    for r in st['rsorted']:
+                        r = int(r)
+                        tl = ((tl >> r) << (r + 1)) | (tl & ((1 << r) - 1))
+                    for j in range(16):
+                        seen_g.add(tg | int(st['goff'][j]))
+                        seen_l.add(P.lds_swz(tl) ^ int(st['lsw'][j]))
+                assert seen_g == tile
+                assert seen_l == set(range(1 << L))
