@@ -28,7 +28,10 @@
  *                         torch backend); same gradient, computed by the adjoint method
  *   svh_apply_block5      no counterpart: op_action applies a 5-qubit gate with one
  *                         tensordot; here it is an FP32 MFMA GEMM (c64)
- *   svh_batch_run         tensorQ/monte_carlo_sim.py:14-35 (many small StatevectorSim runs)
+ *   svh_batch_*           tensorQ/monte_carlo_sim.py:14-35 (many small StatevectorSim runs,
+ *                         one after another); here a whole batch of parametrised circuits
+ *                         with their adjoint gradients runs as one launch sequence
+ *   svh_timestamp         no counterpart (device clock for per-circuit latencies)
  */
 #ifndef QARAVAN_HIP_H
 #define QARAVAN_HIP_H
@@ -99,10 +102,19 @@ typedef struct {
     int32_t grad_begin;  /* first grad slot of this pass (adjoint)                  */
     int32_t n_grad;
     int64_t pool_off;    /* DENSE: offset (doubles) of the 2^k x 2^k complex matrix  */
+    int32_t flags;       /* SVH_PASSF_*                                              */
+    int32_t pad;
     int32_t bits[SVH_MAX_TILE_BITS]; /* TILE: global bit of tile-local bit i (i < L);
                                         DENSE: global bit of matrix qubit i (i = 0 is
                                         the most significant row bit)                */
-} svh_pass_t; /* 104 bytes */
+} svh_pass_t; /* 112 bytes */
+
+enum {
+    /* TILE (forward only): persistent workgroups; the next tile is copied
+     * HBM -> LDS asynchronously while the current one is computed.  Needs
+     * tile-local bit 0 = address bit 0. */
+    SVH_PASSF_PREFETCH = 1
+};
 
 /* A stage's layout: thread t holds registers j = 0..15 at tile-local index
  * insert-zeros(t at rsorted) | loc(j), i.e. at global amplitude
@@ -129,6 +141,39 @@ typedef struct {
     double m[32];        /* complex row-major matrix (re, im pairs)                  */
     float mf[32];        /* the same matrix rounded to float (used by SVH_C64)       */
 } svh_op_t; /* 416 bytes */
+
+/* ---- batches of circuits ----------------------------------------------------
+ * Many independent circuits (different n, depth, θ) advance together: round r
+ * of a batch runs pass r of every circuit that has one, in ONE launch per
+ * (tile shape, mode), so small circuits fill the GPU together.  The caller
+ * (host planner) builds the item tables once, in device memory; a whole batch
+ * (init, trig, forward rounds, H·ψ and energies, adjoint rounds with gradient
+ * reduction) is a fixed launch sequence that can be captured in a HIP graph. */
+typedef struct {
+    uint64_t psi, lam;          /* device state / λ pointers (lam = 0 forward)      */
+    uint64_t pass, stages, ops; /* device program pointers: this circuit's pass r   */
+    uint64_t trig;              /* device (cos, sin) table of the circuit          */
+    uint64_t gpart;             /* device per-workgroup gradient partials           */
+    int64_t wg_begin;           /* first workgroup of the item in the launch         */
+    int32_t n, pad;
+} svh_batch_item_t; /* 72 bytes */
+
+typedef struct {
+    uint64_t part;              /* device partial sums: row b of ng values          */
+    uint64_t slot_param;        /* device int32 slot -> output index                 */
+    uint64_t out;               /* device float64 outputs (grad or energy)           */
+    uint64_t tstamp;            /* device uint64 completion timestamp or 0           */
+    int32_t nblocks, ng;        /* partial rows, slots                                */
+    int64_t slot_begin;         /* first launch block of the item                     */
+} svh_batch_red_t; /* 48 bytes */
+
+typedef struct {
+    uint64_t psi, lam;          /* device ψ (read) and λ = Hψ (written)             */
+    uint64_t terms;             /* device term table (32 B per term: x, z, re, im)    */
+    uint64_t epart;             /* device FP64 per-block partials of Re⟨ψ|λ⟩         */
+    int64_t block_begin;        /* first launch block of the item                     */
+    int32_t n, nterms;
+} svh_batch_ham_t; /* 48 bytes */
 
 /* ---- status ---------------------------------------------------------------- */
 const char *svh_last_error(void);
@@ -187,6 +232,31 @@ int svh_pauli_expect(const void *psi, int n, int dtype, int n_terms,
 int svh_pauli_apply(const void *psi, void *out, int n, int dtype, int n_terms,
                     const uint64_t *xmask, const uint64_t *zmask, const double *coeff,
                     void *scratch, size_t scratch_bytes, void *stream);
+
+/* One batch round: k_tile over every item (items_dev sorted by wg_begin,
+ * total_wg workgroups), all with the same tb and mode (dual = adjoint). */
+int svh_batch_tile(const void *items_dev, int nitems, int64_t total_wg, int dtype, int tb, int dual,
+                   int max_ngrad, void *stream);
+
+/* Sum partial rows per (item, slot) in fixed order and add into
+ * out[slot_param[slot]]; writes a completion timestamp (s_memrealtime, 100 MHz)
+ * with atomicMax when tstamp != 0.  total_blocks = Σ ng. */
+int svh_batch_reduce(const void *red_dev, int nitems, int64_t total_blocks, void *stream);
+
+/* λ = Hψ for every item and per-block partials of Re⟨ψ|λ⟩ (1024 amplitudes
+ * per block; total_blocks = Σ 2^n / 1024). */
+int svh_batch_hamiltonian(const void *ham_dev, int nitems, int64_t total_blocks, int dtype, void *stream);
+
+/* buf[i] = (i in offsets) ? 1 : 0 over a flat buffer holding many states
+ * (offsets sorted ascending, device int64). */
+int svh_batch_init(void *buf, int64_t total, const int64_t *offsets_dev, int noffsets, int dtype,
+                   void *stream);
+
+/* (cos θ, sin θ) for a concatenated θ vector, rounded to the state precision. */
+int svh_batch_trig(const double *theta_dev, int64_t np, void *trig_dev, int dtype, void *stream);
+
+/* out_dev[0] = wall clock (s_memrealtime, 100 MHz ticks), for batch latencies. */
+int svh_timestamp(uint64_t *out_dev, void *stream);
 
 /* out_dev[0..1] = ⟨a|b⟩ (re, im) accumulated in FP64. */
 int svh_inner(const void *a, const void *b, int n, int dtype, double *out_dev,

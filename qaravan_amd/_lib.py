@@ -43,6 +43,12 @@ SIGNATURES = {
                                  _c_void_p, _c_void_p, _c_size_t, _c_void_p]),
     "svh_inner": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_size_t,
                            _c_void_p]),
+    "svh_batch_tile": (_c_int, [_c_void_p, _c_int, ctypes.c_int64, _c_int, _c_int, _c_int, _c_int, _c_void_p]),
+    "svh_batch_reduce": (_c_int, [_c_void_p, _c_int, ctypes.c_int64, _c_void_p]),
+    "svh_batch_hamiltonian": (_c_int, [_c_void_p, _c_int, ctypes.c_int64, _c_int, _c_void_p]),
+    "svh_batch_init": (_c_int, [_c_void_p, ctypes.c_int64, _c_void_p, _c_int, _c_int, _c_void_p]),
+    "svh_batch_trig": (_c_int, [_c_void_p, ctypes.c_int64, _c_void_p, _c_int, _c_void_p]),
+    "svh_timestamp": (_c_int, [_c_void_p, _c_void_p]),
 }
 
 _lib = None

@@ -35,8 +35,10 @@ def _adjoint_cfg(n, dtype, cfg):
 class ParamProgram:
     """Forward and adjoint programs of a parametrised circuit plus its loss."""
 
-    def __init__(self, circ, terms, n_params, cfg=None, adj_cfg=None, dtype=torch.complex64):
-        self.n = circ.num_sites
+    def __init__(self, circ, terms, n_params, cfg=None, adj_cfg=None, dtype=torch.complex64, ops=None, n=None):
+        """``circ``: a qaravan-style Circuit whose rotations carry ``param_index``;
+        or ``circ=None`` with ``ops`` (elementary ops, e.g. workloads.hea_ops) and ``n``."""
+        self.n = circ.num_sites if circ is not None else int(n)
         if self.n < 8:
             raise NotImplementedError("the adjoint path needs n >= 8 qubits")
         self.terms = list(terms)
@@ -44,7 +46,8 @@ class ParamProgram:
         self.dtype = dtype
         cfg = cfg or TileConfig()
         adj_cfg = adj_cfg or cfg
-        ops = lower_circuit(circ if circ.built else circ.build(), self.n, param_mode=True)
+        if ops is None:
+            ops = lower_circuit(circ if circ.built else circ.build(), self.n, param_mode=True)
         for op in ops:
             if op.kind in ('dense', 'norm'):
                 raise NotImplementedError("gates on > 2 qubits and resets are not differentiable here")

@@ -45,11 +45,13 @@ def best_tile_config(n, dtype):
     if e is None:
         cfg = base
     else:
-        cfg = TileConfig(tb=e["tb"], low_bits=e["low_bits"], max_cost=e["max_cost"])
-    return TileConfig(tb=min(cfg.tb, n - 4), low_bits=cfg.low_bits, max_cost=cfg.max_cost)
+        cfg = TileConfig(tb=e["tb"], low_bits=e["low_bits"], max_cost=e["max_cost"],
+                         prefetch=e.get("prefetch", False))
+    return TileConfig(tb=min(cfg.tb, n - 4), low_bits=cfg.low_bits, max_cost=cfg.max_cost,
+                      prefetch=cfg.prefetch)
 
 
-def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print):
+def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_path=None):
     """Time every configuration in ``space`` on ``circ`` and cache the fastest."""
     import torch
 
@@ -57,16 +59,18 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print):
     from .program import compile_ops, lower_circuit
     from .statevector_sim import DeviceScratch, run_program
     n = circ.num_sites
-    space = space or dict(tb=[6, 7, 8, 9], low_bits=[4, 5, 6], max_cost=[48.0, 72.0, 96.0, 144.0, 1e9])
+    space = space or dict(tb=[6, 7, 8, 9], low_bits=[3, 4, 5, 6], max_cost=[48.0, 96.0, 160.0, 1e9],
+                          prefetch=[False, True])
     lib = _lib.load()
     ops = lower_circuit(circ.build(), n)
     psi = torch.empty(2 ** n, dtype=dtype, device=device)
     scratch = DeviceScratch(device)
     results = []
-    for tb, lb, mc in itertools.product(space["tb"], space["low_bits"], space["max_cost"]):
-        if tb + 4 > n or (dtype == torch.complex128 and tb > 9):
+    for tb, lb, mc, pf in itertools.product(space["tb"], space["low_bits"], space["max_cost"],
+                                            space.get("prefetch", [False])):
+        if tb + 4 > n or (dtype == torch.complex128 and tb > (8 if pf else 9)) or (pf and tb > 9):
             continue
-        cfg = TileConfig(tb=tb, low_bits=lb, max_cost=mc)
+        cfg = TileConfig(tb=tb, low_bits=lb, max_cost=mc, prefetch=pf)
         prog = compile_ops(ops, n, cfg)
         times = []
         for r in range(repeats + 1):
@@ -80,14 +84,14 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print):
                 times.append(time.perf_counter() - t0)
         t = min(times)
         results.append((t, cfg, prog.n_tile_passes()))
-        log(f"[tune] n={n} {dtype} tb={tb} low={lb} cap={mc:g}: {t * 1e3:.3f} ms, "
+        log(f"[tune] n={n} {dtype} tb={tb} low={lb} cap={mc:g} prefetch={pf}: {t * 1e3:.3f} ms, "
             f"{prog.n_tile_passes()} passes")
     results.sort(key=lambda x: x[0])
     t, cfg, npass = results[0]
     if write:
         cache = _load()
         cache[_key(n, dtype)] = dict(tb=cfg.tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost,
-                                     ms=t * 1e3, passes=npass)
-        with open(CACHE, "w") as f:
+                                     prefetch=cfg.prefetch, ms=t * 1e3, passes=npass)
+        with open(out_path or CACHE, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)
     return cfg, results

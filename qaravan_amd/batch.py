@@ -1,0 +1,300 @@
+"""Batched forward + adjoint gradients for many parametrised circuits.
+
+qaravan runs many small simulations one after another (e.g. the trajectories
+of tensorQ/monte_carlo_sim.py:14-35, or one StatevectorSim per optimisation
+step in applications/compilation.py:73-133).  Here a whole batch of circuits
+of uneven qubit counts and depths, each with its own θ and Pauli-sum loss, is
+one fixed launch sequence on one stream:
+
+  init (all |0…0⟩) → trig tables → forward rounds → λ = Hψ and E → adjoint rounds
+
+Round r launches pass r of every circuit that has one (one kernel per tile
+shape), so small circuits share the GPU instead of each under-filling it.  The
+sequence allocates nothing and reads nothing from the host, so it is captured
+in one HIP graph and replayed as a single launch (``capture()``/``replay()``).
+Per-circuit completion times come from the device clock (s_memrealtime,
+100 MHz) written when a circuit's last gradient reduction finishes.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .adjoint import ParamProgram
+from .program import HDR_DT, PASS_DT, PASS_TILE, RB, TileConfig
+from .statevector_sim import _pauli_masks
+
+ITEM_DT = np.dtype([('psi', '<u8'), ('lam', '<u8'), ('pass', '<u8'), ('stages', '<u8'), ('ops', '<u8'),
+                    ('trig', '<u8'), ('gpart', '<u8'), ('wg_begin', '<i8'), ('n', '<i4'), ('pad', '<i4')])
+RED_DT = np.dtype([('part', '<u8'), ('slot_param', '<u8'), ('out', '<u8'), ('tstamp', '<u8'),
+                   ('nblocks', '<i4'), ('ng', '<i4'), ('slot_begin', '<i8')])
+HAM_DT = np.dtype([('psi', '<u8'), ('lam', '<u8'), ('terms', '<u8'), ('epart', '<u8'),
+                   ('block_begin', '<i8'), ('n', '<i4'), ('nterms', '<i4')])
+TERM_DT = np.dtype([('x', '<u8'), ('z', '<u8'), ('cre', '<f8'), ('cim', '<f8')])
+assert ITEM_DT.itemsize == 72 and RED_DT.itemsize == 48 and HAM_DT.itemsize == 48
+
+
+def default_tb(n, dtype):
+    """Workgroup size per circuit: a 12-bit tile (256 threads) once the state
+    has room for it, otherwise the whole state in one tile."""
+    return max(4, min(8, n - RB))
+
+
+def _term_table(n, terms):
+    t = np.zeros(len(terms), dtype=TERM_DT)
+    xm, zm = _pauli_masks(n, [p for _, p in terms])
+    ph_re = [1.0, 0.0, -1.0, 0.0]
+    ph_im = [0.0, 1.0, 0.0, -1.0]
+    for i, (c, _) in enumerate(terms):
+        ny = bin(int(xm[i]) & int(zm[i])).count("1") & 3
+        t[i] = (xm[i], zm[i], float(c) * ph_re[ny], float(c) * ph_im[ny])
+    return t
+
+
+class BatchedAdjoint:
+    """Energies and gradients of E_c(θ_c) = ⟨ψ_c(θ_c)|H_c|ψ_c(θ_c)⟩ for a list of
+    circuits (rotations carry ``param_index`` into their own θ_c)."""
+
+    def __init__(self, circuits, terms_list, n_params_list, device="cuda", dtype=torch.complex64,
+                 tb_of_n=None, max_cost=1e9, keys=None, op_lists=None, ns=None):
+        """``circuits``: qaravan-style circuits (rotations carry ``param_index``),
+        or None with ``op_lists`` (elementary ops) and ``ns``.  Circuits with equal
+        ``keys`` share compiled programs (θ lives on the device, so programs
+        depend on the structure only)."""
+        _lib.require_gpu()
+        self.device = torch.device(device)
+        self.dtype = dtype
+        self.code = _lib.dtype_code(dtype)
+        tb_of_n = tb_of_n or (lambda n: default_tb(n, dtype))
+        nc = len(circuits) if circuits is not None else len(op_lists)
+        self.nc = nc
+        self.ns = [c.num_sites for c in circuits] if circuits is not None else [int(n) for n in ns]
+        if min(self.ns) < 10:
+            raise NotImplementedError("batched Hamiltonian kernel works on blocks of 1024 amplitudes: n >= 10")
+        self.pps = []
+        cache = {}
+        for c in range(nc):
+            key = None if keys is None else keys[c]
+            if key is not None and key in cache:
+                self.pps.append(cache[key])
+                continue
+            n = self.ns[c]
+            cfg = TileConfig(tb=tb_of_n(n), low_bits=4, max_cost=max_cost)
+            if circuits is not None:
+                pp = ParamProgram(circuits[c], terms_list[c], n_params_list[c], cfg=cfg, dtype=dtype)
+            else:
+                pp = ParamProgram(None, terms_list[c], n_params_list[c], cfg=cfg, dtype=dtype,
+                                  ops=op_lists[c], n=n)
+            if key is not None:
+                cache[key] = pp
+            self.pps.append(pp)
+        es = 8 if dtype == torch.complex64 else 16
+        # ---- flat device buffers
+        self.amp_off = np.concatenate([[0], np.cumsum([2 ** n for n in self.ns])]).astype(np.int64)
+        self.par_off = np.concatenate([[0], np.cumsum(n_params_list)]).astype(np.int64)
+        total = int(self.amp_off[-1])
+        dev = self.device
+        self.psi = torch.empty(total, dtype=dtype, device=dev)
+        self.lam = torch.empty(total, dtype=dtype, device=dev)
+        self.theta = torch.zeros(int(self.par_off[-1]), dtype=torch.float64, device=dev)
+        self.trig = torch.empty(int(self.par_off[-1]), dtype=dtype, device=dev)
+        self.grad = torch.zeros(int(self.par_off[-1]), dtype=torch.float64, device=dev)
+        self.energy = torch.zeros(nc, dtype=torch.float64, device=dev)
+        self.t0 = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.t_end = torch.zeros(nc, dtype=torch.int64, device=dev)
+        self.offsets = torch.tensor(self.amp_off[:-1], dtype=torch.int64, device=dev)
+        self.cidx = torch.arange(nc, dtype=torch.int32, device=dev)
+        # programs: one device blob holding every distinct forward and adjoint program
+        blobs, where = [], {}
+        off = 0
+        for pp in self.pps:
+            for prog in (pp.fwd, pp.adj):
+                if id(prog) not in where:
+                    where[id(prog)] = off
+                    blobs.append((off, prog.blob))
+                    off += (len(prog.blob) + 255) & ~255
+        host = np.zeros(off, dtype=np.uint8)
+        for o, b in blobs:
+            host[o:o + len(b)] = b
+        self.progs = torch.from_numpy(host).to(dev)
+        pbase = self.progs.data_ptr()
+        self.prog_addr = [(pbase + where[id(pp.fwd)], pbase + where[id(pp.adj)]) for pp in self.pps]
+        # gradient partial buffers: the largest adjoint pass of each circuit
+        gp_sizes = []
+        for c, pp in enumerate(self.pps):
+            m = 1
+            for p in pp.adj.passes:
+                if p.kind == PASS_TILE:
+                    ng = sum(1 for st in p.stages for op, _ in st.ops if op.flags & 2)
+                    m = max(m, (2 ** (self.ns[c] - p.tb - RB)) * max(ng, 1))
+            gp_sizes.append(m)
+        self.gp_off = np.concatenate([[0], np.cumsum(gp_sizes)]).astype(np.int64)
+        self.gpart = torch.zeros(int(self.gp_off[-1]), dtype=torch.float64, device=dev)
+        self.ep_off = np.concatenate([[0], np.cumsum([2 ** n // 1024 for n in self.ns])]).astype(np.int64)
+        self.epart = torch.zeros(int(self.ep_off[-1]), dtype=torch.float64, device=dev)
+        tt = [_term_table(self.ns[c], terms_list[c]) for c in range(nc)]
+        self.t_off = np.concatenate([[0], np.cumsum([len(t) for t in tt])]).astype(np.int64)
+        self.terms = torch.from_numpy(np.concatenate(tt).view(np.uint8).copy()).to(dev)
+        self._plan(es)
+        self.graph = None
+
+    # ---------------------------------------------------------------- plan
+    def _pass_ptrs(self, prog_addr, prog, pi):
+        h = np.frombuffer(prog.blob[:HDR_DT.itemsize].tobytes(), dtype=HDR_DT)[0]
+        return (prog_addr + int(h['off_passes']) + pi * PASS_DT.itemsize,
+                prog_addr + int(h['off_stages']), prog_addr + int(h['off_ops']),
+                prog_addr + int(h['off_grad']))
+
+    def _plan(self, es):
+        nc = self.nc
+        psi0, lam0 = self.psi.data_ptr(), self.lam.data_ptr()
+        trig0, gp0 = self.trig.data_ptr(), self.gpart.data_ptr()
+        tables = []          # numpy arrays appended to one device blob
+        self.launches = []   # (kind, args...) with table offsets resolved later
+
+        def add_table(arr):
+            off = sum((t.nbytes + 255) & ~255 for t in tables)
+            tables.append(arr)
+            return off
+
+        def rounds(which):
+            progs = [pp.fwd if which == 0 else pp.adj for pp in self.pps]
+            R = max(len(p.passes) for p in progs)
+            for r in range(R):
+                groups = {}
+                reds = []
+                for c in range(nc):
+                    prog = progs[c]
+                    if r >= len(prog.passes):
+                        continue
+                    p = prog.passes[r]
+                    tb = p.tb
+                    L = tb + RB
+                    n = self.ns[c]
+                    ppass, pst, pops, pgrad = self._pass_ptrs(self.prog_addr[c][which], prog, r)
+                    ng = sum(1 for st in p.stages for op, _ in st.ops if op.flags & 2) if which else 0
+                    groups.setdefault(tb, []).append(dict(
+                        psi=psi0 + es * int(self.amp_off[c]), lam=(lam0 + es * int(self.amp_off[c])) if which else 0,
+                        pass_=ppass, stages=pst, ops=pops, trig=trig0 + es * int(self.par_off[c]),
+                        gpart=gp0 + 8 * int(self.gp_off[c]), nwg=2 ** (n - L), n=n, ng=ng))
+                    if which and ng > 0:
+                        blob = progs[c].blob
+                        h = np.frombuffer(blob[:HDR_DT.itemsize].tobytes(), dtype=HDR_DT)[0]
+                        pr = np.frombuffer(blob[int(h['off_passes']) + r * PASS_DT.itemsize:
+                                                int(h['off_passes']) + (r + 1) * PASS_DT.itemsize].tobytes(),
+                                           dtype=PASS_DT)[0]
+                        last = r == len(prog.passes) - 1
+                        reds.append(dict(part=gp0 + 8 * int(self.gp_off[c]),
+                                         slot_param=pgrad + 4 * int(pr['grad_begin']),
+                                         out=self.grad.data_ptr() + 8 * int(self.par_off[c]),
+                                         tstamp=(self.t_end.data_ptr() + 8 * c) if last else 0,
+                                         nblocks=2 ** (n - L), ng=ng))
+                for tb, items in sorted(groups.items()):
+                    arr = np.zeros(len(items), dtype=ITEM_DT)
+                    wg = 0
+                    for i, it in enumerate(items):
+                        arr[i] = (it['psi'], it['lam'], it['pass_'], it['stages'], it['ops'], it['trig'],
+                                  it['gpart'], wg, it['n'], 0)
+                        wg += it['nwg']
+                    mng = max(it['ng'] for it in items)
+                    self.launches.append(('tile', add_table(arr), len(items), wg, tb, which, mng))
+                if reds:
+                    arr = np.zeros(len(reds), dtype=RED_DT)
+                    sb = 0
+                    for i, rd in enumerate(reds):
+                        arr[i] = (rd['part'], rd['slot_param'], rd['out'], rd['tstamp'], rd['nblocks'], rd['ng'], sb)
+                        sb += rd['ng']
+                    self.launches.append(('reduce', add_table(arr), len(reds), sb))
+
+        rounds(0)
+        # λ = Hψ and energies
+        ham = np.zeros(nc, dtype=HAM_DT)
+        bb = 0
+        tb0 = self.terms.data_ptr()
+        for c in range(nc):
+            n = self.ns[c]
+            ham[c] = (psi0 + es * int(self.amp_off[c]), lam0 + es * int(self.amp_off[c]),
+                      tb0 + 32 * int(self.t_off[c]), self.epart.data_ptr() + 8 * int(self.ep_off[c]), bb, n,
+                      int(self.t_off[c + 1] - self.t_off[c]))
+            bb += 2 ** n // 1024
+        self.launches.append(('ham', add_table(ham), nc, bb))
+        red = np.zeros(nc, dtype=RED_DT)
+        for c in range(nc):
+            red[c] = (self.epart.data_ptr() + 8 * int(self.ep_off[c]), self.cidx.data_ptr() + 4 * c,
+                      self.energy.data_ptr(), 0, 2 ** self.ns[c] // 1024, 1, c)
+        self.launches.append(('reduce', add_table(red), nc, nc))
+        rounds(1)
+        size = sum((t.nbytes + 255) & ~255 for t in tables)
+        blob = np.zeros(max(size, 256), dtype=np.uint8)
+        off = 0
+        for t in tables:
+            blob[off:off + t.nbytes] = np.frombuffer(t.tobytes(), dtype=np.uint8)
+            off += (t.nbytes + 255) & ~255
+        self.tables = torch.from_numpy(blob).to(self.device)
+        self.n_launches = len(self.launches) + 3
+
+    # ---------------------------------------------------------------- run
+    def _launch_all(self):
+        lib = _lib.load()
+        st = _lib.stream_ptr(self.device)
+        tb0 = self.tables.data_ptr()
+        self.grad.zero_()
+        self.energy.zero_()
+        self.t_end.zero_()
+        _lib.check(lib.svh_timestamp(_lib.ptr(self.t0), st), "svh_timestamp")
+        _lib.check(lib.svh_batch_init(_lib.ptr(self.psi), self.psi.numel(), _lib.ptr(self.offsets), self.nc,
+                                      self.code, st), "svh_batch_init")
+        _lib.check(lib.svh_batch_trig(_lib.ptr(self.theta), self.theta.numel(), _lib.ptr(self.trig), self.code, st),
+                   "svh_batch_trig")
+        for L in self.launches:
+            if L[0] == 'tile':
+                _, off, ni, wg, tb, dual, mng = L
+                _lib.check(lib.svh_batch_tile(tb0 + off, ni, wg, self.code, tb, dual, mng, st), "svh_batch_tile")
+            elif L[0] == 'reduce':
+                _, off, ni, nb = L
+                _lib.check(lib.svh_batch_reduce(tb0 + off, ni, nb, st), "svh_batch_reduce")
+            elif L[0] == 'ham':
+                _, off, ni, nb = L
+                _lib.check(lib.svh_batch_hamiltonian(tb0 + off, ni, nb, self.code, st), "svh_batch_hamiltonian")
+
+    def set_theta(self, thetas):
+        """θ for every circuit (list of arrays/tensors, or one flat tensor)."""
+        if isinstance(thetas, torch.Tensor) and thetas.dim() == 1 and thetas.numel() == self.theta.numel():
+            self.theta.copy_(thetas.to(torch.float64))
+        else:
+            flat = np.concatenate([np.asarray(t.detach().cpu() if hasattr(t, "detach") else t, dtype=np.float64)
+                                   for t in thetas])
+            self.theta.copy_(torch.from_numpy(flat))
+
+    def run(self):
+        """Eager run of the whole batch; returns (energies, flat gradient)."""
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self._launch_all()
+        return self.energy, self.grad
+
+    def capture(self):
+        """Record the whole batch into one HIP graph (after one eager warm-up
+        run, so every kernel's one-time setup is done outside the capture)."""
+        self._launch_all()
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self._launch_all()
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        torch.cuda.synchronize(self.device)
+        self.graph = g
+        return g
+
+    def gradients(self):
+        g = self.grad.cpu().numpy()
+        return [g[self.par_off[c]:self.par_off[c + 1]] for c in range(self.nc)]
+
+    def latencies_us(self):
+        """Per-circuit completion time since the batch started (device clock)."""
+        t0 = int(self.t0.item())
+        te = self.t_end.cpu().numpy().astype(np.int64)
+        return (te - t0) / 100.0  # 100 MHz ticks -> µs

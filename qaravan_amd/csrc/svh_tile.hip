@@ -314,12 +314,14 @@ __device__ __forceinline__ uint64_t thr_global(const svh_stage_t &S, int tid) {
 }
 
 // ------------------------------------------------ the pass kernel
+// One tile of one pass: workgroup-local work shared by k_tile (one circuit,
+// tile = blockIdx.x) and k_tile_batch (many circuits in one launch).
 template <typename R, int TB, int RB, bool DUAL>
-__global__ void __launch_bounds__(1 << TB)
-k_tile(cpx<R> *__restrict__ psi, cpx<R> *__restrict__ lam, int n,
-       const svh_pass_t *__restrict__ pass, const svh_stage_t *__restrict__ stages,
-       const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig,
-       double *__restrict__ gpart) {
+__device__ __forceinline__ void tile_body(cpx<R> *__restrict__ psi, cpx<R> *__restrict__ lam, int n,
+                                          const svh_pass_t *__restrict__ pass,
+                                          const svh_stage_t *__restrict__ stages,
+                                          const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig,
+                                          double *__restrict__ gpart, uint64_t wg) {
     static_assert(RB == 4, "stage tables hold 16 registers");
     constexpr int L = TB + RB;
     constexpr int NR = 1 << RB;
@@ -339,7 +341,7 @@ k_tile(cpx<R> *__restrict__ psi, cpx<R> *__restrict__ lam, int n,
     for (int i = 0; i < L; ++i) tmask |= 1ull << pass->bits[i];
     uint64_t base = 0;
     {
-        uint64_t g = blockIdx.x;
+        uint64_t g = wg;
         for (int b = 0; b < n && g; ++b) {
             if (!((tmask >> b) & 1)) {
                 base |= (g & 1ull) << b;
@@ -418,8 +420,141 @@ k_tile(cpx<R> *__restrict__ psi, cpx<R> *__restrict__ lam, int n,
             for (int sl = tid; sl < ng; sl += (1 << TB)) {
                 double acc = 0.0;
                 for (int wv = 0; wv < NW; ++wv) acc += gacc[wv * ng + sl];
-                gpart[(uint64_t)blockIdx.x * ng + sl] = acc;
+                gpart[wg * ng + sl] = acc;
             }
+        }
+    }
+}
+
+template <typename R, int TB, int RB, bool DUAL>
+__global__ void __launch_bounds__(1 << TB)
+k_tile(cpx<R> *__restrict__ psi, cpx<R> *__restrict__ lam, int n,
+       const svh_pass_t *__restrict__ pass, const svh_stage_t *__restrict__ stages,
+       const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig,
+       double *__restrict__ gpart) {
+    tile_body<R, TB, RB, DUAL>(psi, lam, n, pass, stages, ops, trig, gpart, blockIdx.x);
+}
+
+// Many circuits' passes in one launch: workgroup b belongs to the item whose
+// [wg_begin, next wg_begin) range holds b (items sorted by wg_begin).
+template <typename R, int TB, int RB, bool DUAL>
+__global__ void __launch_bounds__(1 << TB)
+k_tile_batch(const svh_batch_item_t *__restrict__ items, int nitems) {
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {  // last item with wg_begin <= b
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].wg_begin <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const svh_batch_item_t &it = items[lo];
+    tile_body<R, TB, RB, DUAL>(reinterpret_cast<cpx<R> *>(it.psi), reinterpret_cast<cpx<R> *>(it.lam), it.n,
+                               reinterpret_cast<const svh_pass_t *>(it.pass),
+                               reinterpret_cast<const svh_stage_t *>(it.stages),
+                               reinterpret_cast<const svh_op_t *>(it.ops),
+                               reinterpret_cast<const cpx<R> *>(it.trig), reinterpret_cast<double *>(it.gpart),
+                               (uint64_t)(b - it.wg_begin));
+}
+
+// ------------------------------------------------ prefetching forward pass
+// Persistent variant of k_tile for forward passes: each workgroup walks tiles
+// t = blockIdx.x, blockIdx.x + gridDim.x, ...  While tile t is computed in
+// registers (exchanges through the swizzled buffer `lds`), tile t + gridDim.x
+// is already streaming HBM -> LDS into `pf` with global_load_lds_dwordx4 (16 B
+// per lane, no VGPRs held), so the memory system stays busy during the ALU
+// work.  Stage 0 reads its registers straight from `pf` (tile-local order).
+template <typename R, int TB>
+__global__ void __launch_bounds__(1 << TB)
+k_tile_pf(cpx<R> *__restrict__ psi, int n, const svh_pass_t *__restrict__ pass,
+          const svh_stage_t *__restrict__ stages, const svh_op_t *__restrict__ ops,
+          const cpx<R> *__restrict__ trig, uint64_t ntiles) {
+    constexpr int RB = 4;
+    constexpr int L = TB + RB;
+    constexpr int NR = 1 << RB;
+    constexpr int NT = 1 << TB;
+    constexpr int CH = 16 / (int)sizeof(cpx<R>);  // amplitudes per 16-byte chunk
+    constexpr int NCH = (1 << L) / CH / NT;        // chunks per thread per tile
+    extern __shared__ unsigned char smem_raw[];
+    cpx<R> *pf = reinterpret_cast<cpx<R> *>(smem_raw);
+    cpx<R> *lds = pf + (1 << L);
+
+    const int tid = threadIdx.x;
+    const int n_stages = pass->n_stages;
+    const int s0 = pass->stage_begin;
+    uint64_t tmask = 0;
+#pragma unroll
+    for (int i = 0; i < L; ++i) tmask |= 1ull << pass->bits[i];
+    // global offset of tile element li (deposit of li's bits into the tile bits)
+    auto gdep = [&](uint64_t li) {
+        uint64_t a = 0;
+#pragma unroll
+        for (int p = 0; p < L; ++p) a |= ((li >> p) & 1ull) << pass->bits[p];
+        return a;
+    };
+    auto tile_base = [&](uint64_t t) {
+        uint64_t base = 0, g = t;
+        for (int b = 0; b < n && g; ++b) {
+            if (!((tmask >> b) & 1)) {
+                base |= (g & 1ull) << b;
+                g >>= 1;
+            }
+        }
+        return base;
+    };
+    const uint64_t gthr = gdep((uint64_t)CH * tid);
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    auto prefetch = [&](uint64_t t) {
+        const cpx<R> *src = psi + (tile_base(t) | gthr);
+#pragma unroll
+        for (int k = 0; k < NCH; ++k) {
+            const uint64_t gk = gdep((uint64_t)CH * k * NT);
+            char *dst = reinterpret_cast<char *>(pf) + 16 * (k * NT + 64 * wv);
+            __builtin_amdgcn_global_load_lds((const void *)(src + gk),
+                                             (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+        }
+    };
+
+    cpx<R> v[NR];
+    cpx<R> w[1];
+    uint64_t t = blockIdx.x;
+    if (t < ntiles) prefetch(t);
+    for (; t < ntiles; t += gridDim.x) {
+        const uint64_t base = tile_base(t);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();  // tile t is in pf for every wave
+        {
+            const svh_stage_t &S = stages[s0];
+            const int tl = thr_local(S, tid);
+#pragma unroll
+            for (int j = 0; j < NR; ++j) v[j] = pf[tl | lds_swz(S.lsw[j])];
+        }
+        __syncthreads();  // everyone has read pf
+        if (t + gridDim.x < ntiles) prefetch(t + gridDim.x);
+
+        for (int s = 0; s < n_stages; ++s) {
+            const svh_stage_t &S = stages[s0 + s];
+            if (s > 0) {
+                const svh_stage_t &P = stages[s0 + s - 1];
+                const int tp = lds_swz(thr_local(P, tid));
+                if (s > 1) __syncthreads();
+#pragma unroll
+                for (int j = 0; j < NR; ++j) lds[tp ^ P.lsw[j]] = v[j];
+                __syncthreads();
+                const int tn = lds_swz(thr_local(S, tid));
+#pragma unroll
+                for (int j = 0; j < NR; ++j) v[j] = lds[tn ^ S.lsw[j]];
+            }
+            for (int o = S.op_begin; o < S.op_end; ++o) {
+                double g = 0.0;
+                bool has_g = false;
+                apply_op<R, RB, false>(v, w, ops[o], trig, &g, &has_g);
+            }
+        }
+        {
+            const svh_stage_t &S = stages[s0 + n_stages - 1];
+            cpx<R> *p = psi + (base | thr_global<TB>(S, tid));
+#pragma unroll
+            for (int j = 0; j < NR; ++j) p[S.goff[j]] = v[j];
         }
     }
 }
@@ -454,6 +589,36 @@ k_grad_reduce(const double *__restrict__ gpart, int nblocks, int ng, const int32
 
 // ------------------------------------------------ host launch
 constexpr size_t kMaxLds = 160 * 1024;
+
+template <typename R, int TB>
+static int launch_tb_pf(cpx<R> *psi, int n, const svh_pass_t &hp, const svh_pass_t *dp, const svh_stage_t *st,
+                        const svh_op_t *op, const cpx<R> *trig, hipStream_t stream) {
+    constexpr int L = TB + 4;
+    const size_t lds = 2 * sizeof(cpx<R>) * ((size_t)1 << L);
+    if (lds > kMaxLds) return fail(-7, "prefetch tile pass needs " + std::to_string(lds) + " B of LDS");
+    auto kern = k_tile_pf<R, TB>;
+    static int resident = 0;  // workgroups per CU, from the occupancy calculator
+    static int cus = 0;
+    if (!resident) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)kMaxLds);
+        if (e != hipSuccess) return hip_status(e, "hipFuncSetAttribute(k_tile_pf)");
+        int nb = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)kern, 1 << TB, lds);
+        if (e != hipSuccess) return hip_status(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        int dev = 0;
+        if (int r = hip_status(hipGetDevice(&dev), "hipGetDevice")) return r;
+        if (int r = hip_status(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                               "hipDeviceGetAttribute"))
+            return r;
+        resident = nb > 0 ? nb : 1;
+    }
+    const uint64_t ntiles = 1ull << (n - L);
+    const uint64_t cap = (uint64_t)resident * (uint64_t)(cus > 0 ? cus : 256);
+    const uint64_t grid = ntiles < cap ? ntiles : cap;
+    kern<<<(unsigned)grid, 1 << TB, lds, stream>>>(psi, n, dp, st, op, trig, ntiles);
+    return checked_launch("k_tile_pf");
+}
 
 template <typename R, int TB, bool DUAL>
 static int launch_tb(cpx<R> *psi, cpx<R> *lam, int n, const svh_pass_t &hp, const svh_pass_t *dp,
@@ -493,6 +658,19 @@ static int launch_tile(cpx<R> *psi, cpx<R> *lam, int n, const svh_pass_t &hp, co
     if (hp.tb < tb_min || hp.tb > tb_max)
         return fail(-7, "tile pass tb=" + std::to_string(hp.tb) + " outside [" + std::to_string(tb_min) + ", " +
                             std::to_string(tb_max) + "] for this dtype/mode");
+    if constexpr (!DUAL) {
+        if (hp.flags & SVH_PASSF_PREFETCH) {
+            if (hp.bits[0] != 0) return fail(-7, "prefetch passes need address bit 0 in the tile");
+            switch (hp.tb) {
+                case 6: return launch_tb_pf<R, 6>(psi, n, hp, dp, st, op, trig, stream);
+                case 7: return launch_tb_pf<R, 7>(psi, n, hp, dp, st, op, trig, stream);
+                case 8: return launch_tb_pf<R, 8>(psi, n, hp, dp, st, op, trig, stream);
+                case 9: if constexpr (C64) return launch_tb_pf<R, 9>(psi, n, hp, dp, st, op, trig, stream); break;
+                default: break;
+            }
+            return fail(-7, "prefetch tile pass tb must be in [6, 8] (c128) or [6, 9] (c64)");
+        }
+    }
     switch (hp.tb) {
         case 4: return launch_tb<R, 4, DUAL>(psi, lam, n, hp, dp, st, op, trig, gpart, stream);
         case 5: return launch_tb<R, 5, DUAL>(psi, lam, n, hp, dp, st, op, trig, gpart, stream);
@@ -713,7 +891,62 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
     return 0;
 }
 
+template <typename R, int TB, bool DUAL>
+static int launch_batch_tb(const svh_batch_item_t *items, int nitems, int64_t total_wg, int max_ngrad,
+                           hipStream_t stream) {
+    constexpr int L = TB + 4;
+    constexpr int NW = ((1 << TB) + 63) / 64;
+    const size_t lds = (DUAL ? 2 : 1) * sizeof(cpx<R>) * ((size_t)1 << L) +
+                       (DUAL ? sizeof(double) * NW * (size_t)max_ngrad : 0);
+    if (lds > kMaxLds) return fail(-7, "batched tile pass needs " + std::to_string(lds) + " B of LDS");
+    auto kern = k_tile_batch<R, TB, 4, DUAL>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)kMaxLds);
+        if (e != hipSuccess) return hip_status(e, "hipFuncSetAttribute(k_tile_batch)");
+        attr_set = true;
+    }
+    if (total_wg <= 0 || total_wg > 0x7fffffff) return fail(-4, "bad workgroup count");
+    kern<<<(unsigned)total_wg, 1 << TB, lds, stream>>>(items, nitems);
+    return checked_launch("k_tile_batch");
+}
+
+template <typename R, bool DUAL>
+static int launch_batch(const svh_batch_item_t *items, int nitems, int64_t total_wg, int tb, int max_ngrad,
+                        hipStream_t stream) {
+    constexpr bool C64 = std::is_same<R, float>::value;
+    constexpr int tb_max = DUAL ? (C64 ? 9 : 8) : (C64 ? 10 : 9);
+    if (tb < 4 || tb > tb_max) return fail(-7, "batched tile pass: tb outside the built range");
+    switch (tb) {
+        case 4: return launch_batch_tb<R, 4, DUAL>(items, nitems, total_wg, max_ngrad, stream);
+        case 5: return launch_batch_tb<R, 5, DUAL>(items, nitems, total_wg, max_ngrad, stream);
+        case 6: return launch_batch_tb<R, 6, DUAL>(items, nitems, total_wg, max_ngrad, stream);
+        case 7: return launch_batch_tb<R, 7, DUAL>(items, nitems, total_wg, max_ngrad, stream);
+        case 8: return launch_batch_tb<R, 8, DUAL>(items, nitems, total_wg, max_ngrad, stream);
+        case 9: if constexpr (tb_max >= 9) return launch_batch_tb<R, 9, DUAL>(items, nitems, total_wg, max_ngrad, stream); break;
+        case 10: if constexpr (tb_max >= 10) return launch_batch_tb<R, 10, DUAL>(items, nitems, total_wg, max_ngrad, stream); break;
+        default: break;
+    }
+    return fail(-7, "batched tile pass: tb not built");
+}
+
 extern "C" {
+
+int svh_batch_tile(const void *items_dev, int nitems, int64_t total_wg, int dtype, int tb, int dual,
+                   int max_ngrad, void *stream) {
+    if (!items_dev || nitems <= 0) return fail(-1, "empty batch item table");
+    if (max_ngrad < 0) return fail(-4, "bad max_ngrad");
+    const auto *it = static_cast<const svh_batch_item_t *>(items_dev);
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == SVH_C64)
+        return dual ? launch_batch<float, true>(it, nitems, total_wg, tb, max_ngrad, st)
+                    : launch_batch<float, false>(it, nitems, total_wg, tb, max_ngrad, st);
+    if (dtype == SVH_C128)
+        return dual ? launch_batch<double, true>(it, nitems, total_wg, tb, max_ngrad, st)
+                    : launch_batch<double, false>(it, nitems, total_wg, tb, max_ngrad, st);
+    return fail(-3, "bad dtype");
+}
 
 int svh_program_scratch_bytes(const void *prog_host, int dtype, size_t *bytes) {
     if (!bytes) return fail(-1, "bytes pointer is NULL");

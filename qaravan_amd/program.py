@@ -31,6 +31,7 @@ import numpy as np
 MAGIC = 0x53564850
 ABI_VERSION = 1
 PASS_TILE, PASS_DENSE, PASS_NORMALIZE = 1, 2, 3
+PASSF_PREFETCH = 1
 OP_MAT1, OP_MAT2, OP_CX, OP_CZ, OP_ROT1, OP_ROT2 = 1, 2, 3, 4, 5, 6
 OPF_NEG, OPF_GRAD = 1, 2
 RB = 4
@@ -43,14 +44,15 @@ HDR_DT = np.dtype([('magic', '<u4'), ('version', '<i4'), ('n_qubits', '<i4'), ('
                    ('reserved', '<i8', (2,))])
 PASS_DT = np.dtype([('kind', '<i4'), ('L', '<i4'), ('tb', '<i4'), ('rb', '<i4'),
                     ('stage_begin', '<i4'), ('n_stages', '<i4'), ('grad_begin', '<i4'),
-                    ('n_grad', '<i4'), ('pool_off', '<i8'), ('bits', '<i4', (MAX_TILE_BITS,))])
+                    ('n_grad', '<i4'), ('pool_off', '<i8'), ('flags', '<i4'), ('pad', '<i4'),
+                    ('bits', '<i4', (MAX_TILE_BITS,))])
 STAGE_DT = np.dtype([('goff', '<i8', (16,)), ('lsw', '<i4', (16,)), ('rbits', '<i4', (8,)),
                      ('rsorted', '<i4', (4,)), ('tgbit', '<i4', (12,)), ('op_begin', '<i4'),
                      ('op_end', '<i4'), ('pad', '<i4', (2,))])
 OP_DT = np.dtype([('kind', '<i4'), ('a', '<i4'), ('b', '<i4'), ('param', '<i4'), ('pauli', '<i4'),
                   ('flags', '<i4'), ('grad_slot', '<i4'), ('pad', '<i4'), ('m', '<f8', (32,)),
                   ('mf', '<f4', (32,))])
-assert HDR_DT.itemsize == 96 and PASS_DT.itemsize == 104
+assert HDR_DT.itemsize == 96 and PASS_DT.itemsize == 112
 assert STAGE_DT.itemsize == 304 and OP_DT.itemsize == 416
 
 
@@ -220,6 +222,7 @@ class Pass:
     tb: int = 0
     stages: list = field(default_factory=list)
     mat: np.ndarray = None
+    flags: int = 0
 
 
 @dataclass
@@ -229,11 +232,14 @@ class TileConfig:
     tb: log2 threads per workgroup (tile L = tb + 4 bits);
     low_bits: lowest address bits every tile contains (contiguous runs of
               2^low_bits amplitudes per load);
-    max_cost: cap on Σ per-amplitude VALU cost of the ops in one pass.
+    max_cost: cap on Σ per-amplitude VALU cost of the ops in one pass;
+    prefetch: persistent workgroups that copy the next tile HBM -> LDS while
+              computing the current one (forward passes only).
     """
     tb: int = 8
     low_bits: int = 6
     max_cost: float = 96.0
+    prefetch: bool = False
 
 
 def _is_barrier(op):
@@ -287,7 +293,8 @@ def partition(ops, n, cfg):
                 T.add(x)
             x += 1
         tset = set(taken)
-        p = Pass(PASS_TILE, sorted(T), [rem[i] for i in taken], tb=cfg.tb)
+        p = Pass(PASS_TILE, sorted(T), [rem[i] for i in taken], tb=cfg.tb,
+                 flags=PASSF_PREFETCH if cfg.prefetch else 0)
         passes.append(p)
         rem = [op for i, op in enumerate(rem) if i not in tset]
     return passes
@@ -395,6 +402,7 @@ def encode(n, passes, grad=False, n_params=0):
         pr = pass_rows[pi]
         pr['kind'] = p.kind
         pr['rb'] = RB
+        pr['flags'] = p.flags
         if p.kind == PASS_TILE:
             L = p.tb + RB
             pr['L'] = L
@@ -507,7 +515,8 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
                 passes.append(Pass(PASS_DENSE, list(_dense_bits(op)), [op], mat=_dense_mat(op)))
         return encode(n, passes, grad, n_params)
     tb = min(cfg.tb, n - RB)
-    cfg = TileConfig(tb=tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost)
+    cfg = TileConfig(tb=tb, low_bits=max(1, cfg.low_bits), max_cost=cfg.max_cost,
+                     prefetch=cfg.prefetch and not grad)
     passes = partition(ops, n, cfg)
     for p in passes:
         if p.kind == PASS_TILE:

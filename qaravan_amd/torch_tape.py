@@ -37,9 +37,9 @@ def op_action(op, qubits, state):
 
 def rotation_matrix(word, theta):
     """torch.linalg.matrix_exp(-1j·θ·P) in complex128 (param_gates.py:55-76)."""
-    p = _pauli(word[0])
+    p = _pauli(word[0], theta.device)
     for ch in word[1:]:
-        p = torch.kron(p, _pauli(ch))
+        p = torch.kron(p, _pauli(ch, theta.device))
     return torch.linalg.matrix_exp(-1j * theta.to(torch.complex128) * p)
 
 
@@ -56,7 +56,7 @@ def tape_energy(circ, theta, terms, device):
             if getattr(g, "param_index", None) is not None:
                 mat = rotation_matrix(g.generator_word, theta[g.param_index])
             else:
-                mat = torch.as_tensor(g.matrix, dtype=torch.complex128)
+                mat = torch.as_tensor(g.matrix, dtype=torch.complex128, device=theta.device)
             mat = mat.to(dtype=torch.complex64, device=device)
             state = op_action(mat, list(g.indices), state)
     e = torch.zeros((), dtype=torch.float32, device=device)
@@ -69,8 +69,12 @@ def tape_energy(circ, theta, terms, device):
     return e
 
 
-def tape_energy_and_grad(circ, theta_cpu, terms, device):
-    th = theta_cpu.detach().clone().requires_grad_(True)
+def tape_energy_and_grad(circ, theta, terms, device):
+    """Value and gradient by ``loss.backward()``.  ``theta`` on the CPU is the
+    reference's setup (host Parameter, host matrix_exp, one small host->device
+    copy per gate); ``theta`` on the GPU keeps the whole tape on the device,
+    the strongest form of the tape baseline."""
+    th = theta.detach().clone().requires_grad_(True)
     e = tape_energy(circ, th, terms, device)
     e.backward()
     return e.detach(), th.grad.detach()

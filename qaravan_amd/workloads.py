@@ -89,6 +89,21 @@ def hea_circuit(n, layers, theta):
     return Circuit(gates, n=n)
 
 
+def hea_ops(n, layers):
+    """The hardware-efficient ansatz of ``hea_circuit`` as elementary ops
+    (device-angle rotations and CZ), built without gate objects: the same op
+    list ``lower_circuit(hea_circuit(...), param_mode=True)`` produces."""
+    from .program import EOp, PAULI_CODE
+    ops = []
+    for l in range(layers):
+        for k, w in enumerate("yz"):
+            for q in range(n):
+                ops.append(EOp('rot1', (n - 1 - q,), param=(2 * l + k) * n + q, pauli=PAULI_CODE[w]))
+        for i in range(n - 1):
+            ops.append(EOp('cz', (n - 1 - i, n - 2 - i)))
+    return ops
+
+
 def tfim_terms(n):
     """(coefficient, Pauli string) list of TFI(n, jz=-1, h=-1)."""
     h = TFI(n, jz=-1, h=-1)

@@ -102,17 +102,20 @@ def random_circuit(n, n_gates, rng, k3=False):
 
 CFGS = [P.TileConfig(tb=4, low_bits=6, max_cost=96), P.TileConfig(tb=6, low_bits=4, max_cost=32),
         P.TileConfig(tb=8, low_bits=6, max_cost=96), P.TileConfig(tb=9, low_bits=6, max_cost=1e9),
-        P.TileConfig(tb=10, low_bits=5, max_cost=64)]
+        P.TileConfig(tb=10, low_bits=5, max_cost=64),
+        P.TileConfig(tb=6, low_bits=1, max_cost=48, prefetch=True),
+        P.TileConfig(tb=8, low_bits=4, max_cost=1e9, prefetch=True),
+        P.TileConfig(tb=9, low_bits=3, max_cost=96, prefetch=True)]
 
 
 @pytest.mark.parametrize("ci", range(len(CFGS)))
 @pytest.mark.parametrize("dtype", [C64, C128])
 def test_random_circuits_vs_oracle(gpu, ci, dtype):
     cfg = CFGS[ci]
-    if dtype == C128 and cfg.tb > 9:
-        pytest.skip("c128 tiles stop at 512 threads")
+    if dtype == C128 and (cfg.tb > 9 or (cfg.prefetch and cfg.tb > 8)):
+        pytest.skip("c128 tiles stop at 512 threads (256 with prefetch)")
     rng = np.random.default_rng(1000 + ci)
-    n = max(cfg.tb + 4, 10) + 2
+    n = max(cfg.tb + 4, 10) + 2 + (ci % 3)
     circ = random_circuit(n, 120, rng, k3=True)
     print(f"random n={n} cfg={cfg} dtype={dtype}", flush=True)
     _, sv = sim_state(circ, gpu, dtype, cfg)
@@ -172,14 +175,16 @@ def test_expectations_vs_oracle(gpu):
     assert abs(sim.local_expectation(ops) - np.real(np.vdot(ref, full))) < 1e-12
 
 
-def test_brickwork_24q_c64_vs_c128_and_inverse(gpu):
+@pytest.mark.parametrize("prefetch", [False, True])
+def test_brickwork_24q_c64_vs_c128_and_inverse(gpu, prefetch):
     """Size-independent checks at 24 qubits: c64 agrees with c128, the norm is
     kept, and running the circuit then its inverse returns |0…0⟩."""
     circ = brickwork_circuit(24, 12)
-    _, a = sim_state(circ, gpu, C128)
-    _, b = sim_state(circ, gpu, C64)
+    cfg = P.TileConfig(tb=8, low_bits=4, max_cost=1e9, prefetch=prefetch)
+    _, a = sim_state(circ, gpu, C128, cfg)
+    _, b = sim_state(circ, gpu, C64, cfg)
     assert abs(np.linalg.norm(a) - 1) < 1e-12
     assert np.linalg.norm(a - b) < 1e-5
     inv = Circuit(circ.gate_list + [g.dag() for g in circ.gate_list[::-1]], n=24)
-    _, z = sim_state(inv, gpu, C128)
+    _, z = sim_state(inv, gpu, C128, cfg)
     assert abs(z[0] - 1) < 1e-11
