@@ -13,6 +13,12 @@ from |0…0⟩ in complex64, state resident in HBM.  Reported:
 * adjoint gradients: full 640-parameter gradients/s of E = ⟨ψ|H_TFIM|ψ⟩ for the
   20-qubit, 16-layer hardware-efficient ansatz, next to the restated torch
   tape-autograd path on the same GPU;
+* fused_blocks: the 30-qubit circuit of 24 layers of Haar-random 5-qubit
+  blocks (125 blocks) in complex64 on the FP32 matrix cores: blocks/s, MFMA
+  TFLOP/s and utilisation against 157.3 TFLOP/s, error against complex128;
+* batch: 512 hardware-efficient-ansatz circuits (10..22 qubits, 4..32 layers)
+  forward + adjoint in one HIP-graph launch: circuits/s, p50/p99 latency,
+  gradient error against per-circuit complex128 runs on a sample;
 * cpu_baseline: the numpy complex128 oracle (tensordot per gate) on a bounded
   sample of the same 28-qubit circuit, timed on the host cores.
 """
@@ -35,6 +41,15 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-blocks", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--profile-forward", action="store_true",
+                    help="only the timed forward leg (for rocprofv3 runs)")
+    ap.add_argument("--profile-blocks", action="store_true", help="only the MFMA block leg")
+    ap.add_argument("--profile-copy", action="store_true",
+                    help="only a 2 GiB device copy (calibrates the HBM byte counters)")
+    ap.add_argument("--tape-host", action="store_true",
+                    help="also time the tape path with θ on the host (the reference setup; ~1 min)")
     ap.add_argument("--n", type=int, default=28)
     ap.add_argument("--layers", type=int, default=40)
     return ap.parse_args()
@@ -113,7 +128,7 @@ def check_forward(res, dev):
     return err, norm
 
 
-def bench_adjoint(dev, iters=5):
+def bench_adjoint(dev, iters=5, tape_host=False):
     import numpy as np
     import torch
 
@@ -145,19 +160,19 @@ def bench_adjoint(dev, iters=5):
     rel_err = float(np.linalg.norm(g_adj - g128) / np.linalg.norm(g128))
 
     th_cpu = torch.tensor(theta, dtype=torch.float64)
-    # warm the torch kernels on a 12-qubit copy of the ansatz, then time one
-    # full gradient at 20 qubits (the tape path takes seconds per gradient)
     th_small = torch.tensor(hea_theta(12, 2), dtype=torch.float64)
-    tape_energy_and_grad(hea_circuit(12, 2, th_small.numpy()), th_small, tfim_terms(12), dev)
-    torch.cuda.synchronize(dev)
-    log("[bench] torch tape: timing one 20-qubit gradient")
-    t0 = time.perf_counter()
-    tape_iters = 1
-    for _ in range(tape_iters):
+    t_tape = tape_err = None
+    if tape_host:
+        # warm the torch kernels on a 12-qubit copy of the ansatz, then time one
+        # full gradient at 20 qubits (the host-θ tape takes about a minute)
+        tape_energy_and_grad(hea_circuit(12, 2, th_small.numpy()), th_small, tfim_terms(12), dev)
+        torch.cuda.synchronize(dev)
+        log("[bench] torch tape (host θ): timing one 20-qubit gradient")
+        t0 = time.perf_counter()
         e_t, g_t = tape_energy_and_grad(circ, th_cpu, terms, dev)
-    torch.cuda.synchronize(dev)
-    t_tape = (time.perf_counter() - t0) / tape_iters
-    tape_err = float(np.linalg.norm(g_t.numpy() - g128) / np.linalg.norm(g128))
+        torch.cuda.synchronize(dev)
+        t_tape = time.perf_counter() - t0
+        tape_err = float(np.linalg.norm(g_t.numpy() - g128) / np.linalg.norm(g128))
     # the same tape with θ and every gate matrix on the GPU (no host round trips)
     th_dev = torch.tensor(theta, dtype=torch.float64, device=dev)
     tape_energy_and_grad(hea_circuit(12, 2, th_small.numpy()), th_small.to(dev), tfim_terms(12), dev)
@@ -170,14 +185,101 @@ def bench_adjoint(dev, iters=5):
     torch.cuda.synchronize(dev)
     t_dev = (time.perf_counter() - t0) / dev_iters
     dev_err = float(np.linalg.norm(g_d.cpu().numpy() - g128) / np.linalg.norm(g128))
-    return dict(adjoint_grads_per_s=1.0 / t_adj, torch_autograd_grads_per_s=1.0 / t_dev,
-                speedup=t_dev / t_adj, rel_grad_err_vs_c128=rel_err,
-                torch_autograd_rel_grad_err_vs_c128=dev_err,
-                torch_autograd_host_theta_grads_per_s=1.0 / t_tape,
-                speedup_vs_host_theta_tape=t_tape / t_adj,
-                torch_autograd_host_theta_rel_grad_err_vs_c128=tape_err,
-                energy=e_adj, energy_c128=float(e128.item()),
-                n_params=len(theta), n_gates=len(circ.gate_list))
+    out = dict(adjoint_grads_per_s=1.0 / t_adj, torch_autograd_grads_per_s=1.0 / t_dev,
+               speedup=t_dev / t_adj, rel_grad_err_vs_c128=rel_err,
+               torch_autograd_rel_grad_err_vs_c128=dev_err,
+               energy=e_adj, energy_c128=float(e128.item()),
+               n_params=len(theta), n_gates=len(circ.gate_list))
+    if t_tape is not None:
+        out.update(torch_autograd_host_theta_grads_per_s=1.0 / t_tape, speedup_vs_host_theta_tape=t_tape / t_adj,
+                   torch_autograd_host_theta_rel_grad_err_vs_c128=tape_err)
+    return out
+
+
+def bench_blocks(dev, n=30, layers=24, reps=3):
+    """Dense 5-qubit Haar blocks on MFMA (complex64) + a complex128 check."""
+    import torch
+
+    from qaravan_amd import _lib
+    from qaravan_amd.program import TileConfig, compile_ops, lower_circuit
+    from qaravan_amd.statevector_sim import DeviceScratch, run_program
+    from qaravan_amd.workloads import haar_block_circuit
+    circ = haar_block_circuit(n, layers).build()
+    nb = len(circ.gate_list)
+    prog = compile_ops(lower_circuit(circ, n), n, TileConfig(tb=8))
+    lib = _lib.load()
+    sc = DeviceScratch(dev)
+    psi = torch.empty(2 ** n, dtype=torch.complex64, device=dev)
+    times = []
+    for r in range(reps + 1):
+        _lib.check(lib.svh_init_basis(_lib.ptr(psi), n, 0, 0, _lib.stream_ptr(dev)), "init")
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        run_program(psi, prog, sc)
+        torch.cuda.synchronize(dev)
+        if r:
+            times.append(time.perf_counter() - t0)
+    dt = min(times)
+    flops = nb * (64 * 64 * 2) * 2 ** (n - 5)
+    psi128 = torch.empty(2 ** n, dtype=torch.complex128, device=dev)
+    _lib.check(lib.svh_init_basis(_lib.ptr(psi128), n, 1, 0, _lib.stream_ptr(dev)), "init")
+    run_program(psi128, prog, sc)
+    err = torch.linalg.vector_norm(psi.to(torch.complex128) - psi128).item()
+    del psi, psi128
+    torch.cuda.empty_cache()
+    return dict(workload=f"{n}q x {layers} layers of Haar 5-qubit blocks ({nb} blocks), complex64",
+                ms=dt * 1e3, ms_per_block=dt * 1e3 / nb, blocks_per_s=nb / dt,
+                mfma_tflops=flops / dt / 1e12, mfma_util=flops / dt / 157.3e12,
+                hbm_gbps=nb * 2 * 2 ** n * 8 / dt / 1e9, l2_err_vs_c128=err)
+
+
+def bench_batch(dev, nc=512, seed=20260004, check=6):
+    """512 HEA circuits (10..22 qubits, 4..32 layers) forward + adjoint as one
+    HIP-graph launch; latency from device timestamps."""
+    import numpy as np
+    import torch
+
+    from qaravan_amd.adjoint import AdjointModel
+    from qaravan_amd.batch import BatchedAdjoint
+    from qaravan_amd.workloads import hea_circuit, hea_ops, tfim_terms
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(10, 23, size=nc)
+    ls = rng.integers(4, 33, size=nc)
+    thetas = [rng.uniform(-np.pi, np.pi, size=2 * int(n) * int(l)) for n, l in zip(ns, ls)]
+    t0 = time.perf_counter()
+    b = BatchedAdjoint(None, [tfim_terms(int(n)) for n in ns], [len(t) for t in thetas], device=dev,
+                       dtype=torch.complex64, keys=list(zip(ns.tolist(), ls.tolist())),
+                       op_lists=[hea_ops(int(n), int(l)) for n, l in zip(ns, ls)], ns=ns.tolist())
+    log(f"[bench] batch: planned {nc} circuits in {time.perf_counter() - t0:.1f} s, {len(b.launches)} launches")
+    b.set_theta(thetas)
+    b.capture()
+    for _ in range(2):
+        b.run()
+    torch.cuda.synchronize(dev)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.run()
+    torch.cuda.synchronize(dev)
+    dt = (time.perf_counter() - t0) / reps
+    lat = b.latencies_us()
+    grads = b.gradients()
+    # accuracy: a sample of circuits against complex128 single-circuit adjoint runs
+    worst = 0.0
+    idx = list(np.argsort(ns)[-2:]) + list(rng.choice(nc, size=check - 2, replace=False))
+    for c in idx:
+        c = int(c)
+        m = AdjointModel(hea_circuit(int(ns[c]), int(ls[c]), thetas[c]), tfim_terms(int(ns[c])), len(thetas[c]),
+                         device=dev, dtype=torch.complex128)
+        _, g = m.energy_and_grad(torch.tensor(thetas[c], device=dev))
+        g = g.cpu().numpy()
+        worst = max(worst, float(np.linalg.norm(grads[c] - g) / np.linalg.norm(g)))
+    amps = float(sum(2.0 ** int(n) for n in ns))
+    return dict(workload=f"{nc} HEA circuits, n~U{{10..22}}, layers~U{{4..32}}, TFIM loss, forward+adjoint, "
+                         f"one HIP graph ({b.n_launches} kernels)",
+                circuits_per_s=nc / dt, ms_per_batch=dt * 1e3, p50_latency_ms=float(np.percentile(lat, 50)) / 1e3,
+                p99_latency_ms=float(np.percentile(lat, 99)) / 1e3, max_rel_grad_err_vs_c128_sample=worst,
+                checked_circuits=len(idx), total_amplitudes=amps)
 
 
 def cpu_baseline(n, layers, max_seconds=20.0):
@@ -208,6 +310,17 @@ def cpu_baseline(n, layers, max_seconds=20.0):
                        f"circuit, numpy complex128 oracle, {dt:.1f} s")
 
 
+def traffic_bytes(fwd):
+    """HBM bytes per tile-pass launch from the committed rocprofv3 counter runs
+    (FETCH_SIZE x2 calibration + WRITE_SIZE, KiB per dispatch), or None."""
+    path = os.path.join(ROOT, "profiles", "tile_pass_traffic_28q.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -216,15 +329,38 @@ def main():
         sys.exit(2)
     import torch
     dev = torch.device("cuda:0")
+    if args.profile_copy:
+        a = torch.ones(2 ** 28, dtype=torch.complex64, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize(dev)
+        print(json.dumps({"profile": "copy", "bytes_read": a.numel() * 8, "bytes_written": b.numel() * 8}))
+        return
+    if args.profile_blocks:
+        print(json.dumps({"profile": "blocks", **bench_blocks(dev, reps=1)}))
+        return
     fwd = bench_forward(args, dev)
+    if args.profile_forward:
+        print(json.dumps({"profile": "forward", "ms_per_step": fwd["ms_step"], "tile_ms": fwd["tile_ms"],
+                          "n_tile": fwd["n_tile"]}))
+        return
     log(f"[bench] forward: {fwd['ms_step']:.2f} ms/step, {fwd['gbps']:.0f} GB/s, "
         f"{fwd['gates_per_s']:.0f} gates/s, tile pass {fwd['tile_ms']:.3f} ms = {fwd['tile_gbps']:.0f} GB/s")
     err, norm = check_forward(fwd, dev)
     log(f"[bench] c64 vs c128 at {fwd['n']}q: {err:.3e} (norm {norm:.12f})")
     adj = None
     if not args.no_adjoint:
-        adj = bench_adjoint(dev)
+        adj = bench_adjoint(dev, tape_host=args.tape_host)
         log(f"[bench] adjoint: {adj}")
+    blocks = None
+    if not args.no_blocks:
+        blocks = bench_blocks(dev)
+        log(f"[bench] fused blocks: {blocks}")
+    batch = None
+    if not args.no_batch:
+        batch = bench_batch(dev)
+        log(f"[bench] batch: {batch}")
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(fwd["n"], fwd["layers"])
@@ -251,13 +387,18 @@ def main():
         "gates_per_s": round(fwd["gates_per_s"], 1),
         "c64_vs_c128_l2_err": err,
         "roofline": {"bound": "hbm", "achieved": round(fwd["tile_gbps"], 1), "peak": peak, "unit": "GB/s",
-                     "frac": round(fwd["tile_gbps"] / peak, 4), "traffic": None,
-                     "kernel": "k_tile<float, tb, 4, false>", "bytes_per_launch": fwd["state_bytes"],
-                     "avg_launch_ms": round(fwd["tile_ms"], 4)},
+                     "frac": round(fwd["tile_gbps"] / peak, 4), "traffic": traffic_bytes(fwd),
+                     "kernel": ("svh_sp_<p> (specialised tile pass)" if fwd["cfg"].specialize
+                                else "k_tile<float, tb, 4, false>"),
+                     "bytes_per_launch": fwd["state_bytes"], "avg_launch_ms": round(fwd["tile_ms"], 4)},
         "cpu_baseline": cpu,
     }
     if adj is not None:
         out["adjoint"] = adj
+    if blocks is not None:
+        out["fused_blocks"] = blocks
+    if batch is not None:
+        out["batch"] = batch
     print(json.dumps(out))
 
 

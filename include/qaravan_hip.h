@@ -17,6 +17,7 @@
  *   svh_init_basis        tensorQ/statevector_sim.py:31-48  StatevectorSim.initialize_state
  *   svh_apply_dense       tensorQ/statevector_sim.py:50-52,124-140  apply_gate -> op_action
  *   svh_run_program       core/base_sim.py:30-53  BaseSim.run (the per-layer/per-gate loop)
+ *   svh_sp_*              the same loop, with each fused pass compiled for its gate list
  *   svh_norm2             tensorQ/statevector_sim.py:107-109  normalize_state (norm)
  *   svh_scale             tensorQ/statevector_sim.py:107-109  normalize_state (divide)
  *   svh_pauli_expect      tensorQ/statevector_sim.py:217-234  pauli_expectation_sv, summed
@@ -203,6 +204,18 @@ int svh_program_scratch_bytes(const void *prog_host, int dtype, size_t *bytes);
 int svh_run_program(void *psi, int n, int dtype, const void *prog_host,
                     const void *prog_dev, const double *theta_dev, void *scratch,
                     size_t scratch_bytes, void *stream);
+
+/* Specialised programs: code objects (hipcc --genco) holding kernels named
+ * svh_sp_<p>, one per tile pass p they cover, generated for one program with
+ * every op spelled out (qaravan_amd/specialize.py).  svh_sp_load resolves the
+ * kernels of n_passes passes across the given code-object files;
+ * svh_run_program_sp runs the program launching them where present and the
+ * generic kernels elsewhere.  The handle owns the loaded modules. */
+int svh_sp_load(const char *const *paths, int npaths, int n_passes, void **handle);
+int svh_sp_free(void *handle);
+int svh_run_program_sp(void *handle, void *psi, int n, int dtype, const void *prog_host,
+                       const void *prog_dev, const double *theta_dev, void *scratch,
+                       size_t scratch_bytes, void *stream);
 
 /* Adjoint sweep: the program holds the reversed, daggered gate list with
  * SVH_OPF_GRAD on rotations.  Walks ψ and λ back together and ADDS

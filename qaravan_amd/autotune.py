@@ -46,9 +46,9 @@ def best_tile_config(n, dtype):
         cfg = base
     else:
         cfg = TileConfig(tb=e["tb"], low_bits=e["low_bits"], max_cost=e["max_cost"],
-                         prefetch=e.get("prefetch", False))
+                         prefetch=e.get("prefetch", False), specialize=e.get("specialize", False))
     return TileConfig(tb=min(cfg.tb, n - 4), low_bits=cfg.low_bits, max_cost=cfg.max_cost,
-                      prefetch=cfg.prefetch)
+                      prefetch=cfg.prefetch, specialize=cfg.specialize)
 
 
 def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_path=None):
@@ -66,11 +66,13 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_
     psi = torch.empty(2 ** n, dtype=dtype, device=device)
     scratch = DeviceScratch(device)
     results = []
-    for tb, lb, mc, pf in itertools.product(space["tb"], space["low_bits"], space["max_cost"],
-                                            space.get("prefetch", [False])):
+    for tb, lb, mc, pf, spz in itertools.product(space["tb"], space["low_bits"], space["max_cost"],
+                                                 space.get("prefetch", [False]), space.get("specialize", [False])):
         if tb + 4 > n or (dtype == torch.complex128 and tb > (8 if pf else 9)) or (pf and tb > 9):
             continue
-        cfg = TileConfig(tb=tb, low_bits=lb, max_cost=mc, prefetch=pf)
+        if pf and spz:
+            continue
+        cfg = TileConfig(tb=tb, low_bits=lb, max_cost=mc, prefetch=pf, specialize=spz)
         prog = compile_ops(ops, n, cfg)
         times = []
         for r in range(repeats + 1):
@@ -84,14 +86,15 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_
                 times.append(time.perf_counter() - t0)
         t = min(times)
         results.append((t, cfg, prog.n_tile_passes()))
-        log(f"[tune] n={n} {dtype} tb={tb} low={lb} cap={mc:g} prefetch={pf}: {t * 1e3:.3f} ms, "
-            f"{prog.n_tile_passes()} passes")
+        log(f"[tune] n={n} {dtype} tb={tb} low={lb} cap={mc:g} prefetch={pf} specialize={spz}: "
+            f"{t * 1e3:.3f} ms, {prog.n_tile_passes()} passes")
     results.sort(key=lambda x: x[0])
     t, cfg, npass = results[0]
     if write:
         cache = _load()
         cache[_key(n, dtype)] = dict(tb=cfg.tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost,
-                                     prefetch=cfg.prefetch, ms=t * 1e3, passes=npass)
+                                     prefetch=cfg.prefetch, specialize=cfg.specialize, ms=t * 1e3,
+                                     passes=npass)
         with open(out_path or CACHE, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)
     return cfg, results

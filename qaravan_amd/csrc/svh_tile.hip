@@ -11,307 +11,11 @@
 // DUAL = true is the adjoint sweep: the same walk over ψ and λ together, with
 // 2·Im⟨λ|P|ψ⟩ accumulated in FP64 for every gate flagged SVH_OPF_GRAD before
 // U† is applied to both vectors.
-#include "svh_common.h"
+#include "svh_tile_ops.h"
+
+#include <vector>
 
 namespace svh {
-
-// LDS swizzle: XOR the low 5 bits with the next 5.  Spreads lane groups whose
-// tile-local bits skip the register bits across the LDS banks.  It is linear
-// over GF(2), so swz(a | b) = swz(a) ^ swz(b) for disjoint a, b: the host stores
-// swz(register offset) per stage and the kernel XORs in swz(thread offset).
-__device__ __forceinline__ int lds_swz(int li) { return li ^ ((li >> 5) & 31); }
-
-// ------------------------------------------------ in-register gate kernels
-template <typename R, int RB, int A>
-__device__ __forceinline__ void ap_mat1(cpx<R> *v, cpx<R> m0, cpx<R> m1, cpx<R> m2, cpx<R> m3) {
-#pragma unroll
-    for (int p = 0; p < (1 << (RB - 1)); ++p) {
-        const int i0 = ins0<A>(p), i1 = i0 | (1 << A);
-        const cpx<R> x0 = v[i0], x1 = v[i1];
-        v[i0] = cfma(m0, x0, cmul(m1, x1));
-        v[i1] = cfma(m2, x0, cmul(m3, x1));
-    }
-}
-
-// exp(-iθP) = c·I - i s·P
-template <typename R, int RB, int A>
-__device__ __forceinline__ void ap_rot1(cpx<R> *v, int pauli, R c, R s) {
-#pragma unroll
-    for (int p = 0; p < (1 << (RB - 1)); ++p) {
-        const int i0 = ins0<A>(p), i1 = i0 | (1 << A);
-        const cpx<R> x0 = v[i0], x1 = v[i1];
-        if (pauli == 1) {  // X: y0 = c x0 - i s x1
-            v[i0] = cpx<R>{c * x0.x + s * x1.y, c * x0.y - s * x1.x};
-            v[i1] = cpx<R>{c * x1.x + s * x0.y, c * x1.y - s * x0.x};
-        } else if (pauli == 2) {  // Y: y0 = c x0 - s x1, y1 = s x0 + c x1
-            v[i0] = cpx<R>{c * x0.x - s * x1.x, c * x0.y - s * x1.y};
-            v[i1] = cpx<R>{s * x0.x + c * x1.x, s * x0.y + c * x1.y};
-        } else {  // Z: y0 = (c - i s) x0, y1 = (c + i s) x1
-            v[i0] = cpx<R>{c * x0.x + s * x0.y, c * x0.y - s * x0.x};
-            v[i1] = cpx<R>{c * x1.x - s * x1.y, c * x1.y + s * x1.x};
-        }
-    }
-}
-
-// Σ Im(conj(λ)·(Pψ)) over this thread's pairs.
-template <typename R, int RB, int A>
-__device__ __forceinline__ double gr_rot1(const cpx<R> *v, const cpx<R> *w, int pauli) {
-    double acc = 0.0;
-#pragma unroll
-    for (int p = 0; p < (1 << (RB - 1)); ++p) {
-        const int i0 = ins0<A>(p), i1 = i0 | (1 << A);
-        const cpx<R> x0 = v[i0], x1 = v[i1], l0 = w[i0], l1 = w[i1];
-        if (pauli == 1) {
-            acc += im_conj_mul(l0, x1) + im_conj_mul(l1, x0);
-        } else if (pauli == 2) {  // (Pψ)0 = -i x1, (Pψ)1 = i x0
-            acc += im_conj_mul(l0, cpx<R>{x1.y, -x1.x}) + im_conj_mul(l1, cpx<R>{-x0.y, x0.x});
-        } else {
-            acc += im_conj_mul(l0, x0) - im_conj_mul(l1, x1);
-        }
-    }
-    return acc;
-}
-
-template <typename R, int HI, int LO>
-struct Quad {
-    static __device__ __forceinline__ int base(int q) { return ins0<HI>(ins0<LO>(q)); }
-};
-
-template <typename R, int RB, int HI, int LO>
-__device__ __forceinline__ void ap_mat2(cpx<R> *v, const cpx<R> (&m)[16]) {
-#pragma unroll
-    for (int q = 0; q < (1 << (RB - 2)); ++q) {
-        const int i00 = Quad<R, HI, LO>::base(q);
-        const int id[4] = {i00, i00 | (1 << LO), i00 | (1 << HI), i00 | (1 << HI) | (1 << LO)};
-        const cpx<R> x[4] = {v[id[0]], v[id[1]], v[id[2]], v[id[3]]};
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            cpx<R> acc = cmul(m[4 * r], x[0]);
-            acc = cfma(m[4 * r + 1], x[1], acc);
-            acc = cfma(m[4 * r + 2], x[2], acc);
-            acc = cfma(m[4 * r + 3], x[3], acc);
-            v[id[r]] = acc;
-        }
-    }
-}
-
-// CX: TGT_HI -> target HI, control LO; otherwise target LO, control HI.
-// (Both orientations are compile-time so every register index is constant.)
-template <typename R, int RB, int HI, int LO, bool TGT_HI>
-__device__ __forceinline__ void ap_cx_t(cpx<R> *v) {
-#pragma unroll
-    for (int q = 0; q < (1 << (RB - 2)); ++q) {
-        const int i00 = Quad<R, HI, LO>::base(q);
-        const int i11 = i00 | (1 << HI) | (1 << LO);
-        constexpr int flip = TGT_HI ? (1 << LO) : (1 << HI);
-        const int ia = i00 | flip;
-        const cpx<R> t = v[ia];
-        v[ia] = v[i11];
-        v[i11] = t;
-    }
-}
-
-template <typename R, int RB, int HI, int LO>
-__device__ __forceinline__ void ap_cz(cpx<R> *v) {
-#pragma unroll
-    for (int q = 0; q < (1 << (RB - 2)); ++q) {
-        const int i11 = Quad<R, HI, LO>::base(q) | (1 << HI) | (1 << LO);
-        v[i11] = cneg(v[i11]);
-    }
-}
-
-// P⊗P acting on a quad: returns (P⊗P x) in y.
-template <typename R>
-__device__ __forceinline__ void pp_apply(int pauli, const cpx<R> (&x)[4], cpx<R> (&y)[4]) {
-    if (pauli == 1) {  // XX: |b⟩ -> |~b⟩
-        y[0] = x[3]; y[1] = x[2]; y[2] = x[1]; y[3] = x[0];
-    } else if (pauli == 2) {  // YY: (YYx)_00 = -x_11, _11 = -x_00, _01 = x_10, _10 = x_01
-        y[0] = cneg(x[3]); y[1] = x[2]; y[2] = x[1]; y[3] = cneg(x[0]);
-    } else {  // ZZ: diag(1,-1,-1,1)
-        y[0] = x[0]; y[1] = cneg(x[1]); y[2] = cneg(x[2]); y[3] = x[3];
-    }
-}
-
-template <typename R, int RB, int HI, int LO>
-__device__ __forceinline__ void ap_rot2(cpx<R> *v, int pauli, R c, R s) {
-#pragma unroll
-    for (int q = 0; q < (1 << (RB - 2)); ++q) {
-        const int i00 = Quad<R, HI, LO>::base(q);
-        const int id[4] = {i00, i00 | (1 << LO), i00 | (1 << HI), i00 | (1 << HI) | (1 << LO)};
-        const cpx<R> x[4] = {v[id[0]], v[id[1]], v[id[2]], v[id[3]]};
-        cpx<R> y[4];
-        pp_apply<R>(pauli, x, y);
-#pragma unroll
-        for (int r = 0; r < 4; ++r)  // c x - i s y
-            v[id[r]] = cpx<R>{c * x[r].x + s * y[r].y, c * x[r].y - s * y[r].x};
-    }
-}
-
-template <typename R, int RB, int HI, int LO>
-__device__ __forceinline__ double gr_rot2(const cpx<R> *v, const cpx<R> *w, int pauli) {
-    double acc = 0.0;
-#pragma unroll
-    for (int q = 0; q < (1 << (RB - 2)); ++q) {
-        const int i00 = Quad<R, HI, LO>::base(q);
-        const int id[4] = {i00, i00 | (1 << LO), i00 | (1 << HI), i00 | (1 << HI) | (1 << LO)};
-        const cpx<R> x[4] = {v[id[0]], v[id[1]], v[id[2]], v[id[3]]};
-        cpx<R> y[4];
-        pp_apply<R>(pauli, x, y);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc += im_conj_mul(w[id[r]], y[r]);
-    }
-    return acc;
-}
-
-// ------------------------------------------------ compile-time dispatch
-// Register-bit indices arrive at run time (uniform across the workgroup); each
-// switch arm instantiates the gate with constant indices so v[] stays in VGPRs.
-#define SVH_SWITCH1(a, CALL)                                   \
-    switch (a) {                                               \
-        case 0: CALL(0); break;                                \
-        case 1: if constexpr (RB > 1) { CALL(1); } break;      \
-        case 2: if constexpr (RB > 2) { CALL(2); } break;      \
-        case 3: if constexpr (RB > 3) { CALL(3); } break;      \
-        case 4: if constexpr (RB > 4) { CALL(4); } break;      \
-        default: break;                                        \
-    }
-#define SVH_C2(H, L, CALL) \
-    case (H)*8 + (L): if constexpr (RB > (H)) { CALL(H, L); } break;
-#define SVH_SWITCH2(hi, lo, CALL)                                                   \
-    switch ((hi)*8 + (lo)) {                                                        \
-        SVH_C2(1, 0, CALL)                                                          \
-        SVH_C2(2, 0, CALL) SVH_C2(2, 1, CALL)                                       \
-        SVH_C2(3, 0, CALL) SVH_C2(3, 1, CALL) SVH_C2(3, 2, CALL)                    \
-        SVH_C2(4, 0, CALL) SVH_C2(4, 1, CALL) SVH_C2(4, 2, CALL) SVH_C2(4, 3, CALL) \
-        default: break;                                                             \
-    }
-
-// Sum over the lanes of one wave; W = active lanes (64, or the whole
-// workgroup when it is smaller than a wave).
-template <int W>
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, W);
-    return v;
-}
-
-template <typename R>
-__device__ __forceinline__ cpx<R> ld_m(const svh_op_t &op, int i) {
-    if constexpr (std::is_same<R, float>::value) return cpx<R>{op.mf[2 * i], op.mf[2 * i + 1]};
-    else return cpx<R>{op.m[2 * i], op.m[2 * i + 1]};
-}
-
-// (cos θ, sin θ) of the op's angle from the per-call table (FP64-evaluated).
-template <typename R>
-__device__ __forceinline__ void op_sincos(const cpx<R> *trig, const svh_op_t &op, R &cs, R &sn) {
-    const cpx<R> t = trig[op.param];
-    cs = t.x;
-    sn = (op.flags & SVH_OPF_NEG) ? -t.y : t.y;
-}
-
-// Apply one op to the registers (v: ψ, w: λ when DUAL).  Returns the gradient
-// partial (this thread's Σ Im(conj λ · Pψ)) through *g when the op asks for it.
-template <typename R, int RB, bool DUAL>
-__device__ __forceinline__ void apply_op(cpx<R> (&v)[1 << RB], cpx<R> (&w)[DUAL ? (1 << RB) : 1],
-                                         const svh_op_t &op, const cpx<R> *trig, double *g,
-                                         bool *has_g) {
-    const int kind = op.kind;
-    if (kind == SVH_OP_MAT1) {
-        const cpx<R> m0 = ld_m<R>(op, 0), m1 = ld_m<R>(op, 1), m2 = ld_m<R>(op, 2),
-                     m3 = ld_m<R>(op, 3);
-#define CALL_M1(A)                                      \
-    ap_mat1<R, RB, A>(v, m0, m1, m2, m3);               \
-    if constexpr (DUAL) ap_mat1<R, RB, A>(w, m0, m1, m2, m3);
-        SVH_SWITCH1(op.a, CALL_M1)
-#undef CALL_M1
-    } else if (kind == SVH_OP_ROT1) {
-        R cs, sn;
-        op_sincos<R>(trig, op, cs, sn);
-        const int pauli = op.pauli;
-        if constexpr (DUAL) {
-            if (op.flags & SVH_OPF_GRAD) {
-                *has_g = true;
-#define CALL_G1(A) *g = gr_rot1<R, RB, A>(v, w, pauli);
-                SVH_SWITCH1(op.a, CALL_G1)
-#undef CALL_G1
-            }
-        }
-#define CALL_R1(A)                                   \
-    ap_rot1<R, RB, A>(v, pauli, cs, sn);             \
-    if constexpr (DUAL) ap_rot1<R, RB, A>(w, pauli, cs, sn);
-        SVH_SWITCH1(op.a, CALL_R1)
-#undef CALL_R1
-    } else if (kind == SVH_OP_CX) {
-        const int hi = op.a > op.b ? op.a : op.b, lo = op.a > op.b ? op.b : op.a;
-        if (op.a > op.b) {
-#define CALL_CXH(H, L)                               \
-    ap_cx_t<R, RB, H, L, true>(v);                   \
-    if constexpr (DUAL) ap_cx_t<R, RB, H, L, true>(w);
-            SVH_SWITCH2(hi, lo, CALL_CXH)
-#undef CALL_CXH
-        } else {
-#define CALL_CXL(H, L)                               \
-    ap_cx_t<R, RB, H, L, false>(v);                  \
-    if constexpr (DUAL) ap_cx_t<R, RB, H, L, false>(w);
-            SVH_SWITCH2(hi, lo, CALL_CXL)
-#undef CALL_CXL
-        }
-    } else if (kind == SVH_OP_CZ) {
-        const int hi = op.a > op.b ? op.a : op.b, lo = op.a > op.b ? op.b : op.a;
-#define CALL_CZ(H, L)                 \
-    ap_cz<R, RB, H, L>(v);            \
-    if constexpr (DUAL) ap_cz<R, RB, H, L>(w);
-        SVH_SWITCH2(hi, lo, CALL_CZ)
-#undef CALL_CZ
-    } else if (kind == SVH_OP_MAT2) {
-        cpx<R> m[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) m[i] = ld_m<R>(op, i);
-#define CALL_M2(H, L)                    \
-    ap_mat2<R, RB, H, L>(v, m);          \
-    if constexpr (DUAL) ap_mat2<R, RB, H, L>(w, m);
-        SVH_SWITCH2(op.a, op.b, CALL_M2)
-#undef CALL_M2
-    } else if (kind == SVH_OP_ROT2) {
-        R cs, sn;
-        op_sincos<R>(trig, op, cs, sn);
-        const int pauli = op.pauli;
-        if constexpr (DUAL) {
-            if (op.flags & SVH_OPF_GRAD) {
-                *has_g = true;
-#define CALL_G2(H, L) *g = gr_rot2<R, RB, H, L>(v, w, pauli);
-                SVH_SWITCH2(op.a, op.b, CALL_G2)
-#undef CALL_G2
-            }
-        }
-#define CALL_R2(H, L)                             \
-    ap_rot2<R, RB, H, L>(v, pauli, cs, sn);       \
-    if constexpr (DUAL) ap_rot2<R, RB, H, L>(w, pauli, cs, sn);
-        SVH_SWITCH2(op.a, op.b, CALL_R2)
-#undef CALL_R2
-    }
-}
-
-// Tile-local index of this thread's register 0 in a stage's layout: the thread
-// index with zeros inserted at the (ascending) register bits.
-__device__ __forceinline__ int thr_local(const svh_stage_t &S, int tid) {
-    int t = tid;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int r = S.rsorted[k];
-        t = ((t >> r) << (r + 1)) | (t & ((1 << r) - 1));
-    }
-    return t;
-}
-
-// Global amplitude offset of this thread's register 0 (thread bits only).
-template <int TB>
-__device__ __forceinline__ uint64_t thr_global(const svh_stage_t &S, int tid) {
-    uint64_t a = 0;
-#pragma unroll
-    for (int i = 0; i < TB; ++i) a |= (uint64_t)((tid >> i) & 1) << S.tgbit[i];
-    return a;
-}
 
 // ------------------------------------------------ the pass kernel
 // One tile of one pass: workgroup-local work shared by k_tile (one circuit,
@@ -810,9 +514,31 @@ static size_t scratch_need(const svh_prog_header_t *h, int dtype, bool dual) {
     return trig_bytes(h, dtype) + ((need + 255) & ~(size_t)255);
 }
 
+// Kernels of a specialised program: svh_sp_<p> for the passes it covers.
+struct SpHandle {
+    std::vector<hipModule_t> modules;
+    std::vector<hipFunction_t> fn;  // per pass, nullptr = not specialised
+};
+
+static int launch_sp(hipFunction_t f, void *psi, int n, int dtype, const svh_pass_t &hp, const svh_pass_t *dp,
+                     const svh_stage_t *dst, const svh_op_t *dop, const svh_stage_t *hst, const void *trig,
+                     hipStream_t st) {
+    const int L = hp.L;
+    const size_t es = dtype == SVH_C64 ? 8 : 16;
+    const size_t lds = es * ((size_t)1 << L);
+    if (lds > 65536) return fail(-7, "specialised pass needs more than 64 KiB of LDS");
+    const svh_stage_t *stp = dst + hp.stage_begin;
+    const svh_op_t *opp = dop + hst[hp.stage_begin].op_begin;
+    int nn = n;
+    void *args[] = {&psi, &nn, (void *)&dp, (void *)&stp, (void *)&opp, (void *)&trig};
+    const unsigned grid = (unsigned)(1ull << (n - L));
+    return hip_status(hipModuleLaunchKernel(f, grid, 1, 1, 1u << hp.tb, 1, 1, (unsigned)lds, st, args, nullptr),
+                      "hipModuleLaunchKernel(svh_sp)");
+}
+
 static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_host, const void *prog_dev,
                     const double *theta_dev, double *grad_dev, void *scratch, size_t scratch_bytes,
-                    void *stream, bool dual) {
+                    void *stream, bool dual, const SpHandle *sp = nullptr) {
     if (!psi || (dual && (!lam || !grad_dev))) return fail(-1, "state/lambda/grad pointer is NULL");
     if (n < 1 || n > kMaxQubits) return fail(-2, "n out of range");
     if (dtype != SVH_C64 && dtype != SVH_C128) return fail(-3, "bad dtype");
@@ -844,10 +570,14 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
         if (int e = checked_launch("k_trig")) return e;
     }
 
+    const auto *hstage = reinterpret_cast<const svh_stage_t *>(hb + h->off_stages);
+    if (sp && (int)sp->fn.size() != h->n_passes) return fail(-8, "specialised kernels do not match the program");
     for (int p = 0; p < h->n_passes; ++p) {
         const svh_pass_t &P = hpass[p];
         int e = 0;
-        if (P.kind == SVH_PASS_TILE) {
+        if (P.kind == SVH_PASS_TILE && !dual && sp && sp->fn[p]) {
+            e = launch_sp(sp->fn[p], psi, n, dtype, P, dpass + p, dstage, dop, hstage, trig, st);
+        } else if (P.kind == SVH_PASS_TILE) {
             if (dtype == SVH_C64)
                 e = dual ? launch_tile<float, true>((cpx<float> *)psi, (cpx<float> *)lam, n, P, dpass + p, dstage,
                                                     dop, (const cpx<float> *)trig, sd, st)
@@ -960,6 +690,50 @@ int svh_run_program(void *psi, int n, int dtype, const void *prog_host, const vo
                     const double *theta_dev, void *scratch, size_t scratch_bytes, void *stream) {
     return run_impl(psi, nullptr, n, dtype, prog_host, prog_dev, theta_dev, nullptr, scratch, scratch_bytes,
                     stream, false);
+}
+
+int svh_sp_load(const char *const *paths, int npaths, int n_passes, void **handle) {
+    if (!paths || npaths <= 0 || n_passes < 0 || !handle) return fail(-1, "bad arguments");
+    auto *H = new SpHandle();
+    H->fn.assign(n_passes, nullptr);
+    for (int i = 0; i < npaths; ++i) {
+        hipModule_t m = nullptr;
+        hipError_t e = hipModuleLoad(&m, paths[i]);
+        if (e != hipSuccess) {
+            for (auto mm : H->modules) (void)hipModuleUnload(mm);
+            delete H;
+            return hip_status(e, "hipModuleLoad");
+        }
+        H->modules.push_back(m);
+        for (int p = 0; p < n_passes; ++p) {
+            if (H->fn[p]) continue;
+            hipFunction_t f = nullptr;
+            const std::string name = "svh_sp_" + std::to_string(p);
+            if (hipModuleGetFunction(&f, m, name.c_str()) == hipSuccess) H->fn[p] = f;
+        }
+    }
+    (void)hipGetLastError();  // lookups of absent kernels are expected
+    *handle = H;
+    return 0;
+}
+
+int svh_sp_free(void *handle) {
+    auto *H = static_cast<SpHandle *>(handle);
+    if (!H) return 0;
+    int rc = 0;
+    for (auto m : H->modules) {
+        const int e = hip_status(hipModuleUnload(m), "hipModuleUnload");
+        if (e && !rc) rc = e;
+    }
+    delete H;
+    return rc;
+}
+
+int svh_run_program_sp(void *handle, void *psi, int n, int dtype, const void *prog_host, const void *prog_dev,
+                       const double *theta_dev, void *scratch, size_t scratch_bytes, void *stream) {
+    if (!handle) return fail(-1, "specialised handle is NULL");
+    return run_impl(psi, nullptr, n, dtype, prog_host, prog_dev, theta_dev, nullptr, scratch, scratch_bytes,
+                    stream, false, static_cast<const SpHandle *>(handle));
 }
 
 int svh_run_adjoint(void *psi, void *lam, int n, int dtype, const void *prog_host, const void *prog_dev,

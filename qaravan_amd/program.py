@@ -56,20 +56,27 @@ assert HDR_DT.itemsize == 96 and PASS_DT.itemsize == 112
 assert STAGE_DT.itemsize == 304 and OP_DT.itemsize == 416
 
 
+_ZERO_M = np.zeros(32)
+
+
 def lds_swz(li):
     """The kernel's LDS swizzle (linear over GF(2)): low 5 bits ^= next 5."""
     return li ^ ((li >> 5) & 31)
+
+
+_JBITS = np.array([[(j >> k) & 1 for k in range(RB)] for j in range(1 << RB)], dtype=np.int64)
 
 
 def stage_tables(tile_bits, tb, rbits):
     """Address tables of one stage layout (see svh_stage_t): thread bits fill
     the non-register tile-local bits in ascending order."""
     L = tb + RB
-    tpos = [x for x in range(L) if x not in rbits]
-    tgbit = [tile_bits[x] for x in tpos]
-    loc = [sum(((j >> k) & 1) << rbits[k] for k in range(RB)) for j in range(1 << RB)]
-    goff = [sum(((j >> k) & 1) << tile_bits[rbits[k]] for k in range(RB)) for j in range(1 << RB)]
-    return dict(rsorted=sorted(rbits), tgbit=tgbit, lsw=[lds_swz(x) for x in loc], goff=goff, loc=loc)
+    rset = set(rbits)
+    tgbit = [tile_bits[x] for x in range(L) if x not in rset]
+    rb = np.asarray(rbits, dtype=np.int64)
+    loc = _JBITS @ (np.int64(1) << rb)
+    goff = _JBITS @ (np.int64(1) << np.asarray([tile_bits[x] for x in rbits], dtype=np.int64))
+    return dict(rsorted=sorted(rbits), tgbit=tgbit, lsw=loc ^ ((loc >> 5) & 31), goff=goff, loc=loc)
 
 PAULI_CODE = {'x': 1, 'y': 2, 'z': 3}
 
@@ -234,12 +241,15 @@ class TileConfig:
               2^low_bits amplitudes per load);
     max_cost: cap on Σ per-amplitude VALU cost of the ops in one pass;
     prefetch: persistent workgroups that copy the next tile HBM -> LDS while
-              computing the current one (forward passes only).
+              computing the current one (forward passes only);
+    specialize: compile each forward pass into its own straight-line kernel
+              (qaravan_amd.specialize) instead of the op-interpreting kernel.
     """
     tb: int = 8
     low_bits: int = 6
     max_cost: float = 96.0
     prefetch: bool = False
+    specialize: bool = False
 
 
 def _is_barrier(op):
@@ -369,6 +379,15 @@ class Program:
         self.grad_params = grad_params    # slot -> param index (all passes)
         self.n_params = n_params
         self._dev = {}
+        self._sp = {}
+        self.specialize = False
+
+    def specialized(self, dtype_code):
+        """Loaded straight-line kernels of this program (compiled on first use)."""
+        if dtype_code not in self._sp:
+            from .specialize import Specialized
+            self._sp[dtype_code] = Specialized(self, dtype_code)
+        return self._sp[dtype_code]
 
     @property
     def n_passes(self):
@@ -423,39 +442,38 @@ def encode(n, passes, grad=False, n_params=0):
                 s['op_begin'] = len(op_rows)
                 ridx = {x: k for k, x in enumerate(st.rbits)}
                 for op, lb in st.ops:
-                    o = np.zeros((), dtype=OP_DT)
                     regs = [ridx[x] for x in lb]
-                    o['param'] = op.param
-                    o['pauli'] = op.pauli
-                    o['flags'] = op.flags
+                    m = _ZERO_M
+                    a = regs[0]
+                    b = 0
                     if op.kind == 'mat1':
-                        o['kind'] = OP_MAT1
-                        o['a'] = regs[0]
-                        o['m'][:8] = np.ascontiguousarray(op.mat, dtype=np.complex128).reshape(-1).view(np.float64)
+                        kind = OP_MAT1
+                        m = np.zeros(32)
+                        m[:8] = np.ascontiguousarray(op.mat, dtype=np.complex128).reshape(-1).view(np.float64)
                     elif op.kind == 'mat2':
-                        m = op.mat
+                        mm = op.mat
                         a, b = regs
                         if a < b:
                             a, b = b, a
-                            m = _swap_qubits_4x4(m)
-                        o['kind'], o['a'], o['b'] = OP_MAT2, a, b
-                        o['m'][:] = np.ascontiguousarray(m).reshape(-1).view(np.float64)
+                            mm = _swap_qubits_4x4(mm)
+                        kind = OP_MAT2
+                        m = np.ascontiguousarray(mm, dtype=np.complex128).reshape(-1).view(np.float64)
                     elif op.kind == 'cx':
-                        o['kind'], o['a'], o['b'] = OP_CX, regs[0], regs[1]
+                        kind, a, b = OP_CX, regs[0], regs[1]
                     elif op.kind == 'cz':
-                        o['kind'], o['a'], o['b'] = OP_CZ, max(regs), min(regs)
+                        kind, a, b = OP_CZ, max(regs), min(regs)
                     elif op.kind == 'rot1':
-                        o['kind'], o['a'] = OP_ROT1, regs[0]
+                        kind = OP_ROT1
                     elif op.kind == 'rot2':
-                        o['kind'], o['a'], o['b'] = OP_ROT2, max(regs), min(regs)
+                        kind, a, b = OP_ROT2, max(regs), min(regs)
                     else:
                         raise ValueError(op.kind)
-                    o['mf'][:] = o['m'].astype(np.float32)
+                    gslot = 0
                     if grad and op.kind in ('rot1', 'rot2') and (op.flags & OPF_GRAD):
-                        o['grad_slot'] = slot
+                        gslot = slot
                         grad_params.append(op.param)
                         slot += 1
-                    op_rows.append(o)
+                    op_rows.append((kind, a, b, op.param, op.pauli, op.flags, gslot, 0, m, m.astype(np.float32)))
                 s['op_end'] = len(op_rows)
                 stage_rows.append(s)
             pr['n_grad'] = slot
@@ -467,6 +485,7 @@ def encode(n, passes, grad=False, n_params=0):
             pool.extend(np.ascontiguousarray(p.mat, dtype=np.complex128).reshape(-1).view(np.float64))
     stages = np.array(stage_rows, dtype=STAGE_DT) if stage_rows else np.zeros(0, STAGE_DT)
     opsa = np.array(op_rows, dtype=OP_DT) if op_rows else np.zeros(0, OP_DT)
+    used = [int(r[3]) + 1 for r in op_rows if r[0] in (OP_ROT1, OP_ROT2)]
     grads = np.array(grad_params, dtype=np.int32)
     poola = np.array(pool, dtype=np.float64)
 
@@ -483,7 +502,6 @@ def encode(n, passes, grad=False, n_params=0):
     h['magic'], h['version'], h['n_qubits'] = MAGIC, ABI_VERSION, n
     h['n_passes'], h['n_stages'], h['n_ops'] = len(pass_rows), len(stages), len(opsa)
     h['n_grad_slots'], h['n_pool'] = len(grads), len(poola)
-    used = [int(o['param']) + 1 for o in op_rows if int(o['kind']) in (OP_ROT1, OP_ROT2)]
     h['n_params'] = max([int(n_params)] + used) if used else 0
     for name in offs:
         h['off_' + name] = offs[name]
@@ -516,12 +534,14 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
         return encode(n, passes, grad, n_params)
     tb = min(cfg.tb, n - RB)
     cfg = TileConfig(tb=tb, low_bits=max(1, cfg.low_bits), max_cost=cfg.max_cost,
-                     prefetch=cfg.prefetch and not grad)
+                     prefetch=cfg.prefetch and not grad, specialize=cfg.specialize)
     passes = partition(ops, n, cfg)
     for p in passes:
         if p.kind == PASS_TILE:
             _stage_ops(p)
-    return encode(n, passes, grad, n_params)
+    prog = encode(n, passes, grad, n_params)
+    prog.specialize = bool(cfg.specialize) and not grad
+    return prog
 
 
 def _dense_bits(op):

@@ -1,0 +1,168 @@
+"""Program specialisation: one generated kernel per tile pass.
+
+The generic tile kernel interprets its op list: every op is a run-time switch
+over (kind, register bits).  On gfx950 that costs about a register-file copy
+per op at the switch merge.  A program is fixed once compiled (θ and matrices
+stay in device memory), so its forward passes can instead be spelled out as
+straight-line code: this module writes one ``svh_sp_<p>`` kernel per tile
+pass, built from the device pieces in csrc/svh_tile_ops.h, compiles the file
+with hipcc into a code object (cached by content hash), and the library
+launches those kernels for the passes they cover (``svh_run_program_sp``).
+"""
+import hashlib
+import os
+import subprocess
+
+import numpy as np
+
+from . import program as P
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+CACHE_DIR = os.environ.get("QARAVAN_SP_CACHE", os.path.join(HERE, "_sp_cache"))
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("QARAVAN_HIP_ARCH", "gfx950")
+FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "--genco",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I" + CSRC]
+MAX_LDS = 64 * 1024
+
+
+def _decode(blob):
+    h = np.frombuffer(blob[:P.HDR_DT.itemsize].tobytes(), dtype=P.HDR_DT)[0]
+
+    def arr(name, dt, count):
+        off = int(h['off_' + name])
+        return np.frombuffer(blob[off:off + dt.itemsize * count].tobytes(), dtype=dt)
+    return (h, arr('passes', P.PASS_DT, int(h['n_passes'])), arr('stages', P.STAGE_DT, int(h['n_stages'])),
+            arr('ops', P.OP_DT, int(h['n_ops'])))
+
+
+def eligible(p, dtype_code):
+    es = 8 if dtype_code == 0 else 16
+    return (int(p['kind']) == P.PASS_TILE and int(p['rb']) == 4
+            and (2 ** int(p['L'])) * es <= MAX_LDS and int(p['n_grad']) == 0)
+
+
+def program_source(blob, dtype_code, only=None):
+    """HIP source with one kernel per eligible tile pass of ``blob`` (restricted
+    to the pass indices in ``only`` when given)."""
+    h, passes, stages, ops = _decode(blob)
+    R = "float" if dtype_code == 0 else "double"
+    out = ['#include "svh_tile_ops.h"', "using namespace svh;", ""]
+    for pi, p in enumerate(passes):
+        if not eligible(p, dtype_code) or (only is not None and pi not in only):
+            continue
+        tb = int(p['tb'])
+        s0, ns = int(p['stage_begin']), int(p['n_stages'])
+        op0 = int(stages[s0]['op_begin'])
+        out.append(f'extern "C" __global__ void __launch_bounds__({1 << tb}) '
+                   f"svh_sp_{pi}(cpx<{R}> *__restrict__ psi, int n, const svh_pass_t *__restrict__ pass, "
+                   f"const svh_stage_t *__restrict__ st, const svh_op_t *__restrict__ ops, "
+                   f"const cpx<{R}> *__restrict__ trig) {{")
+        out.append("    extern __shared__ unsigned char smem_raw[];")
+        out.append(f"    cpx<{R}> *lds = reinterpret_cast<cpx<{R}> *>(smem_raw);")
+        out.append(f"    cpx<{R}> v[16];")
+        out.append(f"    const uint64_t base = sp_base<{R}, {tb}>(pass, n);")
+        out.append(f"    sp_load<{R}, {tb}>(psi, base, st[0], v);")
+        for s in range(ns):
+            S = stages[s0 + s]
+            if s > 0:
+                first = "true" if s == 1 else "false"
+                out.append(f"    sp_exchange<{R}, {first}>(lds, st[{s - 1}], st[{s}], v);")
+            for oi in range(int(S['op_begin']), int(S['op_end'])):
+                o = ops[oi]
+                k = oi - op0
+                kind, a, b, pauli = int(o['kind']), int(o['a']), int(o['b']), int(o['pauli'])
+                if kind == P.OP_MAT1:
+                    out.append(f"    sp_mat1<{R}, {a}>(v, ops[{k}]);")
+                elif kind == P.OP_MAT2:
+                    out.append(f"    sp_mat2<{R}, {a}, {b}>(v, ops[{k}]);")
+                elif kind == P.OP_CX:
+                    hi, lo = max(a, b), min(a, b)
+                    th = "true" if a > b else "false"
+                    out.append(f"    ap_cx_t<{R}, 4, {hi}, {lo}, {th}>(v);")
+                elif kind == P.OP_CZ:
+                    out.append(f"    ap_cz<{R}, 4, {max(a, b)}, {min(a, b)}>(v);")
+                elif kind == P.OP_ROT1:
+                    out.append(f"    sp_rot1<{R}, {a}, {pauli}>(v, ops[{k}], trig);")
+                elif kind == P.OP_ROT2:
+                    out.append(f"    sp_rot2<{R}, {a}, {b}, {pauli}>(v, ops[{k}], trig);")
+                else:
+                    raise ValueError(kind)
+        out.append(f"    sp_store<{R}, {tb}>(psi, base, st[{ns - 1}], v);")
+        out.append("}")
+        out.append("")
+    return "\n".join(out)
+
+
+def _headers_digest():
+    m = hashlib.sha1()
+    for f in ("svh_common.h", "svh_tile_ops.h"):
+        with open(os.path.join(CSRC, f), "rb") as fh:
+            m.update(fh.read())
+    with open(os.path.join(HERE, "..", "include", "qaravan_hip.h"), "rb") as fh:
+        m.update(fh.read())
+    return m.hexdigest()
+
+
+def _compile_one(src, tag):
+    flags = " ".join(f for f in FLAGS if not f.startswith("-I"))
+    key = hashlib.sha1((src + _headers_digest() + flags).encode()).hexdigest()[:20]
+    co = os.path.join(CACHE_DIR, f"sp_{key}.hsaco")
+    if os.path.exists(co):
+        return co
+    srcf = os.path.join(CACHE_DIR, f"sp_{key}.hip")
+    with open(srcf, "w") as f:
+        f.write(src)
+    tmp = co + f".tmp{os.getpid()}_{tag}"
+    r = subprocess.run([HIPCC, *FLAGS, srcf, "-o", tmp], capture_output=True, text=True)
+    if r.returncode != 0:
+        err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
+        raise RuntimeError("specialised program compile failed:\n" + err[-4000:])
+    os.replace(tmp, co)
+    return co
+
+
+def compile_program(blob, dtype_code, chunks=None):
+    """Code-object paths specialising ``blob`` (compiled on first use, in
+    ``chunks`` parallel hipcc runs, cached by content hash)."""
+    from concurrent.futures import ThreadPoolExecutor
+    _, passes, _, _ = _decode(blob)
+    todo = [pi for pi, p in enumerate(passes) if eligible(p, dtype_code)]
+    if not todo:
+        return []
+    os.makedirs(CACHE_DIR, exist_ok=True)
+    k = chunks or min(8, os.cpu_count() or 4, len(todo))
+    groups = [set(todo[i::k]) for i in range(k)]
+    srcs = [program_source(blob, dtype_code, only=g) for g in groups if g]
+    with ThreadPoolExecutor(max_workers=len(srcs)) as ex:
+        return list(ex.map(lambda a: _compile_one(a[1], a[0]), enumerate(srcs)))
+
+
+class Specialized:
+    """Loaded kernels of one specialised program (frees the modules on close)."""
+
+    def __init__(self, prog, dtype_code):
+        import ctypes
+
+        from . import _lib
+        self.paths = compile_program(prog.blob, dtype_code)
+        lib = _lib.load()
+        arr = (ctypes.c_char_p * max(1, len(self.paths)))(*[p.encode() for p in self.paths])
+        h = ctypes.c_void_p()
+        if self.paths:
+            _lib.check(lib.svh_sp_load(arr, len(self.paths), prog.n_passes, ctypes.byref(h)), "svh_sp_load")
+        self.handle = h if self.paths else None
+        self.n_kernels = sum(1 for p in _decode(prog.blob)[1] if eligible(p, dtype_code))
+
+    def close(self):
+        if self.handle:
+            from . import _lib
+            _lib.load().svh_sp_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

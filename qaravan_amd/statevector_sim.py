@@ -74,6 +74,12 @@ def run_program(state, prog, scratch, theta=None):
     buf = scratch.get(need.value)
     dev = prog.device_copy(state.device)
     th = None if theta is None else _lib.ptr(theta)
+    sp = prog.specialized(code) if getattr(prog, "specialize", False) else None
+    if sp is not None and sp.handle:
+        _lib.check(lib.svh_run_program_sp(sp.handle, _lib.ptr(state), n, code, prog.blob.ctypes.data,
+                                          _lib.ptr(dev), th, _lib.ptr(buf), buf.numel(),
+                                          _lib.stream_ptr(state.device)), "svh_run_program_sp")
+        return
     _lib.check(lib.svh_run_program(_lib.ptr(state), n, code, prog.blob.ctypes.data, _lib.ptr(dev), th,
                                    _lib.ptr(buf), buf.numel(), _lib.stream_ptr(state.device)),
                "svh_run_program")

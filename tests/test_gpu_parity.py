@@ -105,7 +105,9 @@ CFGS = [P.TileConfig(tb=4, low_bits=6, max_cost=96), P.TileConfig(tb=6, low_bits
         P.TileConfig(tb=10, low_bits=5, max_cost=64),
         P.TileConfig(tb=6, low_bits=1, max_cost=48, prefetch=True),
         P.TileConfig(tb=8, low_bits=4, max_cost=1e9, prefetch=True),
-        P.TileConfig(tb=9, low_bits=3, max_cost=96, prefetch=True)]
+        P.TileConfig(tb=9, low_bits=3, max_cost=96, prefetch=True),
+        P.TileConfig(tb=8, low_bits=4, max_cost=1e9, specialize=True),
+        P.TileConfig(tb=6, low_bits=3, max_cost=48, specialize=True)]
 
 
 @pytest.mark.parametrize("ci", range(len(CFGS)))
@@ -175,12 +177,13 @@ def test_expectations_vs_oracle(gpu):
     assert abs(sim.local_expectation(ops) - np.real(np.vdot(ref, full))) < 1e-12
 
 
-@pytest.mark.parametrize("prefetch", [False, True])
-def test_brickwork_24q_c64_vs_c128_and_inverse(gpu, prefetch):
+@pytest.mark.parametrize("variant", ["generic", "prefetch", "specialize"])
+def test_brickwork_24q_c64_vs_c128_and_inverse(gpu, variant):
     """Size-independent checks at 24 qubits: c64 agrees with c128, the norm is
     kept, and running the circuit then its inverse returns |0…0⟩."""
     circ = brickwork_circuit(24, 12)
-    cfg = P.TileConfig(tb=8, low_bits=4, max_cost=1e9, prefetch=prefetch)
+    cfg = P.TileConfig(tb=8, low_bits=4, max_cost=1e9, prefetch=variant == "prefetch",
+                       specialize=variant == "specialize")
     _, a = sim_state(circ, gpu, C128, cfg)
     _, b = sim_state(circ, gpu, C64, cfg)
     assert abs(np.linalg.norm(a) - 1) < 1e-12
