@@ -216,6 +216,10 @@ int svh_sp_free(void *handle);
 int svh_run_program_sp(void *handle, void *psi, int n, int dtype, const void *prog_host,
                        const void *prog_dev, const double *theta_dev, void *scratch,
                        size_t scratch_bytes, void *stream);
+/* Adjoint sweep with the kernels svh_spd_<p> of a specialised adjoint program. */
+int svh_run_adjoint_sp(void *handle, void *psi, void *lam, int n, int dtype, const void *prog_host,
+                       const void *prog_dev, const double *theta_dev, double *grad_dev, void *scratch,
+                       size_t scratch_bytes, void *stream);
 
 /* Adjoint sweep: the program holds the reversed, daggered gate list with
  * SVH_OPF_GRAD on rotations.  Walks ψ and λ back together and ADDS

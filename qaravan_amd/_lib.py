@@ -37,6 +37,8 @@ SIGNATURES = {
     "svh_sp_free": (_c_int, [_c_void_p]),
     "svh_run_program_sp": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p,
                                     _c_void_p, _c_size_t, _c_void_p]),
+    "svh_run_adjoint_sp": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p,
+                                    _c_void_p, _c_void_p, _c_void_p, _c_size_t, _c_void_p]),
     "svh_run_adjoint": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p,
                                  _c_void_p, _c_void_p, _c_size_t, _c_void_p]),
     "svh_norm2": (_c_int, [_c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_size_t, _c_void_p]),

@@ -29,7 +29,7 @@ def _adjoint_cfg(n, dtype, cfg):
     registers (c64: ≤ 512 threads, c128: ≤ 256)."""
     tb_max = 9 if dtype == torch.complex64 else 8
     tb = min(cfg.tb, tb_max, n - 4)
-    return TileConfig(tb=tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost)
+    return TileConfig(tb=tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost, specialize=cfg.specialize)
 
 
 class ParamProgram:
@@ -54,7 +54,8 @@ class ParamProgram:
             if op.kind in ('rot1', 'rot2') and not 0 <= op.param < self.n_params:
                 raise ValueError(f"param index {op.param} outside [0, {self.n_params})")
         self.fwd = compile_ops(ops, self.n, TileConfig(tb=min(cfg.tb, self.n - 4), low_bits=cfg.low_bits,
-                                                       max_cost=cfg.max_cost), n_params=self.n_params)
+                                                       max_cost=cfg.max_cost, specialize=cfg.specialize),
+                               n_params=self.n_params)
         back = []
         for op in reversed(ops):
             d = op.dagger()
@@ -96,6 +97,12 @@ def adjoint_sweep(pp, psi, lam, th, scratch):
                "svh_program_scratch_bytes")
     buf = scratch.get(need.value)
     dev = pp.adj.device_copy(psi.device)
+    sp = pp.adj.specialized(code, dual=True) if pp.adj.specialize else None
+    if sp is not None and sp.handle:
+        _lib.check(lib.svh_run_adjoint_sp(sp.handle, _lib.ptr(psi), _lib.ptr(lam), n, code, pp.adj.blob.ctypes.data,
+                                          _lib.ptr(dev), _lib.ptr(th), _lib.ptr(grad), _lib.ptr(buf), buf.numel(),
+                                          _lib.stream_ptr(psi.device)), "svh_run_adjoint_sp")
+        return grad
     _lib.check(lib.svh_run_adjoint(_lib.ptr(psi), _lib.ptr(lam), n, code, pp.adj.blob.ctypes.data,
                                    _lib.ptr(dev), _lib.ptr(th), _lib.ptr(grad), _lib.ptr(buf), buf.numel(),
                                    _lib.stream_ptr(psi.device)), "svh_run_adjoint")

@@ -520,18 +520,26 @@ struct SpHandle {
     std::vector<hipFunction_t> fn;  // per pass, nullptr = not specialised
 };
 
-static int launch_sp(hipFunction_t f, void *psi, int n, int dtype, const svh_pass_t &hp, const svh_pass_t *dp,
-                     const svh_stage_t *dst, const svh_op_t *dop, const svh_stage_t *hst, const void *trig,
-                     hipStream_t st) {
+// Launch a specialised pass: svh_sp_<p>(psi, n, pass, st, ops, trig) or, for
+// the adjoint, svh_spd_<p>(psi, lam, n, pass, st, ops, trig, gpart).
+static int launch_sp(hipFunction_t f, void *psi, void *lam, int n, int dtype, const svh_pass_t &hp,
+                     const svh_pass_t *dp, const svh_stage_t *dst, const svh_op_t *dop, const svh_stage_t *hst,
+                     const void *trig, double *gpart, hipStream_t st) {
     const int L = hp.L;
     const size_t es = dtype == SVH_C64 ? 8 : 16;
-    const size_t lds = es * ((size_t)1 << L);
+    const size_t nw = ((size_t)1 << hp.tb) >= 64 ? ((size_t)1 << hp.tb) / 64 : 1;
+    const size_t lds = (lam ? 2 : 1) * es * ((size_t)1 << L) + (lam ? 8 * nw * (size_t)hp.n_grad : 0);
     if (lds > 65536) return fail(-7, "specialised pass needs more than 64 KiB of LDS");
     const svh_stage_t *stp = dst + hp.stage_begin;
     const svh_op_t *opp = dop + hst[hp.stage_begin].op_begin;
     int nn = n;
-    void *args[] = {&psi, &nn, (void *)&dp, (void *)&stp, (void *)&opp, (void *)&trig};
     const unsigned grid = (unsigned)(1ull << (n - L));
+    if (lam) {
+        void *args[] = {&psi, &lam, &nn, (void *)&dp, (void *)&stp, (void *)&opp, (void *)&trig, (void *)&gpart};
+        return hip_status(hipModuleLaunchKernel(f, grid, 1, 1, 1u << hp.tb, 1, 1, (unsigned)lds, st, args, nullptr),
+                          "hipModuleLaunchKernel(svh_spd)");
+    }
+    void *args[] = {&psi, &nn, (void *)&dp, (void *)&stp, (void *)&opp, (void *)&trig};
     return hip_status(hipModuleLaunchKernel(f, grid, 1, 1, 1u << hp.tb, 1, 1, (unsigned)lds, st, args, nullptr),
                       "hipModuleLaunchKernel(svh_sp)");
 }
@@ -575,8 +583,14 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
     for (int p = 0; p < h->n_passes; ++p) {
         const svh_pass_t &P = hpass[p];
         int e = 0;
-        if (P.kind == SVH_PASS_TILE && !dual && sp && sp->fn[p]) {
-            e = launch_sp(sp->fn[p], psi, n, dtype, P, dpass + p, dstage, dop, hstage, trig, st);
+        if (P.kind == SVH_PASS_TILE && sp && sp->fn[p]) {
+            e = launch_sp(sp->fn[p], psi, dual ? lam : nullptr, n, dtype, P, dpass + p, dstage, dop, hstage, trig,
+                          sd, st);
+            if (!e && dual && P.n_grad > 0) {
+                const int nb = (int)(1ull << (n - P.L));
+                k_grad_reduce<<<P.n_grad, 256, 0, st>>>(sd, nb, P.n_grad, dgrad + P.grad_begin, grad_dev);
+                e = checked_launch("k_grad_reduce");
+            }
         } else if (P.kind == SVH_PASS_TILE) {
             if (dtype == SVH_C64)
                 e = dual ? launch_tile<float, true>((cpx<float> *)psi, (cpx<float> *)lam, n, P, dpass + p, dstage,
@@ -709,7 +723,9 @@ int svh_sp_load(const char *const *paths, int npaths, int n_passes, void **handl
             if (H->fn[p]) continue;
             hipFunction_t f = nullptr;
             const std::string name = "svh_sp_" + std::to_string(p);
+            const std::string named = "svh_spd_" + std::to_string(p);
             if (hipModuleGetFunction(&f, m, name.c_str()) == hipSuccess) H->fn[p] = f;
+            else if (hipModuleGetFunction(&f, m, named.c_str()) == hipSuccess) H->fn[p] = f;
         }
     }
     (void)hipGetLastError();  // lookups of absent kernels are expected
@@ -734,6 +750,14 @@ int svh_run_program_sp(void *handle, void *psi, int n, int dtype, const void *pr
     if (!handle) return fail(-1, "specialised handle is NULL");
     return run_impl(psi, nullptr, n, dtype, prog_host, prog_dev, theta_dev, nullptr, scratch, scratch_bytes,
                     stream, false, static_cast<const SpHandle *>(handle));
+}
+
+int svh_run_adjoint_sp(void *handle, void *psi, void *lam, int n, int dtype, const void *prog_host,
+                       const void *prog_dev, const double *theta_dev, double *grad_dev, void *scratch,
+                       size_t scratch_bytes, void *stream) {
+    if (!handle) return fail(-1, "specialised handle is NULL");
+    return run_impl(psi, lam, n, dtype, prog_host, prog_dev, theta_dev, grad_dev, scratch, scratch_bytes, stream,
+                    true, static_cast<const SpHandle *>(handle));
 }
 
 int svh_run_adjoint(void *psi, void *lam, int n, int dtype, const void *prog_host, const void *prog_dev,

@@ -382,12 +382,13 @@ class Program:
         self._sp = {}
         self.specialize = False
 
-    def specialized(self, dtype_code):
+    def specialized(self, dtype_code, dual=False):
         """Loaded straight-line kernels of this program (compiled on first use)."""
-        if dtype_code not in self._sp:
+        key = (dtype_code, dual)
+        if key not in self._sp:
             from .specialize import Specialized
-            self._sp[dtype_code] = Specialized(self, dtype_code)
-        return self._sp[dtype_code]
+            self._sp[key] = Specialized(self, dtype_code, dual)
+        return self._sp[key]
 
     @property
     def n_passes(self):
@@ -540,7 +541,7 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
         if p.kind == PASS_TILE:
             _stage_ops(p)
     prog = encode(n, passes, grad, n_params)
-    prog.specialize = bool(cfg.specialize) and not grad
+    prog.specialize = bool(cfg.specialize)
     return prog
 
 

@@ -37,10 +37,74 @@ def _decode(blob):
             arr('ops', P.OP_DT, int(h['n_ops'])))
 
 
-def eligible(p, dtype_code):
+def lds_bytes(p, dtype_code, dual):
     es = 8 if dtype_code == 0 else 16
+    nw = max(1, (2 ** int(p['tb'])) // 64)
+    return (2 if dual else 1) * (2 ** int(p['L'])) * es + (8 * nw * int(p['n_grad']) if dual else 0)
+
+
+def eligible(p, dtype_code, dual=False):
     return (int(p['kind']) == P.PASS_TILE and int(p['rb']) == 4
-            and (2 ** int(p['L'])) * es <= MAX_LDS and int(p['n_grad']) == 0)
+            and lds_bytes(p, dtype_code, dual) <= MAX_LDS and (dual or int(p['n_grad']) == 0))
+
+
+def adjoint_source(blob, dtype_code, only=None):
+    """Adjoint (ψ, λ) kernels ``svh_spd_<p>`` for the tile passes of an adjoint
+    program: gradient ops accumulate 2·Im⟨λ|P|ψ⟩ before applying U† to both."""
+    h, passes, stages, ops = _decode(blob)
+    R = "float" if dtype_code == 0 else "double"
+    out = ['#include "svh_tile_ops.h"', "using namespace svh;", ""]
+    for pi, p in enumerate(passes):
+        if not eligible(p, dtype_code, True) or (only is not None and pi not in only):
+            continue
+        tb = int(p['tb'])
+        s0, ns = int(p['stage_begin']), int(p['n_stages'])
+        op0 = int(stages[s0]['op_begin'])
+        ng = int(p['n_grad'])
+        out.append(f'extern "C" __global__ void __launch_bounds__({1 << tb}) '
+                   f"svh_spd_{pi}(cpx<{R}> *__restrict__ psi, cpx<{R}> *__restrict__ lam, int n, "
+                   f"const svh_pass_t *__restrict__ pass, const svh_stage_t *__restrict__ st, "
+                   f"const svh_op_t *__restrict__ ops, const cpx<{R}> *__restrict__ trig, "
+                   f"double *__restrict__ gpart) {{")
+        out.append("    extern __shared__ unsigned char smem_raw[];")
+        out.append(f"    cpx<{R}> *lds = reinterpret_cast<cpx<{R}> *>(smem_raw);")
+        out.append(f"    cpx<{R}> *lds2 = lds + {2 ** int(p['L'])};")
+        out.append(f"    double *gacc = reinterpret_cast<double *>(lds2 + {2 ** int(p['L'])});")
+        out.append(f"    cpx<{R}> v[16], w[16];")
+        out.append(f"    const uint64_t base = sp_base<{R}, {tb}>(pass, n);")
+        out.append(f"    sp_load2<{R}, {tb}>(psi, lam, base, st[0], v, w);")
+        for s in range(ns):
+            S = stages[s0 + s]
+            if s > 0:
+                first = "true" if s == 1 else "false"
+                out.append(f"    sp_exchange2<{R}, {first}>(lds, lds2, st[{s - 1}], st[{s}], v, w);")
+            for oi in range(int(S['op_begin']), int(S['op_end'])):
+                o = ops[oi]
+                k = oi - op0
+                kind, a, b, pauli = int(o['kind']), int(o['a']), int(o['b']), int(o['pauli'])
+                g = "true" if int(o['flags']) & P.OPF_GRAD else "false"
+                if kind == P.OP_MAT1:
+                    out.append(f"    sp_mat1<{R}, {a}>(v, ops[{k}]); sp_mat1<{R}, {a}>(w, ops[{k}]);")
+                elif kind == P.OP_MAT2:
+                    out.append(f"    sp_mat2<{R}, {a}, {b}>(v, ops[{k}]); sp_mat2<{R}, {a}, {b}>(w, ops[{k}]);")
+                elif kind == P.OP_CX:
+                    hi, lo = max(a, b), min(a, b)
+                    th = "true" if a > b else "false"
+                    out.append(f"    ap_cx_t<{R}, 4, {hi}, {lo}, {th}>(v); ap_cx_t<{R}, 4, {hi}, {lo}, {th}>(w);")
+                elif kind == P.OP_CZ:
+                    out.append(f"    ap_cz<{R}, 4, {max(a, b)}, {min(a, b)}>(v); ap_cz<{R}, 4, {max(a, b)}, {min(a, b)}>(w);")
+                elif kind == P.OP_ROT1:
+                    out.append(f"    spd_rot1<{R}, {tb}, {a}, {pauli}, {g}>(v, w, ops[{k}], trig, gacc, {ng});")
+                elif kind == P.OP_ROT2:
+                    out.append(f"    spd_rot2<{R}, {tb}, {a}, {b}, {pauli}, {g}>(v, w, ops[{k}], trig, gacc, {ng});")
+                else:
+                    raise ValueError(kind)
+        out.append(f"    sp_store2<{R}, {tb}>(psi, lam, base, st[{ns - 1}], v, w);")
+        if ng:
+            out.append(f"    sp_grad_flush<{tb}>(gacc, {ng}, gpart);")
+        out.append("}")
+        out.append("")
+    return "\n".join(out)
 
 
 def program_source(blob, dtype_code, only=None):
@@ -123,18 +187,19 @@ def _compile_one(src, tag):
     return co
 
 
-def compile_program(blob, dtype_code, chunks=None):
+def compile_program(blob, dtype_code, chunks=None, dual=False):
     """Code-object paths specialising ``blob`` (compiled on first use, in
     ``chunks`` parallel hipcc runs, cached by content hash)."""
     from concurrent.futures import ThreadPoolExecutor
     _, passes, _, _ = _decode(blob)
-    todo = [pi for pi, p in enumerate(passes) if eligible(p, dtype_code)]
+    todo = [pi for pi, p in enumerate(passes) if eligible(p, dtype_code, dual)]
     if not todo:
         return []
     os.makedirs(CACHE_DIR, exist_ok=True)
     k = chunks or min(8, os.cpu_count() or 4, len(todo))
     groups = [set(todo[i::k]) for i in range(k)]
-    srcs = [program_source(blob, dtype_code, only=g) for g in groups if g]
+    gen = adjoint_source if dual else program_source
+    srcs = [gen(blob, dtype_code, only=g) for g in groups if g]
     with ThreadPoolExecutor(max_workers=len(srcs)) as ex:
         return list(ex.map(lambda a: _compile_one(a[1], a[0]), enumerate(srcs)))
 
@@ -142,18 +207,18 @@ def compile_program(blob, dtype_code, chunks=None):
 class Specialized:
     """Loaded kernels of one specialised program (frees the modules on close)."""
 
-    def __init__(self, prog, dtype_code):
+    def __init__(self, prog, dtype_code, dual=False):
         import ctypes
 
         from . import _lib
-        self.paths = compile_program(prog.blob, dtype_code)
+        self.paths = compile_program(prog.blob, dtype_code, dual=dual)
         lib = _lib.load()
         arr = (ctypes.c_char_p * max(1, len(self.paths)))(*[p.encode() for p in self.paths])
         h = ctypes.c_void_p()
         if self.paths:
             _lib.check(lib.svh_sp_load(arr, len(self.paths), prog.n_passes, ctypes.byref(h)), "svh_sp_load")
         self.handle = h if self.paths else None
-        self.n_kernels = sum(1 for p in _decode(prog.blob)[1] if eligible(p, dtype_code))
+        self.n_kernels = sum(1 for p in _decode(prog.blob)[1] if eligible(p, dtype_code, dual))
 
     def close(self):
         if self.handle:

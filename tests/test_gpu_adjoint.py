@@ -24,16 +24,17 @@ def hea_spec(n, layers):
     return spec
 
 
-@pytest.mark.parametrize("n,layers,tb", [(8, 2, 4), (10, 3, 6), (12, 2, 8)])
+@pytest.mark.parametrize("n,layers,tb,spz", [(8, 2, 4, False), (10, 3, 6, False), (12, 2, 8, False),
+                                           (10, 3, 6, True), (12, 2, 5, True)])
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
-def test_adjoint_vs_oracle(gpu, n, layers, tb, dtype):
+def test_adjoint_vs_oracle(gpu, n, layers, tb, spz, dtype):
     from qaravan_amd.adjoint import AdjointModel
     theta = hea_theta(n, layers)
     circ = hea_circuit(n, layers, theta)
     terms = tfim_terms(n)
-    print(f"adjoint n={n} layers={layers} tb={tb} {dtype}", flush=True)
+    print(f"adjoint n={n} layers={layers} tb={tb} specialize={spz} {dtype}", flush=True)
     model = AdjointModel(circ, terms, len(theta), device=gpu, dtype=dtype,
-                         cfg=P.TileConfig(tb=tb, low_bits=6, max_cost=48))
+                         cfg=P.TileConfig(tb=tb, low_bits=6, max_cost=48, specialize=spz))
     th = torch.tensor(theta, device=gpu, requires_grad=True)
     e = model(th)
     e.backward()
