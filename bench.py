@@ -128,11 +128,14 @@ def check_forward(res, dev):
     return err, norm
 
 
-def bench_adjoint(dev, iters=5, tape_host=False):
+def bench_adjoint(dev, iters=10, tape_host=False):
+    """20-qubit, 16-layer HEA (640 parameters), E = ⟨ψ|H_TFIM|ψ⟩: adjoint
+    gradients (eager and HIP-graph replay) vs torch tape autograd."""
     import numpy as np
     import torch
 
     from qaravan_amd.adjoint import AdjointModel
+    from qaravan_amd.autotune import best_adjoint_config
     from qaravan_amd.torch_tape import tape_energy_and_grad
     from qaravan_amd.workloads import hea_circuit, hea_theta, tfim_terms
 
@@ -140,18 +143,26 @@ def bench_adjoint(dev, iters=5, tape_host=False):
     theta = hea_theta(n, layers)
     circ = hea_circuit(n, layers, theta)
     terms = tfim_terms(n)
-    model = AdjointModel(circ, terms, len(theta), device=dev, dtype=torch.complex64)
+    cfg = best_adjoint_config(n, torch.complex64)
+    model = AdjointModel(circ, terms, len(theta), device=dev, dtype=torch.complex64, cfg=cfg)
     th = torch.tensor(theta, device=dev)
-    for _ in range(2):
-        model.energy_and_grad(th)
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    for _ in range(iters):
-        e, g = model.energy_and_grad(th)
-    torch.cuda.synchronize(dev)
-    t_adj = (time.perf_counter() - t0) / iters
+
+    def timed(fn, k):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        for _ in range(k):
+            out = fn()
+        torch.cuda.synchronize(dev)
+        return (time.perf_counter() - t0) / k, out
+
+    t_eager, _ = timed(lambda: model.energy_and_grad(th), iters)
+    model.capture(th)
+    t_graph, (e, g) = timed(lambda: model.replay(th), iters)
     g_adj = g.cpu().numpy()
     e_adj = float(e.item())
+    t_adj = min(t_eager, t_graph)
 
     # c128 adjoint on the GPU as the accuracy reference
     m128 = AdjointModel(circ, terms, len(theta), device=dev, dtype=torch.complex128)
@@ -159,40 +170,37 @@ def bench_adjoint(dev, iters=5, tape_host=False):
     g128 = g128.cpu().numpy()
     rel_err = float(np.linalg.norm(g_adj - g128) / np.linalg.norm(g128))
 
-    th_cpu = torch.tensor(theta, dtype=torch.float64)
-    th_small = torch.tensor(hea_theta(12, 2), dtype=torch.float64)
-    t_tape = tape_err = None
-    if tape_host:
-        # warm the torch kernels on a 12-qubit copy of the ansatz, then time one
-        # full gradient at 20 qubits (the host-θ tape takes about a minute)
-        tape_energy_and_grad(hea_circuit(12, 2, th_small.numpy()), th_small, tfim_terms(12), dev)
+    th_small = torch.tensor(hea_theta(12, 2), dtype=torch.float64, device=dev)
+    small = hea_circuit(12, 2, th_small.cpu().numpy())
+    th_dev = torch.tensor(theta, dtype=torch.float64, device=dev)
+    res = {}
+    for kernel, k in (("elementwise", 2), ("tensordot", 1)):
+        tape_energy_and_grad(small, th_small, tfim_terms(12), dev, kernel)  # warm the torch kernels
         torch.cuda.synchronize(dev)
-        log("[bench] torch tape (host θ): timing one 20-qubit gradient")
+        log(f"[bench] torch tape ({kernel}, device θ): timing 20-qubit gradients")
+        t0 = time.perf_counter()
+        for _ in range(k):
+            e_t, g_t = tape_energy_and_grad(circ, th_dev, terms, dev, kernel)
+        torch.cuda.synchronize(dev)
+        dt = (time.perf_counter() - t0) / k
+        res[kernel] = (dt, float(np.linalg.norm(g_t.cpu().numpy() - g128) / np.linalg.norm(g128)))
+    out = dict(adjoint_grads_per_s=1.0 / t_adj, adjoint_eager_ms=t_eager * 1e3, adjoint_graph_ms=t_graph * 1e3,
+               tile=dict(tb=cfg.tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost, specialize=cfg.specialize),
+               rel_grad_err_vs_c128=rel_err,
+               torch_autograd_grads_per_s=1.0 / res["tensordot"][0],
+               speedup=res["tensordot"][0] / t_adj,
+               torch_autograd_rel_grad_err_vs_c128=res["tensordot"][1],
+               torch_autograd_elementwise_grads_per_s=1.0 / res["elementwise"][0],
+               speedup_vs_elementwise_tape=res["elementwise"][0] / t_adj,
+               torch_autograd_elementwise_rel_grad_err_vs_c128=res["elementwise"][1],
+               energy=e_adj, energy_c128=float(e128.item()),
+               n_params=len(theta), n_gates=len(circ.gate_list))
+    if tape_host:
+        th_cpu = torch.tensor(theta, dtype=torch.float64)
         t0 = time.perf_counter()
         e_t, g_t = tape_energy_and_grad(circ, th_cpu, terms, dev)
         torch.cuda.synchronize(dev)
-        t_tape = time.perf_counter() - t0
-        tape_err = float(np.linalg.norm(g_t.numpy() - g128) / np.linalg.norm(g128))
-    # the same tape with θ and every gate matrix on the GPU (no host round trips)
-    th_dev = torch.tensor(theta, dtype=torch.float64, device=dev)
-    tape_energy_and_grad(hea_circuit(12, 2, th_small.numpy()), th_small.to(dev), tfim_terms(12), dev)
-    torch.cuda.synchronize(dev)
-    log("[bench] torch tape (device θ): timing 20-qubit gradients")
-    t0 = time.perf_counter()
-    dev_iters = 2
-    for _ in range(dev_iters):
-        e_d, g_d = tape_energy_and_grad(circ, th_dev, terms, dev)
-    torch.cuda.synchronize(dev)
-    t_dev = (time.perf_counter() - t0) / dev_iters
-    dev_err = float(np.linalg.norm(g_d.cpu().numpy() - g128) / np.linalg.norm(g128))
-    out = dict(adjoint_grads_per_s=1.0 / t_adj, torch_autograd_grads_per_s=1.0 / t_dev,
-               speedup=t_dev / t_adj, rel_grad_err_vs_c128=rel_err,
-               torch_autograd_rel_grad_err_vs_c128=dev_err,
-               energy=e_adj, energy_c128=float(e128.item()),
-               n_params=len(theta), n_gates=len(circ.gate_list))
-    if t_tape is not None:
-        out.update(torch_autograd_host_theta_grads_per_s=1.0 / t_tape, speedup_vs_host_theta_tape=t_tape / t_adj,
-                   torch_autograd_host_theta_rel_grad_err_vs_c128=tape_err)
+        out.update(torch_autograd_host_theta_grads_per_s=1.0 / (time.perf_counter() - t0))
     return out
 
 

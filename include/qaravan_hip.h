@@ -275,6 +275,12 @@ int svh_batch_trig(const double *theta_dev, int64_t np, void *trig_dev, int dtyp
 /* out_dev[0] = wall clock (s_memrealtime, 100 MHz ticks), for batch latencies. */
 int svh_timestamp(uint64_t *out_dev, void *stream);
 
+/* The same with the term table already in device memory (32 bytes per term:
+ * uint64 xmask, uint64 zmask, double Re(c·i^{ny}), double Im(c·i^{ny})).  No host
+ * copies, so it can be recorded in a HIP graph. */
+int svh_pauli_apply_table(const void *psi, void *out, int n, int dtype, int n_terms, const void *terms_dev,
+                          void *stream);
+
 /* out_dev[0..1] = ⟨a|b⟩ (re, im) accumulated in FP64. */
 int svh_inner(const void *a, const void *b, int n, int dtype, double *out_dev,
               void *scratch, size_t scratch_bytes, void *stream);

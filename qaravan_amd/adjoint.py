@@ -21,7 +21,7 @@ import torch
 
 from . import _lib
 from .program import OPF_GRAD, TileConfig, compile_ops, lower_circuit
-from .statevector_sim import DeviceScratch, inner, pauli_sum_apply, run_program
+from .statevector_sim import DeviceScratch, inner, run_program
 
 
 def _adjoint_cfg(n, dtype, cfg):
@@ -67,6 +67,19 @@ class ParamProgram:
         self.n_gates = sum(1 for _ in ops)
 
 
+def term_table(n, terms):
+    """Device-layout Pauli-sum table: (xmask, zmask, Re, Im of c·i^{ny}) per term."""
+    dt = np.dtype([('x', '<u8'), ('z', '<u8'), ('cre', '<f8'), ('cim', '<f8')])
+    from .statevector_sim import _pauli_masks
+    xm, zm = _pauli_masks(n, [p for _, p in terms])
+    t = np.zeros(len(terms), dtype=dt)
+    ph = [(1.0, 0.0), (0.0, 1.0), (-1.0, 0.0), (0.0, -1.0)]
+    for i, (c, _) in enumerate(terms):
+        ny = bin(int(xm[i]) & int(zm[i])).count("1") & 3
+        t[i] = (xm[i], zm[i], float(np.real(c)) * ph[ny][0], float(np.real(c)) * ph[ny][1])
+    return t
+
+
 def _theta64(theta, device):
     return theta.detach().to(device=device, dtype=torch.float64).contiguous()
 
@@ -81,7 +94,12 @@ def forward_state(pp, theta, device, scratch):
                "svh_init_basis")
     run_program(psi, pp.fwd, scratch, th)
     lam = torch.empty_like(psi)
-    pauli_sum_apply(psi, lam, n, pp.terms, scratch)
+    key = ("terms", str(device))
+    if key not in pp.__dict__:
+        pp.__dict__[key] = torch.from_numpy(term_table(n, pp.terms).view(np.uint8).copy()).to(device)
+    tt = pp.__dict__[key]
+    _lib.check(lib.svh_pauli_apply_table(_lib.ptr(psi), _lib.ptr(lam), n, _lib.dtype_code(pp.dtype), len(pp.terms),
+                                         _lib.ptr(tt), _lib.stream_ptr(device)), "svh_pauli_apply_table")
     return psi, lam, th
 
 
@@ -145,3 +163,29 @@ class AdjointModel:
         e = inner(psi, lam, self.pp.n, self.scratch).real
         g = adjoint_sweep(self.pp, psi, lam, th, self.scratch)
         return e, g
+
+    def capture(self, theta):
+        """Record one energy+gradient evaluation in a HIP graph.  The sequence
+        reads θ from a static device buffer and copies nothing from the host,
+        so ``replay(θ')`` re-runs it for new angles as one launch."""
+        th = theta.detach().to(device=self.device, dtype=torch.float64).contiguous().clone()
+        for _ in range(2):  # warm-up: compile/load kernels, size the scratch buffer
+            self.energy_and_grad(th)
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                e, grad = self.energy_and_grad(th)
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        torch.cuda.synchronize(self.device)
+        self._graph = (g, th, e, grad)
+        return g
+
+    def replay(self, theta):
+        """(E, dE/dθ) for new θ through the captured graph (capture() first)."""
+        g, th, e, grad = self._graph
+        th.copy_(theta.detach().to(device=self.device, dtype=torch.float64))
+        g.replay()
+        return e, grad

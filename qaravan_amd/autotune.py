@@ -98,3 +98,59 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_
         with open(out_path or CACHE, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)
     return cfg, results
+
+
+ADJOINT_DEFAULT = {"complex64": TileConfig(tb=6, low_bits=4, max_cost=1e9, specialize=True),
+                   "complex128": TileConfig(tb=6, low_bits=4, max_cost=1e9, specialize=True)}
+
+
+def best_adjoint_config(n, dtype):
+    """Tile configuration for forward+adjoint of parametrised circuits (cached
+    under 'adjoint:<dtype>:<n>' when tuned, else the default)."""
+    name = str(dtype).replace('torch.', '')
+    e = _load().get(f"adjoint:{name}:{n}")
+    if e is None:
+        cfg = ADJOINT_DEFAULT[name]
+    else:
+        cfg = TileConfig(tb=e["tb"], low_bits=e["low_bits"], max_cost=e["max_cost"],
+                         specialize=e.get("specialize", False))
+    return TileConfig(tb=min(cfg.tb, n - 4), low_bits=cfg.low_bits, max_cost=cfg.max_cost,
+                      specialize=cfg.specialize)
+
+
+def tune_adjoint(n, layers, dtype, device, space=None, repeats=5, log=print, out_path=None):
+    """Time energy+gradient of the n-qubit hardware-efficient ansatz for each
+    tile configuration in ``space``; cache the fastest."""
+    import torch
+
+    from .adjoint import AdjointModel
+    from .workloads import hea_circuit, hea_theta, tfim_terms
+    space = space or dict(tb=[4, 5, 6, 7, 8], max_cost=[48.0, 1e9], specialize=[False, True])
+    theta = hea_theta(n, layers)
+    circ = hea_circuit(n, layers, theta)
+    th = torch.tensor(theta, device=device)
+    results = []
+    for tb, mc, spz in itertools.product(space["tb"], space["max_cost"], space["specialize"]):
+        if tb + 4 > n:
+            continue
+        cfg = TileConfig(tb=tb, low_bits=4, max_cost=mc, specialize=spz)
+        m = AdjointModel(circ, tfim_terms(n), len(theta), device=device, dtype=dtype, cfg=cfg)
+        for _ in range(2):
+            m.energy_and_grad(th)
+        torch.cuda.synchronize(device)
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            m.energy_and_grad(th)
+        torch.cuda.synchronize(device)
+        t = (time.perf_counter() - t0) / repeats
+        results.append((t, cfg))
+        log(f"[tune-adjoint] n={n} tb={tb} cap={mc:g} specialize={spz}: {t * 1e3:.3f} ms")
+    results.sort(key=lambda x: x[0])
+    t, cfg = results[0]
+    cache = _load()
+    cache[f"adjoint:{str(dtype).replace('torch.', '')}:{n}"] = dict(tb=cfg.tb, low_bits=cfg.low_bits,
+                                                                     max_cost=cfg.max_cost,
+                                                                     specialize=cfg.specialize, ms=t * 1e3)
+    with open(out_path or CACHE, "w") as f:
+        json.dump(cache, f, indent=1, sort_keys=True)
+    return cfg, results

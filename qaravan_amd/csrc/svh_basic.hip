@@ -396,4 +396,21 @@ int svh_pauli_apply(const void *psi, void *out, int n, int dtype, int n_terms,
     return checked_launch("svh_pauli_apply");
 }
 
+int svh_pauli_apply_table(const void *psi, void *out, int n, int dtype, int n_terms, const void *terms_dev,
+                          void *stream) {
+    if (int e = check_state(psi, n, dtype)) return e;
+    if (!out || (n_terms > 0 && !terms_dev)) return fail(-1, "NULL pointer");
+    if (out == psi) return fail(-4, "out must not alias psi");
+    if (n_terms < 0 || n_terms > 4096) return fail(-4, "n_terms must be in [0, 4096]");
+    const uint64_t N = 1ull << n;
+    const unsigned g = grid_for(N, 256, 16384);
+    hipStream_t st = (hipStream_t)stream;
+    const auto *terms = static_cast<const PauliTerm *>(terms_dev);
+    if (dtype == SVH_C64)
+        k_pauli_apply<float><<<g, 256, 0, st>>>((const cpx<float> *)psi, (cpx<float> *)out, N, terms, n_terms);
+    else
+        k_pauli_apply<double><<<g, 256, 0, st>>>((const cpx<double> *)psi, (cpx<double> *)out, N, terms, n_terms);
+    return checked_launch("svh_pauli_apply_table");
+}
+
 }  // extern "C"

@@ -13,6 +13,12 @@ statevector_sim.py:124-140, whose ncon call reduces to exactly that); the loss
 
 It is used only as the comparison point for the adjoint kernels (speed and
 gradient agreement); it is not a fallback of the HIP path.
+
+``kernel="elementwise"`` is a second, stronger tape: the same circuit and loss
+with every 1-qubit gate written as slice arithmetic (y0 = m00·x0 + m01·x1,
+y1 = m10·x0 + m11·x1 on the two halves of the qubit's axis), CZ as a sign
+mask and the Pauli loss by flips and masks.  It avoids tensordot's tiny-GEMM
+shapes (and their reduction-shaped backward) while keeping tape autograd.
 """
 import torch
 
@@ -43,7 +49,52 @@ def rotation_matrix(word, theta):
     return torch.linalg.matrix_exp(-1j * theta.to(torch.complex128) * p)
 
 
-def tape_energy(circ, theta, terms, device):
+def _apply_elementwise(mat, qubits, state):
+    if len(qubits) == 1:
+        q = qubits[0]
+        x0, x1 = state.select(q, 0), state.select(q, 1)
+        y0 = mat[0, 0] * x0 + mat[0, 1] * x1
+        y1 = mat[1, 0] * x0 + mat[1, 1] * x1
+        return torch.stack([y0, y1], dim=q)
+    d = torch.diagonal(mat)
+    if len(qubits) == 2 and torch.count_nonzero(mat - torch.diag(d)) == 0:
+        shape = [1] * state.dim()
+        shape[qubits[0]] = 2
+        shape[qubits[1]] = 2
+        if qubits[0] < qubits[1]:
+            return state * d.reshape(2, 2).reshape(shape)
+        return state * d.reshape(2, 2).t().reshape(shape)
+    return op_action(mat, qubits, state)
+
+
+def _energy_elementwise(state, terms):
+    n = state.dim()
+    e = torch.zeros((), dtype=torch.float32, device=state.device)
+    prob = (state.conj() * state).real
+    for c, ps in terms:
+        right_flip = [q for q, ch in enumerate(ps) if ch in 'xy']
+        zs = [q for q, ch in enumerate(ps) if ch in 'zy']
+        if any(ch == 'y' for ch in ps):
+            raise NotImplementedError("Y terms use the tensordot tape")
+        if right_flip:
+            r = torch.flip(state, right_flip)
+            val = state.conj() * r
+            for q in zs:
+                sh = [1] * n
+                sh[q] = 2
+                val = val * torch.tensor([1.0, -1.0], device=state.device).reshape(sh)
+            e = e + float(c) * val.real.sum()
+        else:
+            val = prob
+            for q in zs:
+                sh = [1] * n
+                sh[q] = 2
+                val = val * torch.tensor([1.0, -1.0], device=state.device).reshape(sh)
+            e = e + float(c) * val.sum()
+    return e
+
+
+def tape_energy(circ, theta, terms, device, kernel="tensordot"):
     """E(θ) through the restated torch backend; differentiable by autograd.
     Gates with ``param_index`` take θ[param_index] (a CPU float64 leaf, as in
     the reference where θ is a torch Parameter on the host)."""
@@ -58,7 +109,12 @@ def tape_energy(circ, theta, terms, device):
             else:
                 mat = torch.as_tensor(g.matrix, dtype=torch.complex128, device=theta.device)
             mat = mat.to(dtype=torch.complex64, device=device)
-            state = op_action(mat, list(g.indices), state)
+            if kernel == "elementwise":
+                state = _apply_elementwise(mat, list(g.indices), state)
+            else:
+                state = op_action(mat, list(g.indices), state)
+    if kernel == "elementwise":
+        return _energy_elementwise(state, terms)
     e = torch.zeros((), dtype=torch.float32, device=device)
     for c, ps in terms:
         right = state
@@ -69,12 +125,12 @@ def tape_energy(circ, theta, terms, device):
     return e
 
 
-def tape_energy_and_grad(circ, theta, terms, device):
+def tape_energy_and_grad(circ, theta, terms, device, kernel="tensordot"):
     """Value and gradient by ``loss.backward()``.  ``theta`` on the CPU is the
     reference's setup (host Parameter, host matrix_exp, one small host->device
     copy per gate); ``theta`` on the GPU keeps the whole tape on the device,
     the strongest form of the tape baseline."""
     th = theta.detach().clone().requires_grad_(True)
-    e = tape_energy(circ, th, terms, device)
+    e = tape_energy(circ, th, terms, device, kernel)
     e.backward()
     return e.detach(), th.grad.detach()
