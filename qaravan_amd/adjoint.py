@@ -16,6 +16,8 @@ memory.  Here the backward is the adjoint method in O(2^n):
 Rotations are qaravan's RX/RY/RZ/RXX/RYY/RZZ (U = exp(−iθP), no ½,
 core/param_gates.py:51-103); a gate takes its angle from θ[g.param_index].
 """
+from dataclasses import replace
+
 import numpy as np
 import torch
 
@@ -29,7 +31,7 @@ def _adjoint_cfg(n, dtype, cfg):
     registers (c64: ≤ 512 threads, c128: ≤ 256)."""
     tb_max = 9 if dtype == torch.complex64 else 8
     tb = min(cfg.tb, tb_max, n - 4)
-    return TileConfig(tb=tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost, specialize=cfg.specialize)
+    return replace(cfg, tb=tb, prefetch=False)
 
 
 class ParamProgram:
@@ -53,8 +55,7 @@ class ParamProgram:
                 raise NotImplementedError("gates on > 2 qubits and resets are not differentiable here")
             if op.kind in ('rot1', 'rot2') and not 0 <= op.param < self.n_params:
                 raise ValueError(f"param index {op.param} outside [0, {self.n_params})")
-        self.fwd = compile_ops(ops, self.n, TileConfig(tb=min(cfg.tb, self.n - 4), low_bits=cfg.low_bits,
-                                                       max_cost=cfg.max_cost, specialize=cfg.specialize),
+        self.fwd = compile_ops(ops, self.n, replace(cfg, tb=min(cfg.tb, self.n - 4), prefetch=False),
                                n_params=self.n_params)
         back = []
         for op in reversed(ops):

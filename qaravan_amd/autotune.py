@@ -11,6 +11,7 @@ import itertools
 import json
 import os
 import time
+from dataclasses import replace
 
 from .program import TileConfig
 
@@ -46,9 +47,9 @@ def best_tile_config(n, dtype):
         cfg = base
     else:
         cfg = TileConfig(tb=e["tb"], low_bits=e["low_bits"], max_cost=e["max_cost"],
-                         prefetch=e.get("prefetch", False), specialize=e.get("specialize", False))
-    return TileConfig(tb=min(cfg.tb, n - 4), low_bits=cfg.low_bits, max_cost=cfg.max_cost,
-                      prefetch=cfg.prefetch, specialize=cfg.specialize)
+                         prefetch=e.get("prefetch", False), specialize=e.get("specialize", False),
+                         fuse2q=e.get("fuse2q", True))
+    return replace(cfg, tb=min(cfg.tb, n - 4))
 
 
 def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_path=None):
@@ -66,13 +67,14 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_
     psi = torch.empty(2 ** n, dtype=dtype, device=device)
     scratch = DeviceScratch(device)
     results = []
-    for tb, lb, mc, pf, spz in itertools.product(space["tb"], space["low_bits"], space["max_cost"],
-                                                 space.get("prefetch", [False]), space.get("specialize", [False])):
+    for tb, lb, mc, pf, spz, f2 in itertools.product(space["tb"], space["low_bits"], space["max_cost"],
+                                                     space.get("prefetch", [False]), space.get("specialize", [False]),
+                                                     space.get("fuse2q", [True])):
         if tb + 4 > n or (dtype == torch.complex128 and tb > (8 if pf else 9)) or (pf and tb > 9):
             continue
         if pf and spz:
             continue
-        cfg = TileConfig(tb=tb, low_bits=lb, max_cost=mc, prefetch=pf, specialize=spz)
+        cfg = TileConfig(tb=tb, low_bits=lb, max_cost=mc, prefetch=pf, specialize=spz, fuse2q=f2)
         prog = compile_ops(ops, n, cfg)
         times = []
         for r in range(repeats + 1):
@@ -86,14 +88,15 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_
                 times.append(time.perf_counter() - t0)
         t = min(times)
         results.append((t, cfg, prog.n_tile_passes()))
-        log(f"[tune] n={n} {dtype} tb={tb} low={lb} cap={mc:g} prefetch={pf} specialize={spz}: "
+        log(f"[tune] n={n} {dtype} tb={tb} low={lb} cap={mc:g} prefetch={pf} specialize={spz} fuse2q={f2}: "
             f"{t * 1e3:.3f} ms, {prog.n_tile_passes()} passes")
     results.sort(key=lambda x: x[0])
     t, cfg, npass = results[0]
     if write:
         cache = _load()
         cache[_key(n, dtype)] = dict(tb=cfg.tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost,
-                                     prefetch=cfg.prefetch, specialize=cfg.specialize, ms=t * 1e3,
+                                     prefetch=cfg.prefetch, specialize=cfg.specialize, fuse2q=cfg.fuse2q,
+                                     ms=t * 1e3,
                                      passes=npass)
         with open(out_path or CACHE, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)
@@ -114,8 +117,7 @@ def best_adjoint_config(n, dtype):
     else:
         cfg = TileConfig(tb=e["tb"], low_bits=e["low_bits"], max_cost=e["max_cost"],
                          specialize=e.get("specialize", False))
-    return TileConfig(tb=min(cfg.tb, n - 4), low_bits=cfg.low_bits, max_cost=cfg.max_cost,
-                      specialize=cfg.specialize)
+    return replace(cfg, tb=min(cfg.tb, n - 4))
 
 
 def tune_adjoint(n, layers, dtype, device, space=None, repeats=5, log=print, out_path=None):

@@ -11,8 +11,9 @@ the library runs as a handful of fused passes:
    become permutation / sign ops; other 1- and 2-qubit matrices stay dense;
    3..6-qubit gates become stand-alone dense passes; ``Reset`` becomes its
    projector followed by a renormalisation pass (base_sim.py:25-28).
-2. 1-qubit fusion.  Runs of dense 1-qubit ops on the same bit (with no other op
-   on that bit in between) are multiplied into one 2x2.
+2. Fusion.  Runs of dense 1-qubit ops on the same bit (with no other op on
+   that bit in between) are multiplied into one 2x2; then 1-qubit ops are
+   folded into an adjacent 2-qubit op where that saves ALU work (fuse_2q).
 3. Pass partition.  Ops are packed greedily, in order, into tile passes: a pass
    owns L address bits (always the lowest ``low_bits`` ones, so every tile
    load is a run of contiguous amplitudes) and takes every op whose bits fit,
@@ -214,6 +215,65 @@ def fuse_1q(ops):
     return out
 
 
+_OP2_MAT = {'cx': _CX_TC, 'cz': _CZ}
+
+
+def fuse_2q(ops):
+    """Fold dense 1-qubit ops into a neighbouring 2-qubit op.
+
+    For each 2-qubit op (mat2 / cx / cz) in order, the candidates are the mat1
+    that is the last op on either of its bits before it and the mat1 that is
+    the next op on either bit after it (nothing else touches that bit in
+    between, so moving the 1-qubit gate to the 2-qubit op keeps the product).
+    The ops are replaced by one mat2 (post_a⊗post_b)·G·(pre_a⊗pre_b) when that
+    saves ALU work: always for a dense mat2, and for a CNOT/CZ (nearly free
+    as a permutation / sign) when it absorbs at least three 1-qubit gates, the
+    point where one 16-FMA 2-qubit product beats the 8-FMA 1-qubit products.
+    Ops are assumed 1-qubit-fused already (at most one mat1 per bit between
+    other ops)."""
+    N = len(ops)
+    allbits = set()
+    for op in ops:
+        allbits.update(op.bits)
+    prev = [dict() for _ in range(N)]
+    nxt = [dict() for _ in range(N)]
+    last = {}
+    for i, op in enumerate(ops):
+        touched = allbits if op.kind == 'norm' else op.bits
+        for b in touched:
+            p = last.get(b)
+            prev[i][b] = p
+            if p is not None:
+                nxt[p][b] = i
+            last[b] = i
+    dead = set()
+    repl = {}
+    for j, op in enumerate(ops):
+        if op.kind not in ('mat2', 'cx', 'cz'):
+            continue
+        a, b = op.bits
+        pre, post = {}, {}
+        for bit in (a, b):
+            p = prev[j].get(bit)
+            if p is not None and p not in dead and ops[p].kind == 'mat1':
+                pre[bit] = p
+            q = nxt[j].get(bit)
+            if q is not None and q not in dead and ops[q].kind == 'mat1':
+                post[bit] = q
+        k = len(pre) + len(post)
+        if 8.0 * k + OP_COST[op.kind] - OP_COST['mat2'] < 8.0:
+            continue
+        eye = np.eye(2, dtype=np.complex128)
+
+        def side(d):
+            return np.kron(ops[d[a]].mat if a in d else eye, ops[d[b]].mat if b in d else eye)
+        g = op.mat if op.kind == 'mat2' else _OP2_MAT[op.kind].astype(np.complex128)
+        repl[j] = EOp('mat2', (a, b), side(post) @ g @ side(pre))
+        dead.update(pre.values())
+        dead.update(post.values())
+    return [repl.get(i, op) for i, op in enumerate(ops) if i not in dead]
+
+
 # ----------------------------------------------------------------- scheduling
 @dataclass
 class Stage:
@@ -243,13 +303,15 @@ class TileConfig:
     prefetch: persistent workgroups that copy the next tile HBM -> LDS while
               computing the current one (forward passes only);
     specialize: compile each forward pass into its own straight-line kernel
-              (qaravan_amd.specialize) instead of the op-interpreting kernel.
+              (qaravan_amd.specialize) instead of the op-interpreting kernel;
+    fuse2q: fold 1-qubit gates into neighbouring 2-qubit gates (fuse_2q).
     """
     tb: int = 8
     low_bits: int = 6
     max_cost: float = 96.0
     prefetch: bool = False
     specialize: bool = False
+    fuse2q: bool = True
 
 
 def _is_barrier(op):
@@ -523,6 +585,8 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
     cfg = cfg or TileConfig()
     if fuse:
         ops = fuse_1q(ops)
+        if cfg.fuse2q:
+            ops = fuse_2q(ops)
     if n < min_tile_n():
         if any(op.kind in ('rot1', 'rot2') for op in ops):
             raise NotImplementedError("device-angle rotations need n >= 8")
@@ -535,7 +599,7 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
         return encode(n, passes, grad, n_params)
     tb = min(cfg.tb, n - RB)
     cfg = TileConfig(tb=tb, low_bits=max(1, cfg.low_bits), max_cost=cfg.max_cost,
-                     prefetch=cfg.prefetch and not grad, specialize=cfg.specialize)
+                     prefetch=cfg.prefetch and not grad, specialize=cfg.specialize, fuse2q=cfg.fuse2q)
     passes = partition(ops, n, cfg)
     for p in passes:
         if p.kind == PASS_TILE:

@@ -24,7 +24,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("QARAVAN_HIP_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "--genco",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I" + CSRC]
-MAX_LDS = 64 * 1024
+MAX_LDS = 160 * 1024          # LDS per workgroup on gfx950
 
 
 def _decode(blob):

@@ -115,6 +115,60 @@ def test_gate_count_and_fusion_c1():
     assert n1 <= 28 * 40          # RX·RY·RZ per qubit per layer fused into one 2x2
 
 
+def _ops_unitary(ops, n):
+    """Full 2^n x 2^n matrix of an op list, built column by column from the
+    oracle's gate application on basis states (bits -> qubits n-1-b)."""
+    cols = []
+    for k in range(2 ** n):
+        sv = np.zeros(2 ** n, dtype=np.complex128)
+        sv[k] = 1.0
+        for op in ops:
+            if op.kind in ('mat1', 'mat2'):
+                m = op.mat
+            elif op.kind == 'cx':
+                m = P._CX_TC
+            else:
+                m = P._CZ
+            sv = O.op_action(np.asarray(m, dtype=np.complex128), [n - 1 - b for b in op.bits], sv)
+        cols.append(sv)
+    return np.stack(cols, axis=1)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_fuse_2q_preserves_the_unitary(seed):
+    rng = np.random.default_rng(300 + seed)
+    n = 5
+    circ = random_circuit(n, 40, rng, kinds=("u1", "u1", "u1", "u2", "cx", "cz"))
+    ops = P.fuse_1q(P.lower_circuit(circ.build(), n))
+    fused = P.fuse_2q(ops)
+    assert len(fused) < len(ops)
+    assert np.linalg.norm(_ops_unitary(fused, n) - _ops_unitary(ops, n)) < 1e-12
+    ref = np.stack([O.run(circ.gate_list, n, init=np.eye(2 ** n, dtype=np.complex128)[k])
+                    for k in range(2 ** n)], axis=1)
+    assert np.linalg.norm(_ops_unitary(ops, n) - ref) < 1e-12
+
+
+def test_fuse_2q_halves_c1_alu_cost():
+    """Brickwork: each even-layer CNOT takes the 1-qubit gates before and after
+    it on both of its qubits (4 x 8 FMA -> one 16-FMA 2-qubit product)."""
+    circ = brickwork_circuit(28, 40)
+    ops = P.fuse_1q(P.lower_circuit(circ.build(), 28))
+    fused = P.fuse_2q(ops)
+    cost = lambda ol: sum(P.OP_COST[o.kind] for o in ol)
+    kinds = [o.kind for o in fused]
+    assert kinds.count('mat2') == 20 * 14 and kinds.count('cx') == 20 * 13
+    assert kinds.count('mat1') < 60
+    assert cost(fused) < 0.56 * cost(ops)
+
+
+def test_fuse_2q_program_matches_oracle():
+    circ = brickwork_circuit(12, 20)
+    for f2 in (False, True):
+        prog = compile_circ(circ, P.TileConfig(tb=6, low_bits=4, max_cost=48, fuse2q=f2))
+        got, _ = prog_interp.run(prog.blob, O.init_state(12))
+        assert np.linalg.norm(got - O.run(circ.gate_list, 12)) < 1e-12
+
+
 def _param_ops(circ, n):
     return P.lower_circuit(circ.build(), n, param_mode=True)
 
