@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-blocks", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-measure", action="store_true")
     ap.add_argument("--profile-forward", action="store_true",
                     help="only the timed forward leg (for rocprofv3 runs)")
     ap.add_argument("--profile-blocks", action="store_true", help="only the MFMA block leg")
@@ -241,6 +242,54 @@ def bench_blocks(dev, n=30, layers=24, reps=3):
                 hbm_gbps=nb * 2 * 2 ** n * 8 / dt / 1e9, l2_err_vs_c128=err)
 
 
+def bench_measure(dev, n=28, reps=5, shots=1_000_000):
+    """Measurement kernels on a 28-qubit complex64 state (2 GiB, resident in
+    HBM): marginal probabilities over 2 and over all 28 qubits, a 10^6-shot
+    draw over all 28, and a 2- and a 5-qubit reduced density matrix."""
+    import numpy as np
+    import torch
+
+    from qaravan_amd import _lib
+    from qaravan_amd import measure as M
+    from qaravan_amd.program import TileConfig, compile_ops, lower_circuit
+    from qaravan_amd.statevector_sim import DeviceScratch, run_program
+    from qaravan_amd.workloads import brickwork_circuit
+    lib = _lib.load()
+    psi = torch.empty(2 ** n, dtype=torch.complex64, device=dev)
+    _lib.check(lib.svh_init_basis(_lib.ptr(psi), n, 0, 0, _lib.stream_ptr(dev)), "init")
+    run_program(psi, compile_ops(lower_circuit(brickwork_circuit(n, 4).build(), n), n, TileConfig(tb=8)),
+                DeviceScratch(dev))
+    state_bytes = 2 ** n * 8
+
+    def timed(fn):
+        fn()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            fn()
+        e.record()
+        torch.cuda.synchronize(dev)
+        return s.elapsed_time(e) / reps
+
+    out = {"workload": f"{n}-qubit complex64 state (brickwork, 4 layers)"}
+    t = timed(lambda: M.marginal_probabilities(psi, [0, n - 1]))
+    out["probs_2q_ms"], out["probs_2q_gbps"] = t, state_bytes / t / 1e6
+    t = timed(lambda: M.marginal_probabilities(psi, list(range(n))))
+    out["probs_all_ms"], out["probs_all_gbps"] = t, (state_bytes + 2 ** n * 8) / t / 1e6
+    for k, sites in ((2, [3, 17]), (5, [0, 6, 13, 20, 27])):
+        t = timed(lambda: M.rdm_from_sv(psi, sites))
+        out[f"rdm_{k}q_ms"], out[f"rdm_{k}q_gbps"] = t, state_bytes / t / 1e6
+    np.random.seed(1)
+    M.measure_sv(psi, list(range(n)), shots=shots)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    M.measure_sv(psi, list(range(n)), shots=shots)
+    out["sample_all_1e6_shots_ms"] = (time.perf_counter() - t0) * 1e3
+    del psi
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_batch(dev, nc=512, seed=20260004, check=6):
     """512 HEA circuits (10..22 qubits, 4..32 layers) forward + adjoint as one
     HIP-graph launch; latency from device timestamps."""
@@ -369,6 +418,10 @@ def main():
     if not args.no_batch:
         batch = bench_batch(dev)
         log(f"[bench] batch: {batch}")
+    meas = None
+    if not args.no_measure:
+        meas = bench_measure(dev)
+        log(f"[bench] measure: {meas}")
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(fwd["n"], fwd["layers"])
@@ -407,6 +460,8 @@ def main():
         out["fused_blocks"] = blocks
     if batch is not None:
         out["batch"] = batch
+    if meas is not None:
+        out["measure"] = meas
     print(json.dumps(out))
 
 

@@ -33,6 +33,11 @@
  *                         one after another); here a whole batch of parametrised circuits
  *                         with their adjoint gradients runs as one launch sequence
  *   svh_timestamp         no counterpart (device clock for per-circuit latencies)
+ *   svh_marginal_probs    tensorQ/statevector_sim.py:185-187 measure_sv (diag of rdm_from_sv)
+ *   svh_sample_cdf        tensorQ/statevector_sim.py:188-193 np.random.choice(outcomes, p=probs)
+ *   svh_collapse_gather   tensorQ/statevector_sim.py:195-215 measure_and_collapse_sv (projection)
+ *   svh_partial_overlap   tensorQ/statevector_sim.py:155-183 partial_overlap / rdm_from_sv,
+ *                         and the environment tensors of tensorQ/environments.py:7-23
  */
 #ifndef QARAVAN_HIP_H
 #define QARAVAN_HIP_H
@@ -284,6 +289,37 @@ int svh_pauli_apply_table(const void *psi, void *out, int n, int dtype, int n_te
 /* out_dev[0..1] = ⟨a|b⟩ (re, im) accumulated in FP64. */
 int svh_inner(const void *a, const void *b, int n, int dtype, double *out_dev,
               void *scratch, size_t scratch_bytes, void *stream);
+
+/* ---------------------------------------------------------------- measurement
+ * A set of k "kept" qubits is passed as k distinct address bits, most
+ * significant first: kept[0] carries bit k-1 of the kept index m.  The other
+ * n-k address bits carry the rest index r in ascending order (big-endian over
+ * the remaining qubits).  Sums accumulate in FP64 in a fixed order. */
+enum { SVH_MEAS_PROBS = 1, SVH_MEAS_SAMPLE = 2, SVH_MEAS_OVERLAP = 3 };
+
+/* Scratch bytes that svh_marginal_probs / svh_sample_cdf / svh_partial_overlap
+ * (what = SVH_MEAS_*) need for n qubits and k kept ones (0 = invalid). */
+size_t svh_measure_scratch_bytes(int what, int n, int k);
+
+/* probs[m] = Σ_r |ψ(m, r)|² (FP64, 2^k entries). */
+int svh_marginal_probs(const void *psi, int n, int dtype, const int *kept, int k, double *probs,
+                       void *scratch, size_t scratch_bytes, void *stream);
+
+/* Inverse-CDF sampling, the algorithm of numpy's np.random.choice(a, p=probs):
+ * cdf = cumsum(probs), idx[s] = first i with cdf[i]/cdf[M-1] > uniforms[s]
+ * (searchsorted, side='right').  total[0] = cdf[M-1] (for the caller's sum
+ * check).  M = 2^k; the uniforms come from the caller. */
+int svh_sample_cdf(const double *probs, int k, const double *uniforms, int64_t shots, int64_t *idx,
+                   double *total, void *scratch, size_t scratch_bytes, void *stream);
+
+/* out[r] = ψ(outcome, r): the 2^(n-k) amplitudes whose kept bits equal outcome
+ * (same dtype as ψ; normalise with svh_norm2 + svh_scale_by_inv_norm). */
+int svh_collapse_gather(const void *psi, int n, int dtype, const int *kept, int k, uint64_t outcome, void *out,
+                        void *stream);
+
+/* out[i*2^k + j] = Σ_r a(i, r)·conj(b(j, r)) as complex128, k ≤ 6. */
+int svh_partial_overlap(const void *a, const void *b, int n, int dtype, const int *kept, int k, void *out,
+                        void *scratch, size_t scratch_bytes, void *stream);
 
 #ifdef __cplusplus
 }

@@ -21,6 +21,20 @@ What each function restates (paths relative to the qaravan source tree):
                      convention (qubit q ↔ bit n-1-q).
 * ``pauli_expectation`` statevector_sim.py:217-234.
 * ``tfi_terms``      src/qaravan/core/hamiltonians.py:19-88,261-272.
+* ``partial_overlap`` / ``rdm_from_sv``  statevector_sim.py:155-183 (free legs of
+                     the kept sites in ascending site order; rows from sv1,
+                     columns from conj(sv2)).
+* ``measure_sv``     statevector_sim.py:185-193: probabilities = diag of the rdm
+                     over the SORTED sites, outcomes = binary strings of width k,
+                     drawn by np.random.choice (legacy global RNG).
+* ``choice_indices`` the inverse-CDF steps of numpy's RandomState.choice with
+                     p given (cumsum, divide by the last entry, searchsorted
+                     side='right' of random_sample draws), made explicit so the
+                     GPU sampler can be fed the same uniforms.
+* ``measure_and_collapse_sv`` statevector_sim.py:195-215, including its order
+                     convention: outcome character j is projected onto site
+                     meas_sites[j] although the characters were drawn for the
+                     sorted sites (the two agree when meas_sites is sorted).
 
 Pinning (each is exercised in tests/test_oracle.py):
 * reference known answers: tests/test_core.py:8-41,127-137 (GHZ amplitudes,
@@ -31,7 +45,12 @@ Pinning (each is exercised in tests/test_oracle.py):
 * TFI: ground energy of TFI(4, 1, 1)-type chains vs numpy.linalg.eigh of the
   embedded matrix, ⟨+…+|H|+…+⟩ = h·n, ⟨0…0|H|0…0⟩ = jz·(n-1);
 * gradients: adjoint (this file) vs parameter shift (exact for exp(-iθP)) vs
-  central finite differences.
+  central finite differences;
+* partial_overlap vs the partial trace of the full |sv1⟩⟨sv2| (einsum over a
+  brute-force 2^n x 2^n outer product) and closed forms (Bell/GHZ rdms,
+  product states, trace = ⟨sv2|sv1⟩);
+* choice_indices vs np.random.choice itself under the same seed, and
+  measure_sv vs a direct per-outcome probability computation.
 """
 import numpy as np
 
@@ -274,3 +293,67 @@ def fd_gradient(spec, theta, n, terms, eps=1e-3, idx=None):
         tm[i] -= eps
         g[j] = (param_energy(spec, tp, n, terms) - param_energy(spec, tm, n, terms)) / (2 * eps)
     return g
+
+
+# ---------------------------------------------------------------- measurement
+def partial_overlap(sv1, sv2, skip=None):
+    """M[i, j] = Σ_rest sv1[i, rest] · conj(sv2[j, rest]); i, j run over the
+    kept (``skip``) sites in ascending order, big-endian
+    (statevector_sim.py:155-180)."""
+    n = int(round(np.log2(len(sv1))))
+    sites = sorted(skip) if skip is not None else []
+    rest = [q for q in range(n) if q not in sites]
+    a = np.asarray(sv1, dtype=np.complex128).reshape([2] * n).transpose(sites + rest)
+    b = np.asarray(sv2, dtype=np.complex128).reshape([2] * n).transpose(sites + rest)
+    k = len(sites)
+    a = a.reshape(2 ** k, -1)
+    b = b.reshape(2 ** k, -1)
+    return a @ b.conj().T
+
+
+def rdm_from_sv(sv, sites):
+    """statevector_sim.py:182-183."""
+    return partial_overlap(sv, sv, skip=sites)
+
+
+def marginal_probabilities(sv, sites):
+    """Real diagonal of rdm_from_sv(sv, sites) without forming the matrix
+    (sites sorted, big-endian), as measure_sv reads it."""
+    n = int(round(np.log2(len(sv))))
+    sites = sorted(sites)
+    rest = [q for q in range(n) if q not in sites]
+    p = np.abs(np.asarray(sv, dtype=np.complex128).reshape([2] * n)) ** 2
+    return p.transpose(sites + rest).reshape(2 ** len(sites), -1).sum(axis=1)
+
+
+def choice_indices(probs, uniforms):
+    """numpy RandomState.choice(len(p), p=probs) given its random_sample draws."""
+    cdf = np.cumsum(probs)
+    cdf /= cdf[-1]
+    return np.searchsorted(cdf, uniforms, side='right')
+
+
+def measure_sv(sv, meas_sites, shots=1, uniforms=None):
+    """statevector_sim.py:185-193.  With ``uniforms`` (one per shot) the draw
+    is explicit; otherwise they come from np.random.random_sample like
+    np.random.choice's."""
+    probs = np.real(np.diag(rdm_from_sv(sv, meas_sites)))
+    k = len(meas_sites)
+    if uniforms is None:
+        uniforms = np.random.random_sample() if shots == 1 else np.random.random_sample(shots)
+    idx = choice_indices(probs, np.atleast_1d(uniforms))
+    out = [np.binary_repr(int(i), width=k) for i in idx]
+    return out[0] if shots == 1 else out
+
+
+def measure_and_collapse_sv(sv, meas_sites, uniforms=None):
+    """statevector_sim.py:195-215."""
+    sv = np.asarray(sv, dtype=np.complex128)
+    n = int(round(np.log2(len(sv))))
+    outcome = measure_sv(sv, meas_sites, uniforms=uniforms)
+    t = sv.reshape([2] * n)
+    sl = [slice(None)] * n
+    for j, q in enumerate(meas_sites):
+        sl[q] = int(outcome[j])
+    collapsed = t[tuple(sl)].reshape(-1).copy()
+    return outcome, collapsed / np.linalg.norm(collapsed)

@@ -16,9 +16,16 @@ from .utils import endian_transform, pretty_print_sv, string_to_sv
 __version__ = "0.1.0"
 
 
+_MEASURE = ("partial_overlap", "rdm_from_sv", "measure_sv", "measure_and_collapse_sv", "pauli_expectation_sv",
+            "marginal_probabilities")
+
+
 def __getattr__(name):
     # the simulator (and with it the HIP library) loads on first use
     if name == "StatevectorSim":
         from .statevector_sim import StatevectorSim
         return StatevectorSim
+    if name in _MEASURE:
+        from . import measure
+        return getattr(measure, name)
     raise AttributeError(name)

@@ -56,7 +56,18 @@ SIGNATURES = {
     "svh_batch_init": (_c_int, [_c_void_p, ctypes.c_int64, _c_void_p, _c_int, _c_int, _c_void_p]),
     "svh_batch_trig": (_c_int, [_c_void_p, ctypes.c_int64, _c_void_p, _c_int, _c_void_p]),
     "svh_timestamp": (_c_int, [_c_void_p, _c_void_p]),
+    "svh_measure_scratch_bytes": (_c_size_t, [_c_int, _c_int, _c_int]),
+    "svh_marginal_probs": (_c_int, [_c_void_p, _c_int, _c_int, ctypes.POINTER(_c_int), _c_int, _c_void_p,
+                                    _c_void_p, _c_size_t, _c_void_p]),
+    "svh_sample_cdf": (_c_int, [_c_void_p, _c_int, _c_void_p, ctypes.c_int64, _c_void_p, _c_void_p, _c_void_p,
+                                _c_size_t, _c_void_p]),
+    "svh_collapse_gather": (_c_int, [_c_void_p, _c_int, _c_int, ctypes.POINTER(_c_int), _c_int, _c_u64,
+                                     _c_void_p, _c_void_p]),
+    "svh_partial_overlap": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, ctypes.POINTER(_c_int), _c_int,
+                                     _c_void_p, _c_void_p, _c_size_t, _c_void_p]),
 }
+
+SVH_MEAS_PROBS, SVH_MEAS_SAMPLE, SVH_MEAS_OVERLAP = 1, 2, 3
 
 _lib = None
 _lock = threading.Lock()

@@ -219,9 +219,10 @@ __global__ void __launch_bounds__(256) k_pauli_apply(const cpx<R> *__restrict__ 
 
 using namespace svh;
 
-static int check_state(const void *psi, int n, int dtype) {
+static int check_state(const void *psi, int n, int dtype, int n_min = 1) {
     if (!psi) return fail(-1, "state pointer is NULL");
-    if (n < 1 || n > kMaxQubits) return fail(-2, "n must be in [1, 36], got " + std::to_string(n));
+    if (n < n_min || n > kMaxQubits)
+        return fail(-2, "n must be in [" + std::to_string(n_min) + ", 36], got " + std::to_string(n));
     if (dtype != SVH_C64 && dtype != SVH_C128) return fail(-3, "dtype must be SVH_C64 or SVH_C128");
     return 0;
 }
@@ -308,7 +309,7 @@ int svh_apply_dense(void *psi, int n, int dtype, const int *bits, int k, const v
 
 int svh_norm2(const void *psi, int n, int dtype, double *out_dev, void *scratch,
               size_t scratch_bytes, void *stream) {
-    if (int e = check_state(psi, n, dtype)) return e;
+    if (int e = check_state(psi, n, dtype, 0)) return e;
     if (!out_dev || !scratch) return fail(-1, "out or scratch is NULL");
     if (scratch_bytes < kReduceBlocks * sizeof(double)) return fail(-5, "scratch < 8 KiB");
     const uint64_t N = 1ull << n;
@@ -324,7 +325,7 @@ int svh_norm2(const void *psi, int n, int dtype, double *out_dev, void *scratch,
 }
 
 int svh_scale_by_inv_norm(void *psi, int n, int dtype, const double *norm2_dev, void *stream) {
-    if (int e = check_state(psi, n, dtype)) return e;
+    if (int e = check_state(psi, n, dtype, 0)) return e;
     if (!norm2_dev) return fail(-1, "norm pointer is NULL");
     const uint64_t N = 1ull << n;
     hipStream_t st = (hipStream_t)stream;

@@ -227,9 +227,9 @@ class StatevectorSim:
                     bar.update(1)
             else:
                 run_program(self._flat, prog, self._scratch)
-        if circ.meas_sites is not None:
-            raise NotImplementedError("mid-circuit measurement is not on the HIP path")
         self.ran = True
+        if circ.meas_sites is not None:
+            self.measure(circ.meas_sites)      # drawn and discarded, as base_sim.py:49-50
         return self.state
 
     def get_statevector(self):
@@ -247,6 +247,17 @@ class StatevectorSim:
                                  buf.numel(), st), "svh_norm2")
         _lib.check(lib.svh_scale_by_inv_norm(_lib.ptr(self._flat), self.num_sites, code, _lib.ptr(out), st),
                    "svh_scale_by_inv_norm")
+
+    # ------------------------------------------------------------ measurement
+    def measure(self, meas_sites):
+        """A measurement bitstring over the sorted sites (statevector_sim.py:54-57)."""
+        from .measure import measure_sv
+        return measure_sv(self.get_statevector(), meas_sites)
+
+    def measure_and_collapse(self, meas_sites):
+        """(bitstring, post-measurement state of the other sites) (statevector_sim.py:59-61)."""
+        from .measure import measure_and_collapse_sv
+        return measure_and_collapse_sv(self.get_statevector(), meas_sites)
 
     # ------------------------------------------------------------ observables
     def pauli_expectation(self, pauli_string):

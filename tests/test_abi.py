@@ -53,6 +53,25 @@ def test_argument_errors_without_gpu():
     assert b"out of range" in lib.svh_last_error()
 
 
+def test_measurement_argument_errors_without_gpu():
+    lib = _lib.load()
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: rejected before any launch
+    rep = (ctypes.c_int * 2)(3, 3)
+    assert lib.svh_marginal_probs(fake, 6, 0, rep, 2, fake, fake, 1 << 20, None) == -2
+    assert b"repeated" in lib.svh_last_error()
+    far = (ctypes.c_int * 1)(6)
+    assert lib.svh_marginal_probs(fake, 6, 0, far, 1, fake, fake, 1 << 20, None) == -2
+    seven = (ctypes.c_int * 7)(*range(7))
+    assert lib.svh_partial_overlap(fake, fake, 10, 0, seven, 7, fake, fake, 1 << 30, None) == -2
+    two = (ctypes.c_int * 2)(0, 5)
+    assert lib.svh_collapse_gather(fake, 6, 0, two, 2, 4, fake, None) == -4   # outcome beyond 2 bits
+    assert lib.svh_partial_overlap(fake, fake, 10, 0, two, 2, fake, fake, 16, None) == -5  # scratch
+    assert lib.svh_sample_cdf(fake, 3, None, 5, None, fake, fake, 1 << 20, None) == -1
+    assert lib.svh_measure_scratch_bytes(3, 10, 7) == 0
+    assert lib.svh_measure_scratch_bytes(2, 10, 10) >= 8 * 1024
+    assert lib.svh_measure_scratch_bytes(9, 10, 2) == 0
+
+
 def test_program_validation_on_host():
     lib = _lib.load()
     prog = P.compile_ops(P.lower_circuit(brickwork_circuit(12, 2).build(), 12), 12, P.TileConfig(tb=6))
