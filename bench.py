@@ -242,6 +242,41 @@ def bench_blocks(dev, n=30, layers=24, reps=3):
                 hbm_gbps=nb * 2 * 2 ** n * 8 / dt / 1e9, l2_err_vs_c128=err)
 
 
+def bench_c0(dev, reps=20):
+    """The 12-qubit 20-layer brickwork (830 gates, complex128) end to end on
+    the GPU, checked against the committed golden final state."""
+    import numpy as np
+    import torch
+
+    from qaravan_amd import _lib
+    from qaravan_amd.program import TileConfig, compile_ops, lower_circuit
+    from qaravan_amd.statevector_sim import DeviceScratch, run_program
+    from qaravan_amd.workloads import brickwork_circuit
+    n, layers = 12, 20
+    circ = brickwork_circuit(n, layers).build()
+    t0 = time.perf_counter()
+    prog = compile_ops(lower_circuit(circ, n), n, TileConfig(tb=6, low_bits=4, max_cost=1e9))
+    compile_ms = (time.perf_counter() - t0) * 1e3
+    lib = _lib.load()
+    psi = torch.empty(2 ** n, dtype=torch.complex128, device=dev)
+    sc = DeviceScratch(dev)
+    times = []
+    for r in range(reps + 2):
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        _lib.check(lib.svh_init_basis(_lib.ptr(psi), n, 1, 0, _lib.stream_ptr(dev)), "init")
+        run_program(psi, prog, sc)
+        torch.cuda.synchronize(dev)
+        if r >= 2:
+            times.append(time.perf_counter() - t0)
+    ms = float(np.median(times)) * 1e3
+    gold = np.load(os.path.join(ROOT, "tests", "golden", "brickwork_12q_20l_state.npz"))["state"]
+    err = float(np.linalg.norm(psi.cpu().numpy() - gold))
+    return dict(workload="brickwork 12q x 20 layers (830 gates), complex128", median_ms=ms,
+                gates_per_s=len(circ.gate_list) / (ms * 1e-3), compile_ms=compile_ms,
+                passes=prog.n_passes, l2_err_vs_golden=err)
+
+
 def bench_measure(dev, n=28, reps=5, shots=1_000_000):
     """Measurement kernels on a 28-qubit complex64 state (2 GiB, resident in
     HBM): marginal probabilities over 2 and over all 28 qubits, a 10^6-shot
@@ -298,11 +333,8 @@ def bench_batch(dev, nc=512, seed=20260004, check=6):
 
     from qaravan_amd.adjoint import AdjointModel
     from qaravan_amd.batch import BatchedAdjoint
-    from qaravan_amd.workloads import hea_circuit, hea_ops, tfim_terms
-    rng = np.random.default_rng(seed)
-    ns = rng.integers(10, 23, size=nc)
-    ls = rng.integers(4, 33, size=nc)
-    thetas = [rng.uniform(-np.pi, np.pi, size=2 * int(n) * int(l)) for n, l in zip(ns, ls)]
+    from qaravan_amd.workloads import batch_specs, hea_circuit, hea_ops, tfim_terms
+    ns, ls, thetas, rng = batch_specs(nc, seed)
     t0 = time.perf_counter()
     b = BatchedAdjoint(None, [tfim_terms(int(n)) for n in ns], [len(t) for t in thetas], device=dev,
                        dtype=torch.complex64, keys=list(zip(ns.tolist(), ls.tolist())),
@@ -367,6 +399,57 @@ def cpu_baseline(n, layers, max_seconds=20.0):
                        f"circuit, numpy complex128 oracle, {dt:.1f} s")
 
 
+def cpu_other_configs(max_seconds=15.0):
+    """Oracle timings beside the other workloads (part of the CPU-baseline
+    leg): the 12-qubit brickwork (median of 5 runs, 1 thread and all
+    threads), one layer of 5-qubit Haar blocks at 24 qubits, and forward runs
+    of the smallest batch circuits until ~max_seconds."""
+    import numpy as np
+
+    from oracle import sv_oracle as O
+    from qaravan_amd.workloads import batch_specs, brickwork_circuit, haar_block_layers, hea_circuit
+    out = {}
+    c0 = brickwork_circuit(12, 20)
+    try:
+        from threadpoolctl import threadpool_info, threadpool_limits
+        all_threads = max([d.get("num_threads", 1) for d in threadpool_info()] + [1])
+    except Exception:  # pragma: no cover
+        threadpool_limits, all_threads = None, os.cpu_count()
+    for label, lim in (("1_thread", 1), ("all_threads", all_threads)):
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            if threadpool_limits is not None:
+                with threadpool_limits(limits=lim):
+                    O.run(c0.gate_list, 12)
+            else:
+                O.run(c0.gate_list, 12)
+            times.append(time.perf_counter() - t0)
+        ms = float(np.median(times)) * 1e3
+        out[f"c0_{label}"] = dict(median_ms=ms, gates_per_s=len(c0.gate_list) / (ms * 1e-3), threads=lim)
+    out["cpu_count"] = os.cpu_count()
+    layer = haar_block_layers(24, 1)[0]
+    sv = O.init_state(24)
+    t0 = time.perf_counter()
+    for q0, u in layer:
+        sv = O.op_action(u, list(range(q0, q0 + 5)), sv)
+    dt = time.perf_counter() - t0
+    out["haar5_24q_one_layer"] = dict(blocks=len(layer), s=dt, blocks_per_s=len(layer) / dt)
+    ns, ls, thetas, _ = batch_specs()
+    order = np.argsort(ns, kind="stable")
+    done, t0 = 0, time.perf_counter()
+    for c in order[:16]:
+        c = int(c)
+        O.run(hea_circuit(int(ns[c]), int(ls[c]), thetas[c]).gate_list, int(ns[c]))
+        done += 1
+        if time.perf_counter() - t0 > max_seconds:
+            break
+    dt = time.perf_counter() - t0
+    out["batch_forward_subset"] = dict(circuits=done, s=dt, circuits_per_s=done / dt,
+                                       sample="smallest circuits of the 512-circuit batch, forward only")
+    return out
+
+
 def traffic_bytes(fwd):
     """HBM bytes per tile-pass launch from the committed rocprofv3 counter runs
     (FETCH_SIZE x2 calibration + WRITE_SIZE, KiB per dispatch), or None."""
@@ -418,14 +501,18 @@ def main():
     if not args.no_batch:
         batch = bench_batch(dev)
         log(f"[bench] batch: {batch}")
+    c0 = bench_c0(dev)
+    log(f"[bench] c0: {c0}")
     meas = None
     if not args.no_measure:
         meas = bench_measure(dev)
         log(f"[bench] measure: {meas}")
-    cpu = None
+    cpu = cpu_more = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(fwd["n"], fwd["layers"])
         log(f"[bench] cpu baseline: {cpu}")
+        cpu_more = cpu_other_configs()
+        log(f"[bench] cpu, other workloads: {cpu_more}")
     peak = 8000.0
     out = {
         "metric": "HBM GB/s + gates/s at 28q c64; adjoint grads/s vs torch autograd",
@@ -462,6 +549,9 @@ def main():
         out["batch"] = batch
     if meas is not None:
         out["measure"] = meas
+    out["c0"] = c0
+    if cpu_more is not None:
+        out["cpu_other_workloads"] = cpu_more
     print(json.dumps(out))
 
 

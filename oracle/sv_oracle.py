@@ -31,6 +31,11 @@ What each function restates (paths relative to the qaravan source tree):
                      p given (cumsum, divide by the last entry, searchsorted
                      side='right' of random_sample draws), made explicit so the
                      GPU sampler can be fed the same uniforms.
+* ``environment_update`` / ``environment_state_prep``  tensorQ/environments.py:
+                     21-80 on a list of (matrix, indices) gates: env =
+                     partial_overlap(pre, post) over the gate's qubits, new
+                     gate V·W† from the SVD env = W·S·V†, cost 1 − |Tr(U·env)|,
+                     left sweep then right sweep.
 * ``measure_and_collapse_sv`` statevector_sim.py:195-215, including its order
                      convention: outcome character j is projected onto site
                      meas_sites[j] although the characters were drawn for the
@@ -50,7 +55,10 @@ Pinning (each is exercised in tests/test_oracle.py):
   brute-force 2^n x 2^n outer product) and closed forms (Bell/GHZ rdms,
   product states, trace = ⟨sv2|sv1⟩);
 * choice_indices vs np.random.choice itself under the same seed, and
-  measure_sv vs a direct per-outcome probability computation.
+  measure_sv vs a direct per-outcome probability computation;
+* environments: Tr(M·env) = ⟨post|M|pre⟩ for random M (the identity that
+  makes V·W† the maximiser), |Tr(U·env)| = Σ singular values for the update,
+  and a cost sequence that never increases (block-coordinate ascent).
 """
 import numpy as np
 
@@ -337,7 +345,9 @@ def measure_sv(sv, meas_sites, shots=1, uniforms=None):
     """statevector_sim.py:185-193.  With ``uniforms`` (one per shot) the draw
     is explicit; otherwise they come from np.random.random_sample like
     np.random.choice's."""
-    probs = np.real(np.diag(rdm_from_sv(sv, meas_sites)))
+    # the reference takes the diagonal of the full rdm; marginal_probabilities
+    # is that diagonal (tested) without the 2^k x 2^k matrix
+    probs = marginal_probabilities(sv, meas_sites)
     k = len(meas_sites)
     if uniforms is None:
         uniforms = np.random.random_sample() if shots == 1 else np.random.random_sample(shots)
@@ -357,3 +367,51 @@ def measure_and_collapse_sv(sv, meas_sites, uniforms=None):
         sl[q] = int(outcome[j])
     collapsed = t[tuple(sl)].reshape(-1).copy()
     return outcome, collapsed / np.linalg.norm(collapsed)
+
+
+# ---------------------------------------------------------------- environments
+def environment_update(gates, gate_idx, pre_state, post_state, target_sv, init_sv, direction='decreasing'):
+    """tensorQ/environments.py:21-45 on gates = [[matrix, indices], ...]
+    (matrices replaced in place)."""
+    indices = list(gates[gate_idx][1])
+    env = partial_overlap(pre_state, post_state, skip=indices)
+    x, s, yh = np.linalg.svd(env)
+    new_mat = yh.conj().T @ x.conj().T
+    gates[gate_idx][0] = new_mat
+    if direction == 'decreasing':
+        if gate_idx == 1:
+            pre_state = init_sv
+        else:
+            m, idx = gates[gate_idx - 1]
+            pre_state = op_action(np.asarray(m).conj().T, list(idx), pre_state)
+        post_state = op_action(new_mat.conj().T, indices, post_state)
+    else:
+        pre_state = op_action(new_mat, indices, pre_state)
+        if gate_idx == len(gates) - 2:
+            post_state = target_sv
+        else:
+            m, idx = gates[gate_idx + 1]
+            post_state = op_action(np.asarray(m), list(idx), post_state)
+    return 1 - np.abs(np.trace(new_mat @ env)), pre_state, post_state
+
+
+def environment_state_prep(target_sv, gates, n, init_sv=None, sweeps=1):
+    """tensorQ/environments.py:47-80 for a fixed number of sweeps."""
+    init = init_state(n, init_sv)
+    sv = init.copy()
+    for m, idx in gates:
+        sv = op_action(np.asarray(m), list(idx), sv)
+    cost_list = [1 - np.abs(np.vdot(target_sv, sv))]
+    m, idx = gates[-1]
+    pre_state = op_action(np.asarray(m).conj().T, list(idx), sv)
+    post_state = np.asarray(target_sv, dtype=np.complex128)
+    for _ in range(sweeps):
+        for i in reversed(range(1, len(gates))):
+            c, pre_state, post_state = environment_update(gates, i, pre_state, post_state, target_sv, init,
+                                                          'decreasing')
+            cost_list.append(c)
+        for i in range(len(gates) - 1):
+            c, pre_state, post_state = environment_update(gates, i, pre_state, post_state, target_sv, init,
+                                                          'increasing')
+            cost_list.append(c)
+    return gates, cost_list

@@ -111,6 +111,37 @@ def pauli_sum_apply(state, out, n, terms, scratch):
                "svh_pauli_apply")
 
 
+def apply_matrix_inplace(flat, n, mat, indices):
+    """ψ ← (M on ``indices``) ψ in place on a flat device state; ``indices``
+    in gate order (first = row MSB), k ≤ 6 (statevector_sim.py:124-140)."""
+    lib = _lib.load()
+    mat = np.asarray(mat.detach().cpu().numpy() if torch.is_tensor(mat) else mat, dtype=np.complex128)
+    k = len(indices)
+    if mat.shape != (2 ** k, 2 ** k):
+        raise ValueError(f"gate matrix shape {mat.shape} does not match {k} qubit(s)")
+    bits = (ctypes.c_int * k)(*[n - 1 - int(q) for q in indices])
+    m_dev = torch.from_numpy(np.ascontiguousarray(mat)).to(device=flat.device, dtype=flat.dtype)
+    _lib.check(lib.svh_apply_dense(_lib.ptr(flat), n, _lib.dtype_code(flat.dtype), bits, k, _lib.ptr(m_dev),
+                                   _lib.stream_ptr(flat.device)), "svh_apply_dense")
+
+
+def op_action(op, indices, sv, local_dim=2):
+    """New state (M on ``indices``) ψ, same shape as ``sv`` (statevector_sim.py:
+    124-140): a device copy of ``sv`` updated by one dense-gate kernel."""
+    if local_dim != 2:
+        raise NotImplementedError("the HIP statevector path supports qubits (local_dim = 2) only")
+    t = sv if torch.is_tensor(sv) else torch.as_tensor(np.asarray(sv, dtype=np.complex128))
+    if t.device.type != "cuda":
+        _lib.require_gpu()
+        t = t.to("cuda")
+    flat = t.reshape(-1).clone()
+    n = flat.numel().bit_length() - 1
+    if isinstance(indices, (int, np.integer)):
+        indices = [int(indices)]
+    apply_matrix_inplace(flat, n, op, list(indices))
+    return flat.view(t.shape)
+
+
 def inner(a, b, n, scratch):
     """⟨a|b⟩ as a complex128 0-d device tensor (FP64 accumulation)."""
     lib = _lib.load()

@@ -13,6 +13,8 @@
   RY(θ) on every qubit, RZ(θ) on every qubit, then CZ([i, i+1]) for
   i = 0..n-2.  θ index = (2l + k)·n + q (k = 0 RY, 1 RZ).  n=20/16 layers has
   640 parameters and 944 gates.
+* ``batch_specs(nc, seed)``: the 512-circuit batch: n ~ U{10..22},
+  layers ~ U{4..32}, θ ~ U[−π, π) per circuit (seed 20260004).
 * ``tfim_terms(n)``: H = −Σ Z_i Z_{i+1} − Σ X_i on an open chain, i.e.
   qaravan's ``TFI(n, jz=-1, h=-1)`` (core/hamiltonians.py:261-272).
 The gates are the package's own Gate / RX / RY / RZ / CNOT / CZ objects, so
@@ -108,3 +110,12 @@ def tfim_terms(n):
     """(coefficient, Pauli string) list of TFI(n, jz=-1, h=-1)."""
     h = TFI(n, jz=-1, h=-1)
     return h.pauli_terms()
+
+
+def batch_specs(nc=512, seed=SEED_BASE + 4):
+    """(ns, layers, thetas) of the batched-circuit workload."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(10, 23, size=nc)
+    ls = rng.integers(4, 33, size=nc)
+    thetas = [rng.uniform(-np.pi, np.pi, size=2 * int(n) * int(l)) for n, l in zip(ns, ls)]
+    return ns, ls, thetas, rng
