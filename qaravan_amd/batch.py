@@ -298,3 +298,84 @@ class BatchedAdjoint:
         t0 = int(self.t0.item())
         te = self.t_end.cpu().numpy().astype(np.int64)
         return (te - t0) / 100.0  # 100 MHz ticks -> µs
+
+
+class BatchedForward:
+    """Many fixed-matrix circuits on n qubits each, one state per circuit in a
+    flat device buffer, run as pass-synchronous rounds: round r is pass r of
+    every program, one svh_batch_tile launch per tile shape.  Used for
+    Monte-Carlo trajectories (tensorQ/monte_carlo_sim.py:14-35), where every
+    trajectory runs its own sampled circuit."""
+
+    def __init__(self, programs, n, device="cuda", dtype=torch.complex64):
+        _lib.require_gpu()
+        self.device = torch.device(device)
+        self.dtype = dtype
+        self.code = _lib.dtype_code(dtype)
+        self.n = int(n)
+        self.programs = list(programs)
+        for p in self.programs:
+            if p.n != self.n:
+                raise ValueError("every program must act on the same number of qubits")
+            if any(q.kind != PASS_TILE for q in p.passes):
+                raise ValueError("batched rounds take tile passes only")
+        blobs, where, off = [], {}, 0
+        for p in self.programs:
+            if id(p) not in where:
+                where[id(p)] = off
+                blobs.append((off, p.blob))
+                off += (len(p.blob) + 255) & ~255
+        host = np.zeros(max(off, 256), dtype=np.uint8)
+        for o, b in blobs:
+            host[o:o + len(b)] = b
+        self.progs = torch.from_numpy(host).to(self.device)
+        self.addr = [self.progs.data_ptr() + where[id(p)] for p in self.programs]
+        self._tables = None
+        self._for_psi = None
+
+    def _plan(self, psi):
+        es = 8 if self.dtype == torch.complex64 else 16
+        N = 2 ** self.n
+        R = max((len(p.passes) for p in self.programs), default=0)
+        tables, launches, off = [], [], 0
+        for r in range(R):
+            groups = {}
+            for c, p in enumerate(self.programs):
+                if r >= len(p.passes):
+                    continue
+                h = np.frombuffer(p.blob[:HDR_DT.itemsize].tobytes(), dtype=HDR_DT)[0]
+                a = self.addr[c]
+                pa = p.passes[r]
+                groups.setdefault(pa.tb, []).append((psi.data_ptr() + es * N * c,
+                                                     a + int(h['off_passes']) + r * PASS_DT.itemsize,
+                                                     a + int(h['off_stages']), a + int(h['off_ops']),
+                                                     2 ** (self.n - pa.tb - RB)))
+            for tb, items in sorted(groups.items()):
+                arr = np.zeros(len(items), dtype=ITEM_DT)
+                wg = 0
+                for i, (ps, pp, pst, pop, nwg) in enumerate(items):
+                    arr[i] = (ps, 0, pp, pst, pop, 0, 0, wg, self.n, 0)
+                    wg += nwg
+                launches.append((off, len(items), wg, tb))
+                tables.append(arr)
+                off += (arr.nbytes + 255) & ~255
+        blob = np.zeros(max(off, 256), dtype=np.uint8)
+        o = 0
+        for t in tables:
+            blob[o:o + t.nbytes] = np.frombuffer(t.tobytes(), dtype=np.uint8)
+            o += (t.nbytes + 255) & ~255
+        self._tables = torch.from_numpy(blob).to(self.device)
+        self._launches = launches
+        self._for_psi = (psi.data_ptr(), psi.numel())
+
+    def run(self, psi):
+        """Apply program c to psi[c·2^n : (c+1)·2^n] for every c, in place."""
+        if psi.dtype != self.dtype or not psi.is_contiguous() or psi.numel() != len(self.programs) * 2 ** self.n:
+            raise ValueError("psi must be a contiguous flat buffer of one state per program")
+        if self._for_psi != (psi.data_ptr(), psi.numel()):
+            self._plan(psi)
+        lib = _lib.load()
+        st = _lib.stream_ptr(self.device)
+        base = self._tables.data_ptr()
+        for off, ni, wg, tb in self._launches:
+            _lib.check(lib.svh_batch_tile(base + off, ni, wg, self.code, tb, 0, 0, st), "svh_batch_tile")
