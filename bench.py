@@ -186,7 +186,8 @@ def bench_adjoint(dev, iters=10, tape_host=False):
         dt = (time.perf_counter() - t0) / k
         res[kernel] = (dt, float(np.linalg.norm(g_t.cpu().numpy() - g128) / np.linalg.norm(g128)))
     out = dict(adjoint_grads_per_s=1.0 / t_adj, adjoint_eager_ms=t_eager * 1e3, adjoint_graph_ms=t_graph * 1e3,
-               tile=dict(tb=cfg.tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost, specialize=cfg.specialize),
+               tile=dict(tb=cfg.tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost, specialize=cfg.specialize,
+                         io_lane_bits=cfg.io_lane_bits),
                rel_grad_err_vs_c128=rel_err,
                torch_autograd_grads_per_s=1.0 / res["tensordot"][0],
                speedup=res["tensordot"][0] / t_adj,
@@ -530,7 +531,8 @@ def main():
         "config": {"workload": f"brickwork {fwd['n']}q x {fwd['layers']} layers ({fwd['gates']} gates), "
                                f"state 2^{fwd['n']} complex64 in HBM",
                    "tile": {"tb": fwd["cfg"].tb, "low_bits": fwd["cfg"].low_bits,
-                            "max_cost": fwd["cfg"].max_cost},
+                            "max_cost": fwd["cfg"].max_cost, "fuse2q": fwd["cfg"].fuse2q,
+                            "io_lane_bits": fwd["cfg"].io_lane_bits},
                    "fused_passes": fwd["n_tile"]},
         "gates_per_s": round(fwd["gates_per_s"], 1),
         "c64_vs_c128_l2_err": err,

@@ -48,7 +48,7 @@ def best_tile_config(n, dtype):
     else:
         cfg = TileConfig(tb=e["tb"], low_bits=e["low_bits"], max_cost=e["max_cost"],
                          prefetch=e.get("prefetch", False), specialize=e.get("specialize", False),
-                         fuse2q=e.get("fuse2q", True))
+                         fuse2q=e.get("fuse2q", True), io_lane_bits=e.get("io_lane_bits", 6))
     return replace(cfg, tb=min(cfg.tb, n - 4))
 
 
@@ -67,14 +67,15 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_
     psi = torch.empty(2 ** n, dtype=dtype, device=device)
     scratch = DeviceScratch(device)
     results = []
-    for tb, lb, mc, pf, spz, f2 in itertools.product(space["tb"], space["low_bits"], space["max_cost"],
-                                                     space.get("prefetch", [False]), space.get("specialize", [False]),
-                                                     space.get("fuse2q", [True])):
+    for tb, lb, mc, pf, spz, f2, io in itertools.product(space["tb"], space["low_bits"], space["max_cost"],
+                                                         space.get("prefetch", [False]),
+                                                         space.get("specialize", [False]),
+                                                         space.get("fuse2q", [True]), space.get("io_lane_bits", [6])):
         if tb + 4 > n or (dtype == torch.complex128 and tb > (8 if pf else 9)) or (pf and tb > 9):
             continue
         if pf and spz:
             continue
-        cfg = TileConfig(tb=tb, low_bits=lb, max_cost=mc, prefetch=pf, specialize=spz, fuse2q=f2)
+        cfg = TileConfig(tb=tb, low_bits=lb, max_cost=mc, prefetch=pf, specialize=spz, fuse2q=f2, io_lane_bits=io)
         prog = compile_ops(ops, n, cfg)
         times = []
         for r in range(repeats + 1):
@@ -88,7 +89,7 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_
                 times.append(time.perf_counter() - t0)
         t = min(times)
         results.append((t, cfg, prog.n_tile_passes()))
-        log(f"[tune] n={n} {dtype} tb={tb} low={lb} cap={mc:g} prefetch={pf} specialize={spz} fuse2q={f2}: "
+        log(f"[tune] n={n} {dtype} tb={tb} low={lb} cap={mc:g} prefetch={pf} specialize={spz} fuse2q={f2} io={io}: "
             f"{t * 1e3:.3f} ms, {prog.n_tile_passes()} passes")
     results.sort(key=lambda x: x[0])
     t, cfg, npass = results[0]
@@ -96,6 +97,7 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_
         cache = _load()
         cache[_key(n, dtype)] = dict(tb=cfg.tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost,
                                      prefetch=cfg.prefetch, specialize=cfg.specialize, fuse2q=cfg.fuse2q,
+                                     io_lane_bits=cfg.io_lane_bits,
                                      ms=t * 1e3,
                                      passes=npass)
         with open(out_path or CACHE, "w") as f:
@@ -116,7 +118,7 @@ def best_adjoint_config(n, dtype):
         cfg = ADJOINT_DEFAULT[name]
     else:
         cfg = TileConfig(tb=e["tb"], low_bits=e["low_bits"], max_cost=e["max_cost"],
-                         specialize=e.get("specialize", False))
+                         specialize=e.get("specialize", False), io_lane_bits=e.get("io_lane_bits", 6))
     return replace(cfg, tb=min(cfg.tb, n - 4))
 
 
@@ -127,15 +129,16 @@ def tune_adjoint(n, layers, dtype, device, space=None, repeats=5, log=print, out
 
     from .adjoint import AdjointModel
     from .workloads import hea_circuit, hea_theta, tfim_terms
-    space = space or dict(tb=[4, 5, 6, 7, 8], max_cost=[48.0, 1e9], specialize=[False, True])
+    space = space or dict(tb=[4, 5, 6, 7, 8], max_cost=[48.0, 1e9], specialize=[False, True], io_lane_bits=[6])
     theta = hea_theta(n, layers)
     circ = hea_circuit(n, layers, theta)
     th = torch.tensor(theta, device=device)
     results = []
-    for tb, mc, spz in itertools.product(space["tb"], space["max_cost"], space["specialize"]):
+    for tb, mc, spz, io in itertools.product(space["tb"], space["max_cost"], space["specialize"],
+                                             space.get("io_lane_bits", [6])):
         if tb + 4 > n:
             continue
-        cfg = TileConfig(tb=tb, low_bits=4, max_cost=mc, specialize=spz)
+        cfg = TileConfig(tb=tb, low_bits=4, max_cost=mc, specialize=spz, io_lane_bits=io)
         m = AdjointModel(circ, tfim_terms(n), len(theta), device=device, dtype=dtype, cfg=cfg)
         for _ in range(2):
             m.energy_and_grad(th)
@@ -146,13 +149,14 @@ def tune_adjoint(n, layers, dtype, device, space=None, repeats=5, log=print, out
         torch.cuda.synchronize(device)
         t = (time.perf_counter() - t0) / repeats
         results.append((t, cfg))
-        log(f"[tune-adjoint] n={n} tb={tb} cap={mc:g} specialize={spz}: {t * 1e3:.3f} ms")
+        log(f"[tune-adjoint] n={n} tb={tb} cap={mc:g} specialize={spz} io={io}: {t * 1e3:.3f} ms")
     results.sort(key=lambda x: x[0])
     t, cfg = results[0]
     cache = _load()
     cache[f"adjoint:{str(dtype).replace('torch.', '')}:{n}"] = dict(tb=cfg.tb, low_bits=cfg.low_bits,
                                                                      max_cost=cfg.max_cost,
-                                                                     specialize=cfg.specialize, ms=t * 1e3)
+                                                                     specialize=cfg.specialize,
+                                                                     io_lane_bits=cfg.io_lane_bits, ms=t * 1e3)
     with open(out_path or CACHE, "w") as f:
         json.dump(cache, f, indent=1, sort_keys=True)
     return cfg, results

@@ -304,7 +304,10 @@ class TileConfig:
               computing the current one (forward passes only);
     specialize: compile each forward pass into its own straight-line kernel
               (qaravan_amd.specialize) instead of the op-interpreting kernel;
-    fuse2q: fold 1-qubit gates into neighbouring 2-qubit gates (fuse_2q).
+    fuse2q: fold 1-qubit gates into neighbouring 2-qubit gates (fuse_2q);
+    io_lane_bits: tile-local bits that the first and last stage of a pass keep
+              off the registers, so lanes 0..2^io_lane_bits-1 load and store
+              consecutive amplitudes (4 = 128-byte runs for complex64).
     """
     tb: int = 8
     low_bits: int = 6
@@ -312,6 +315,7 @@ class TileConfig:
     prefetch: bool = False
     specialize: bool = False
     fuse2q: bool = True
+    io_lane_bits: int = 6
 
 
 def _is_barrier(op):
@@ -372,12 +376,12 @@ def partition(ops, n, cfg):
     return passes
 
 
-def _stage_ops(p):
+def _stage_ops(p, io_lane_bits=6):
     """Split a tile pass's ops into register stages (tile-local bits)."""
     L = p.tb + RB
     loc = {g: i for i, g in enumerate(p.bits)}
     pend = [(op, tuple(loc[b] for b in op.bits)) for op in p.ops]
-    lane_bits = set(range(min(6, L)))
+    lane_bits = set(range(min(io_lane_bits, L)))
     io_bits = list(range(L - RB, L))            # register bits of a coalesced layout
     stages = []
     cur = None
@@ -599,11 +603,12 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
         return encode(n, passes, grad, n_params)
     tb = min(cfg.tb, n - RB)
     cfg = TileConfig(tb=tb, low_bits=max(1, cfg.low_bits), max_cost=cfg.max_cost,
-                     prefetch=cfg.prefetch and not grad, specialize=cfg.specialize, fuse2q=cfg.fuse2q)
+                     prefetch=cfg.prefetch and not grad, specialize=cfg.specialize, fuse2q=cfg.fuse2q,
+                     io_lane_bits=cfg.io_lane_bits)
     passes = partition(ops, n, cfg)
     for p in passes:
         if p.kind == PASS_TILE:
-            _stage_ops(p)
+            _stage_ops(p, cfg.io_lane_bits)
     prog = encode(n, passes, grad, n_params)
     prog.specialize = bool(cfg.specialize)
     return prog
