@@ -78,7 +78,7 @@ class BatchedAdjoint:
                 self.pps.append(cache[key])
                 continue
             n = self.ns[c]
-            cfg = TileConfig(tb=tb_of_n(n), low_bits=4, max_cost=max_cost)
+            cfg = TileConfig(tb=tb_of_n(n), low_bits=4, max_cost=max_cost, io_lane_bits=4)
             if circuits is not None:
                 pp = ParamProgram(circuits[c], terms_list[c], n_params_list[c], cfg=cfg, dtype=dtype)
             else:

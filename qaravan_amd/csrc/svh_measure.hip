@@ -93,6 +93,37 @@ k_marg_thread(const cpx<R> *__restrict__ psi, KeptBits kb, int k, uint64_t rest_
     part[c * M + m] = acc;
 }
 
+// Up to 16 outcomes: threads stream the whole state in address order (fully
+// coalesced) and add |ψ|² into register bins selected by the kept bits; then
+// one fixed-order tree per bin.  part[blockIdx.x*M + m].
+template <typename R, int M>
+__global__ void __launch_bounds__(256)
+k_marg_reg(const cpx<R> *__restrict__ psi, uint64_t N, KeptBits kb, int k, double *__restrict__ part) {
+    double acc[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) acc[m] = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t a = (uint64_t)blockIdx.x * 256 + threadIdx.x; a < N; a += stride) {
+        const double p = abs2(psi[a]);
+        int bin = 0;
+        for (int i = 0; i < k; ++i) bin = (bin << 1) | (int)((a >> kb.bit[i]) & 1);
+#pragma unroll
+        for (int m = 0; m < M; ++m) acc[m] += bin == m ? p : 0.0;
+    }
+    __shared__ double red[256];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        red[threadIdx.x] = acc[m];
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(uint64_t)blockIdx.x * M + m] = red[0];
+        __syncthreads();
+    }
+}
+
 // out[m] = Σ_c part[c*M + m], c ascending.
 __global__ void __launch_bounds__(256) k_marg_final(const double *__restrict__ part, uint64_t M, uint64_t C,
                                                     double *__restrict__ out) {
@@ -213,58 +244,135 @@ k_gather_rest(const cpx<R> *__restrict__ psi, uint64_t base, uint64_t rest_mask,
 }
 
 // ---------------------------------------------------------------- partial overlap
-// Stage 1: block g covers rest chunk g.  With P = M² ≤ 256 pairs, T = 256/P
-// threads share a pair and stride over r; otherwise each thread owns P/256
-// pairs.  part[g*P + p] holds (re, im) of the block's sum.
-template <typename R, int Q>
+// Few kept qubits (M = 2^k ≤ 4): thread r walks rest indices in order (lanes on
+// consecutive r, so each of the 2M loads per r is coalesced), keeping all M²
+// products in FP64 registers; then one fixed-order tree per pair.
+template <typename R, int M>
 __global__ void __launch_bounds__(256)
-k_overlap(const cpx<R> *__restrict__ a, const cpx<R> *__restrict__ b, KeptBits kb, int k,
-          uint64_t rest_mask, uint64_t chunk, double2 *__restrict__ part) {
-    const int M = 1 << k, P = M * M;
-    const int T = P >= 256 ? 1 : 256 / P;
-    const int sub = P >= 256 ? 0 : (int)threadIdx.x / P;
-    const uint64_t r_begin = (uint64_t)blockIdx.x * chunk;
-    double2 acc[Q];
-    uint64_t bi[Q], bj[Q];
+k_overlap_reg(const cpx<R> *__restrict__ a, const cpx<R> *__restrict__ b, KeptBits kb, int k,
+              uint64_t rest_mask, uint64_t NR, double2 *__restrict__ part) {
+    constexpr int P = M * M;
+    double2 acc[P];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int p = P >= 256 ? (int)threadIdx.x + q * 256 : (int)threadIdx.x % P;
-        acc[q] = make_double2(0.0, 0.0);
-        bi[q] = deposit_kept((uint64_t)(p / M), kb, k);
-        bj[q] = deposit_kept((uint64_t)(p % M), kb, k);
-    }
-    if (sub < (int)chunk) {
-        uint64_t x = deposit_mask(r_begin + sub, rest_mask);
-        const uint64_t d = deposit_mask((uint64_t)T, rest_mask);
-        for (uint64_t r = sub; r < chunk; r += T) {
+    for (int p = 0; p < P; ++p) acc[p] = make_double2(0.0, 0.0);
+    uint64_t off[M];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const cpx<R> u = a[bi[q] | x], w = b[bj[q] | x];
-                // u · conj(w)
-                acc[q].x += (double)u.x * w.x + (double)u.y * w.y;
-                acc[q].y += (double)u.y * w.x - (double)u.x * w.y;
+    for (int i = 0; i < M; ++i) off[i] = deposit_kept((uint64_t)i, kb, k);
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r < NR) {
+        uint64_t x = deposit_mask(r, rest_mask);
+        const uint64_t d = deposit_mask(stride, rest_mask);
+        for (; r < NR; r += stride) {
+            cpx<R> u[M], w[M];
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                u[i] = a[off[i] | x];
+                w[i] = b[off[i] | x];
             }
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    acc[i * M + j].x += (double)u[i].x * w[j].x + (double)u[i].y * w[j].y;
+                    acc[i * M + j].y += (double)u[i].y * w[j].x - (double)u[i].x * w[j].y;
+                }
             x = masked_add(x, d, rest_mask);
         }
     }
+    __shared__ double2 red[256];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        red[threadIdx.x] = acc[p];
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+                red[threadIdx.x].x += red[threadIdx.x + s].x;
+                red[threadIdx.x].y += red[threadIdx.x + s].y;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(uint64_t)blockIdx.x * P + p] = red[0];
+        __syncthreads();
+    }
+}
+
+// More kept qubits (M = 2^k, 8 ≤ M ≤ 64): block g walks its rest chunk in
+// batches of RBT = 1024/M indices; the batch's a(:, r) and b(:, r) are loaded
+// with lanes on consecutive r (coalesced) into LDS as FP64, then each thread
+// accumulates its Q = max(1, M²/256) pairs over the batch (T = 256/M² threads
+// share a pair when M² < 256).  part[g*P + p].
+template <typename R, int M>
+__global__ void __launch_bounds__(256)
+k_overlap_lds(const cpx<R> *__restrict__ a, const cpx<R> *__restrict__ b, KeptBits kb, int k,
+              uint64_t rest_mask, uint64_t chunk, double2 *__restrict__ part) {
+    constexpr int P = M * M;
+    constexpr int RBT = 1024 / M;              // rest indices per batch
+    constexpr int LPT = M * RBT / 256;         // loads per thread per vector (= 4)
+    constexpr int Q = P >= 256 ? P / 256 : 1;
+    constexpr int T = P >= 256 ? 1 : 256 / P;
+    __shared__ double2 sa[M * RBT], sb[M * RBT];
+    const int t = threadIdx.x;
+    const int rr = t % RBT;                    // this thread's rest slot in a batch
+    uint64_t off[LPT];
+#pragma unroll
+    for (int l = 0; l < LPT; ++l) off[l] = deposit_kept((uint64_t)(t / RBT + l * (256 / RBT)), kb, k);
+    const uint64_t drr = deposit_mask((uint64_t)rr, rest_mask);
+    const uint64_t dbatch = deposit_mask((uint64_t)RBT, rest_mask);
+    const int sub = P >= 256 ? 0 : t / P;
+    int pi[Q], pj[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int p = P >= 256 ? t + q * 256 : t % P;
+        pi[q] = p / M;
+        pj[q] = p % M;
+    }
+    double2 acc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc[q] = make_double2(0.0, 0.0);
+    const uint64_t r0 = (uint64_t)blockIdx.x * chunk;
+    uint64_t xb = deposit_mask(r0, rest_mask);
+    const int nb = (int)((chunk + RBT - 1) / RBT);
+    for (int bt = 0; bt < nb; ++bt) {
+        const bool live = (uint64_t)bt * RBT + rr < chunk;
+        const uint64_t x = masked_add(xb, drr, rest_mask);
+#pragma unroll
+        for (int l = 0; l < LPT; ++l) {
+            const int i = t / RBT + l * (256 / RBT);
+            cpx<R> u = live ? a[off[l] | x] : cpx<R>{(R)0, (R)0};
+            cpx<R> w = live ? b[off[l] | x] : cpx<R>{(R)0, (R)0};
+            sa[i * RBT + rr] = make_double2((double)u.x, (double)u.y);
+            sb[i * RBT + rr] = make_double2((double)w.x, (double)w.y);
+        }
+        __syncthreads();
+        for (int c = sub; c < RBT; c += T) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const double2 u = sa[pi[q] * RBT + c], w = sb[pj[q] * RBT + c];
+                acc[q].x += u.x * w.x + u.y * w.y;
+                acc[q].y += u.y * w.x - u.x * w.y;
+            }
+        }
+        __syncthreads();
+        xb = masked_add(xb, dbatch, rest_mask);
+    }
     if (P >= 256) {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) part[(uint64_t)blockIdx.x * P + threadIdx.x + q * 256] = acc[q];
+        for (int q = 0; q < Q; ++q) part[(uint64_t)blockIdx.x * P + t + q * 256] = acc[q];
         return;
     }
-    // fixed-order tree over the T threads of each pair
     __shared__ double2 red[256];
-    red[threadIdx.x] = acc[0];
+    red[t] = acc[0];
     __syncthreads();
-    for (int s = T / 2; s > 0; s >>= 1) {
-        if (sub < s) {
-            const double2 o = red[threadIdx.x + s * P];
-            red[threadIdx.x].x += o.x;
-            red[threadIdx.x].y += o.y;
+    for (int s2 = T / 2; s2 > 0; s2 >>= 1) {
+        if (sub < s2) {
+            const double2 o = red[t + s2 * P];
+            red[t].x += o.x;
+            red[t].y += o.y;
         }
         __syncthreads();
     }
-    if (sub == 0) part[(uint64_t)blockIdx.x * P + threadIdx.x] = red[threadIdx.x];
+    if (sub == 0) part[(uint64_t)blockIdx.x * P + t] = red[t];
 }
 
 __global__ void __launch_bounds__(256) k_overlap_final(const double2 *__restrict__ part, int P, int G,
@@ -297,22 +405,30 @@ static int check_kept(int n, const int *kept, int k, KeptBits &kb, uint64_t &kma
 }
 
 struct MargPlan {
-    bool block;
+    int kind;            // 0: register bins (k <= 4), 1: block per (m, chunk), 2: thread per (m, chunk)
     uint64_t M, C, chunk;
 };
 
 static MargPlan marg_plan(int n, int k) {
     MargPlan p;
     p.M = 1ull << k;
+    const uint64_t N = 1ull << n;
     const uint64_t NR = 1ull << (n - k);
+    if (k <= 4) {
+        p.kind = 0;
+        uint64_t g = (N + 255) / 256;
+        p.C = g < 1024 ? g : 1024;
+        p.chunk = 0;
+        return p;
+    }
     if (p.M <= 4096) {
-        p.block = true;
+        p.kind = 1;
         uint64_t C = 4096 / p.M;
         const uint64_t cap = NR >= 1024 ? NR / 1024 : 1;
         if (C > cap) C = cap;
         p.C = C;
     } else {
-        p.block = false;
+        p.kind = 2;
         uint64_t C = p.M >= (1ull << 18) ? 1 : (1ull << 18) / p.M;
         if (C > NR) C = NR;
         p.C = C;
@@ -323,15 +439,33 @@ static MargPlan marg_plan(int n, int k) {
 
 static size_t scan_blocks(uint64_t M) { return (size_t)((M + kScanTile - 1) / kScanTile); }
 
-static int overlap_blocks(int n, int k) {
+struct OverlapPlan {
+    bool reg;            // k <= 2: k_overlap_reg (grid-stride); else k_overlap_lds (chunks)
+    int G;
+    uint64_t chunk;
+};
+
+static OverlapPlan overlap_plan(int n, int k) {
+    OverlapPlan o;
     const uint64_t P = 1ull << (2 * k);
     const uint64_t NR = 1ull << (n - k);
+    if (k <= 2) {
+        o.reg = true;
+        const uint64_t g = (NR + 255) / 256;
+        o.G = (int)(g < 1024 ? g : 1024);
+        o.chunk = 0;
+        return o;
+    }
+    o.reg = false;
+    const uint64_t rbt = 1024ull >> k;
     uint64_t G = (1ull << 18) / P;
     if (G > 1024) G = 1024;
-    const uint64_t cap = NR >= 64 ? NR / 64 : 1;
+    const uint64_t cap = NR >= rbt ? NR / rbt : 1;
     if (G > cap) G = cap;
     if (G < 1) G = 1;
-    return (int)G;
+    o.G = (int)G;
+    o.chunk = NR / G;
+    return o;
 }
 
 }  // namespace svh
@@ -345,7 +479,7 @@ size_t svh_measure_scratch_bytes(int what, int n, int k) {
     switch (what) {
     case SVH_MEAS_PROBS: {
         const MargPlan p = marg_plan(n, k);
-        return p.C > 1 ? (size_t)(p.C * p.M * sizeof(double)) : sizeof(double);
+        return p.kind == 0 || p.C > 1 ? (size_t)(p.C * p.M * sizeof(double)) : sizeof(double);
     }
     case SVH_MEAS_SAMPLE: {
         const uint64_t M = 1ull << k;
@@ -353,7 +487,7 @@ size_t svh_measure_scratch_bytes(int what, int n, int k) {
     }
     case SVH_MEAS_OVERLAP:
         if (k > 6) return 0;
-        return (size_t)overlap_blocks(n, k) * ((size_t)1 << (2 * k)) * sizeof(double2);
+        return (size_t)overlap_plan(n, k).G * ((size_t)1 << (2 * k)) * sizeof(double2);
     default:
         return 0;
     }
@@ -368,12 +502,32 @@ int svh_marginal_probs(const void *psi, int n, int dtype, const int *kept, int k
     if (int e = check_kept(n, kept, k, kb, kmask)) return e;
     const MargPlan p = marg_plan(n, k);
     const size_t need = svh_measure_scratch_bytes(SVH_MEAS_PROBS, n, k);
-    if (p.C > 1 && (!scratch || scratch_bytes < need))
+    const bool staged = p.kind == 0 || p.C > 1;
+    if (staged && (!scratch || scratch_bytes < need))
         return fail(-5, "scratch too small for svh_marginal_probs (need " + std::to_string(need) + " bytes)");
     const uint64_t rest_mask = ((1ull << n) - 1) & ~kmask;
     hipStream_t st = (hipStream_t)stream;
-    double *part = p.C > 1 ? (double *)scratch : probs;
-    if (p.block) {
+    double *part = staged ? (double *)scratch : probs;
+    if (p.kind == 0) {
+        const unsigned g = (unsigned)p.C;
+        const uint64_t N = 1ull << n;
+#define SVH_MREG(RT, MV) \
+    k_marg_reg<RT, MV><<<g, 256, 0, st>>>((const cpx<RT> *)psi, N, kb, k, part)
+        if (dtype == SVH_C64) {
+            if (p.M == 1) SVH_MREG(float, 1);
+            else if (p.M == 2) SVH_MREG(float, 2);
+            else if (p.M == 4) SVH_MREG(float, 4);
+            else if (p.M == 8) SVH_MREG(float, 8);
+            else SVH_MREG(float, 16);
+        } else {
+            if (p.M == 1) SVH_MREG(double, 1);
+            else if (p.M == 2) SVH_MREG(double, 2);
+            else if (p.M == 4) SVH_MREG(double, 4);
+            else if (p.M == 8) SVH_MREG(double, 8);
+            else SVH_MREG(double, 16);
+        }
+#undef SVH_MREG
+    } else if (p.kind == 1) {
         const unsigned g = (unsigned)(p.M * p.C);
         if (dtype == SVH_C64)
             k_marg_block<float><<<g, 256, 0, st>>>((const cpx<float> *)psi, kb, k, rest_mask, p.M, p.chunk, part);
@@ -388,7 +542,7 @@ int svh_marginal_probs(const void *psi, int n, int dtype, const int *kept, int k
             k_marg_thread<double><<<g, 256, 0, st>>>((const cpx<double> *)psi, kb, k, rest_mask, p.M, p.C, p.chunk,
                                                      part);
     }
-    if (p.C > 1) k_marg_final<<<(unsigned)((p.M + 255) / 256), 256, 0, st>>>(part, p.M, p.C, probs);
+    if (staged) k_marg_final<<<(unsigned)((p.M + 255) / 256), 256, 0, st>>>(part, p.M, p.C, probs);
     return checked_launch("svh_marginal_probs");
 }
 
@@ -449,30 +603,46 @@ int svh_partial_overlap(const void *a, const void *b, int n, int dtype, const in
     KeptBits kb{};
     uint64_t kmask;
     if (int e = check_kept(n, kept, k, kb, kmask)) return e;
-    const int G = overlap_blocks(n, k);
+    const OverlapPlan o = overlap_plan(n, k);
     const int P = 1 << (2 * k);
-    const size_t need = (size_t)G * P * sizeof(double2);
+    const size_t need = (size_t)o.G * P * sizeof(double2);
     if (!scratch || scratch_bytes < need)
         return fail(-5, "scratch too small for svh_partial_overlap (need " + std::to_string(need) + " bytes)");
     const uint64_t rest_mask = ((1ull << n) - 1) & ~kmask;
-    const uint64_t chunk = (1ull << (n - k)) / (uint64_t)G;
+    const uint64_t NR = 1ull << (n - k);
     hipStream_t st = (hipStream_t)stream;
     double2 *part = (double2 *)scratch;
-#define SVH_OVL(RT, QV)                                                                                  \
-    k_overlap<RT, QV><<<(unsigned)G, 256, 0, st>>>((const cpx<RT> *)a, (const cpx<RT> *)b, kb, k, rest_mask, \
-                                                   chunk, part)
-    const int Q = P >= 256 ? P / 256 : 1;
+    const unsigned G = (unsigned)o.G;
+#define SVH_OREG(RT, MV)                                                                                 \
+    k_overlap_reg<RT, MV><<<G, 256, 0, st>>>((const cpx<RT> *)a, (const cpx<RT> *)b, kb, k, rest_mask, NR, \
+                                             part)
+#define SVH_OLDS(RT, MV)                                                                                 \
+    k_overlap_lds<RT, MV><<<G, 256, 0, st>>>((const cpx<RT> *)a, (const cpx<RT> *)b, kb, k, rest_mask,    \
+                                             o.chunk, part)
     if (dtype == SVH_C64) {
-        if (Q == 1) SVH_OVL(float, 1);
-        else if (Q == 4) SVH_OVL(float, 4);
-        else SVH_OVL(float, 16);
+        switch (k) {
+            case 0: SVH_OREG(float, 1); break;
+            case 1: SVH_OREG(float, 2); break;
+            case 2: SVH_OREG(float, 4); break;
+            case 3: SVH_OLDS(float, 8); break;
+            case 4: SVH_OLDS(float, 16); break;
+            case 5: SVH_OLDS(float, 32); break;
+            default: SVH_OLDS(float, 64); break;
+        }
     } else {
-        if (Q == 1) SVH_OVL(double, 1);
-        else if (Q == 4) SVH_OVL(double, 4);
-        else SVH_OVL(double, 16);
+        switch (k) {
+            case 0: SVH_OREG(double, 1); break;
+            case 1: SVH_OREG(double, 2); break;
+            case 2: SVH_OREG(double, 4); break;
+            case 3: SVH_OLDS(double, 8); break;
+            case 4: SVH_OLDS(double, 16); break;
+            case 5: SVH_OLDS(double, 32); break;
+            default: SVH_OLDS(double, 64); break;
+        }
     }
-#undef SVH_OVL
-    k_overlap_final<<<(unsigned)((P + 255) / 256), 256, 0, st>>>(part, P, G, (double2 *)out);
+#undef SVH_OREG
+#undef SVH_OLDS
+    k_overlap_final<<<(unsigned)((P + 255) / 256), 256, 0, st>>>(part, P, o.G, (double2 *)out);
     return checked_launch("svh_partial_overlap");
 }
 

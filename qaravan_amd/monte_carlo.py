@@ -72,7 +72,7 @@ def run_circuits(circuits, states, n, tile=None):
     """Run circuit c on states[c] (a (C, 2^n) device tensor) in place: one
     batched launch sequence when every program is made of tile passes, else
     one program run per circuit."""
-    tile = tile or TileConfig(tb=min(8, n - 4), low_bits=4, max_cost=1e9)
+    tile = tile or TileConfig(tb=min(8, n - 4), low_bits=4, max_cost=1e9, io_lane_bits=4)
     progs = [compile_ops(lower_circuit(c.build() if not c.built else c, n), n, tile) for c in circuits]
     flat = states.reshape(-1)
     if n >= 8 and all(p.kind == PASS_TILE for q in progs for p in q.passes):
@@ -147,7 +147,8 @@ def trotter_sv_sim(ham, step_size, max_steps, op, device="cuda", dtype=torch.com
     circ = ham.trotter_circ(step_size, 1)
     n = ham.num_sites
     dev = torch.device(device)
-    prog = compile_ops(lower_circuit(circ.build(), n), n, TileConfig(tb=min(8, n - 4), low_bits=4, max_cost=1e9))
+    prog = compile_ops(lower_circuit(circ.build(), n), n,
+                       TileConfig(tb=min(8, n - 4), low_bits=4, max_cost=1e9, io_lane_bits=4))
     sc = DeviceScratch(dev)
     state = torch.zeros(2 ** n, dtype=dtype, device=dev)
     state[0] = 1.0

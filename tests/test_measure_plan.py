@@ -33,31 +33,37 @@ def deposit_kept(m, kept):
 
 
 def marg_plan(n, k):
+    """kind 0: register bins over the whole state (k <= 4); 1: block per
+    (m, chunk); 2: thread per (m, chunk)."""
     M = 1 << k
+    N = 1 << n
     NR = 1 << (n - k)
+    if k <= 4:
+        return 0, M, min((N + 255) // 256, 1024), 0
     if M <= 4096:
         C = 4096 // M
         cap = NR // 1024 if NR >= 1024 else 1
-        C = min(C, cap)
-        block = True
-    else:
-        C = 1 if M >= (1 << 18) else (1 << 18) // M
-        C = min(C, NR)
-        block = False
-    return block, M, C, NR // C
+        return 1, M, min(C, cap), NR // min(C, cap)
+    C = 1 if M >= (1 << 18) else (1 << 18) // M
+    C = min(C, NR)
+    return 2, M, C, NR // C
 
 
 def marg_scratch(n, k):
-    block, M, C, chunk = marg_plan(n, k)
-    return C * M * 8 if C > 1 else 8
+    kind, M, C, chunk = marg_plan(n, k)
+    return C * M * 8 if kind == 0 or C > 1 else 8
 
 
-def overlap_blocks(n, k):
+def overlap_plan(n, k):
     P = 1 << (2 * k)
     NR = 1 << (n - k)
+    if k <= 2:
+        return True, min((NR + 255) // 256, 1024), 0
+    rbt = 1024 >> k
     G = min((1 << 18) // P, 1024)
-    cap = NR // 64 if NR >= 64 else 1
-    return max(1, min(G, cap))
+    cap = NR // rbt if NR >= rbt else 1
+    G = max(1, min(G, cap))
+    return False, G, NR // G
 
 
 CASES = [(6, [5, 0]), (7, [0]), (9, [3, 7, 1]), (8, list(range(7, -1, -1))), (10, [2, 9, 4, 6, 0, 1, 8]),
@@ -69,11 +75,25 @@ def test_marginal_plan_covers_each_amplitude_once(n, kept):
     k = len(kept)
     kmask = sum(1 << b for b in kept)
     rest = ((1 << n) - 1) & ~kmask
-    block, M, C, chunk = marg_plan(n, k)
-    assert C * chunk == 1 << (n - k)
+    kind, M, C, chunk = marg_plan(n, k)
     seen = {}
     slots = set()
-    if block:
+    if kind == 0:
+        stride = C * 256
+        for blk in range(C):
+            for t in range(256):
+                a = blk * 256 + t
+                while a < 1 << n:
+                    b = 0
+                    for bit in kept:
+                        b = (b << 1) | ((a >> bit) & 1)
+                    assert b < M
+                    seen[a] = seen.get(a, 0) + 1
+                    a += stride
+            for m in range(M):
+                slots.add(blk * M + m)
+    elif kind == 1:
+        assert C * chunk == 1 << (n - k)
         for blk in range(M * C):
             m, c = blk % M, blk // M
             base = deposit_kept(m, kept)
@@ -92,6 +112,7 @@ def test_marginal_plan_covers_each_amplitude_once(n, kept):
                     x = masked_add(x, d, rest)
                     r += 256
     else:
+        assert C * chunk == 1 << (n - k)
         for g in range(M * C):
             m, c = g % M, g // M
             base = deposit_kept(m, kept)
@@ -102,54 +123,86 @@ def test_marginal_plan_covers_each_amplitude_once(n, kept):
                 seen[a] = seen.get(a, 0) + 1
                 x = masked_add(x, 1, rest)
     assert sorted(seen) == list(range(1 << n)) and set(seen.values()) == {1}
-    if C > 1:
+    if kind == 0 or C > 1:
         assert max(slots) * 8 < marg_scratch(n, k)
 
 
 def test_marginal_plan_shapes_at_large_sizes():
     for n in (20, 28, 30, 34):
         for k in range(0, n + 1):
-            block, M, C, chunk = marg_plan(n, k)
-            assert C >= 1 and chunk >= 1 and C * chunk == 1 << (n - k)
-            if block:
-                assert M * C <= 4096 * 1 or C == 1
-            else:
+            kind, M, C, chunk = marg_plan(n, k)
+            assert C >= 1
+            if kind:
+                assert chunk >= 1 and C * chunk == 1 << (n - k)
+            if kind == 2:
                 assert (M * C + 255) // 256 < 2 ** 31
             assert marg_scratch(n, k) <= (1 << 21)
 
 
-@pytest.mark.parametrize("n,kept", [(6, [0]), (6, [5, 2]), (8, [1, 6, 3]), (9, [8, 7, 0, 4]),
-                                    (10, [1, 2, 3, 5, 9]), (11, [0, 2, 4, 6, 8, 10])])
+@pytest.mark.parametrize("n,kept", [(6, [0]), (6, [5, 2]), (8, [1]), (9, [8, 0]), (8, [1, 6, 3]),
+                                    (9, [8, 7, 0, 4]), (10, [1, 2, 3, 5, 9]), (11, [0, 2, 4, 6, 8, 10]),
+                                    (7, [6, 5, 4, 3, 2, 1])])
 def test_overlap_plan_covers_each_rest_once_per_pair(n, kept):
+    """Mirror of k_overlap_reg (k <= 2) and k_overlap_lds (k >= 3): every
+    (pair, rest index) is accumulated exactly once and every address and LDS
+    slot is in range."""
     k = len(kept)
     kmask = sum(1 << b for b in kept)
     rest = ((1 << n) - 1) & ~kmask
     M, P = 1 << k, 1 << (2 * k)
-    G = overlap_blocks(n, k)
-    chunk = (1 << (n - k)) // G
-    assert G * chunk == 1 << (n - k)
-    T = 1 if P >= 256 else 256 // P
-    Q = P // 256 if P >= 256 else 1
+    NR = 1 << (n - k)
+    reg, G, chunk = overlap_plan(n, k)
     visits = {}
-    for g, t in itertools.product(range(G), range(256)):
-        sub = 0 if P >= 256 else t // P
-        pairs = [t + q * 256 for q in range(Q)] if P >= 256 else [t % P]
-        if sub >= chunk:
-            continue
-        x = deposit_mask(g * chunk + sub, rest)
-        d = deposit_mask(T, rest)
-        r = sub
-        while r < chunk:
-            for p in pairs:
-                assert p < P
-                ai = deposit_kept(p // M, kept) | x
-                bj = deposit_kept(p % M, kept) | x
-                assert ai < (1 << n) and bj < (1 << n)
-                visits[(p, g * chunk + r)] = visits.get((p, g * chunk + r), 0) + 1
-            x = masked_add(x, d, rest)
-            r += T
-    assert len(visits) == P * (1 << (n - k)) and set(visits.values()) == {1}
-    assert (G - 1) * P + P - 1 < G * P          # partial slots within the scratch size
+    if reg:
+        stride = G * 256
+        for g, t in itertools.product(range(G), range(256)):
+            r = g * 256 + t
+            if r >= NR:
+                continue
+            x = deposit_mask(r, rest)
+            d = deposit_mask(stride, rest)
+            while r < NR:
+                for i in range(M):
+                    assert (deposit_kept(i, kept) | x) < 1 << n
+                for p in range(P):
+                    visits[(p, r)] = visits.get((p, r), 0) + 1
+                x = masked_add(x, d, rest)
+                r += stride
+    else:
+        assert G * chunk == NR
+        RBT = 1024 // M
+        LPT = M * RBT // 256
+        T = 1 if P >= 256 else 256 // P
+        Q = P // 256 if P >= 256 else 1
+        nb = (chunk + RBT - 1) // RBT
+        for g in range(G):
+            xb = deposit_mask(g * chunk, rest)
+            for bt in range(nb):
+                rows = set()
+                for t in range(256):
+                    rr = t % RBT
+                    live = bt * RBT + rr < chunk
+                    x = masked_add(xb, deposit_mask(rr, rest), rest)
+                    assert x == deposit_mask(g * chunk + bt * RBT + rr, rest) or not live
+                    for l in range(LPT):
+                        i = t // RBT + l * (256 // RBT)
+                        assert i < M and i * RBT + rr < M * RBT
+                        rows.add((i, rr))
+                        if live:
+                            assert (deposit_kept(i, kept) | x) < 1 << n
+                assert len(rows) == M * RBT             # the batch fills the whole LDS slab
+                for t in range(256):
+                    sub = 0 if P >= 256 else t // P
+                    pairs = [t + q * 256 for q in range(Q)] if P >= 256 else [t % P]
+                    for c in range(sub, RBT, T):
+                        r = bt * RBT + c
+                        if r >= chunk:
+                            continue
+                        for p in pairs:
+                            assert p < P
+                            visits[(p, g * chunk + r)] = visits.get((p, g * chunk + r), 0) + 1
+                xb = masked_add(xb, deposit_mask(RBT, rest), rest)
+    assert len(visits) == P * NR and set(visits.values()) == {1}
 
 
 def test_scan_plan():
