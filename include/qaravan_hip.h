@@ -62,7 +62,7 @@ enum { SVH_C64 = 0, SVH_C128 = 1 };
  */
 #define SVH_PROG_MAGIC 0x53564850u /* 'SVHP' */
 #define SVH_MAX_TILE_BITS 16
-#define SVH_MAX_REG_BITS 4
+#define SVH_MAX_REG_BITS 5
 
 enum {
     SVH_PASS_TILE = 1,      /* fused pass over tiles of 2^L amplitudes            */
@@ -122,19 +122,20 @@ enum {
     SVH_PASSF_PREFETCH = 1
 };
 
-/* A stage's layout: thread t holds registers j = 0..15 at tile-local index
+/* A stage's layout: thread t holds registers j = 0..2^rb-1 at tile-local index
  * insert-zeros(t at rsorted) | loc(j), i.e. at global amplitude
  * base | Σ_i bit_i(t)·2^tgbit[i] | goff[j].  The tables are precomputed by the
- * host so the kernel does no bit arithmetic per amplitude. */
+ * host so the kernel does no bit arithmetic per amplitude.  The op-interpreting
+ * kernels run rb = 4; the per-pass compiled kernels (svh_sp_*) run rb = 4 or 5. */
 typedef struct {
-    int64_t goff[16];    /* global amplitude offset of register j                 */
-    int32_t lsw[16];     /* swizzled tile-local (LDS) offset of register j         */
+    int64_t goff[32];    /* global amplitude offset of register j                 */
+    int32_t lsw[32];     /* swizzled tile-local (LDS) offset of register j         */
     int32_t rbits[8];    /* tile-local bit held in register-index bit k (k < rb)   */
-    int32_t rsorted[4];  /* rbits[0..rb) in ascending order                         */
+    int32_t rsorted[8];  /* rbits[0..rb) in ascending order                         */
     int32_t tgbit[12];   /* global bit of thread-index bit i (i < tb)              */
     int32_t op_begin, op_end;
     int32_t pad[2];
-} svh_stage_t; /* 304 bytes */
+} svh_stage_t; /* 512 bytes */
 
 typedef struct {
     int32_t kind;        /* SVH_OP_*                                                */

@@ -317,8 +317,8 @@ class BatchedForward:
         for p in self.programs:
             if p.n != self.n:
                 raise ValueError("every program must act on the same number of qubits")
-            if any(q.kind != PASS_TILE for q in p.passes):
-                raise ValueError("batched rounds take tile passes only")
+            if any(q.kind != PASS_TILE or q.rb != RB for q in p.passes):
+                raise ValueError("batched rounds take tile passes with 4 register bits only")
         blobs, where, off = [], {}, 0
         for p in self.programs:
             if id(p) not in where:

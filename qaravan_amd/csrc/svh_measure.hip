@@ -299,19 +299,24 @@ k_overlap_reg(const cpx<R> *__restrict__ a, const cpx<R> *__restrict__ b, KeptBi
 
 // More kept qubits (M = 2^k, 8 ≤ M ≤ 64): block g walks its rest chunk in
 // batches of RBT = 1024/M indices; the batch's a(:, r) and b(:, r) are loaded
-// with lanes on consecutive r (coalesced) into LDS as FP64, then each thread
-// accumulates its Q = max(1, M²/256) pairs over the batch (T = 256/M² threads
-// share a pair when M² < 256).  part[g*P + p].
+// with lanes on consecutive r (coalesced) into LDS as FP64 (rows padded by one
+// element so the strided reads below hit distinct banks).  With M² ≥ 256 each
+// thread owns a QI x QJ block of pairs (rows i0.., columns j0..), so every LDS
+// value it reads feeds QI or QJ products; with M² < 256, T = 256/M² threads
+// share a pair and split the batch.  part[g*P + i*M + j].
 template <typename R, int M>
 __global__ void __launch_bounds__(256)
 k_overlap_lds(const cpx<R> *__restrict__ a, const cpx<R> *__restrict__ b, KeptBits kb, int k,
               uint64_t rest_mask, uint64_t chunk, double2 *__restrict__ part) {
     constexpr int P = M * M;
     constexpr int RBT = 1024 / M;              // rest indices per batch
+    constexpr int LD = RBT + 1;                // padded LDS row
     constexpr int LPT = M * RBT / 256;         // loads per thread per vector (= 4)
     constexpr int Q = P >= 256 ? P / 256 : 1;
+    constexpr int QI = Q >= 16 ? 4 : (Q >= 4 ? 2 : 1);
+    constexpr int QJ = Q / QI;
     constexpr int T = P >= 256 ? 1 : 256 / P;
-    __shared__ double2 sa[M * RBT], sb[M * RBT];
+    __shared__ double2 sa[M * LD], sb[M * LD];
     const int t = threadIdx.x;
     const int rr = t % RBT;                    // this thread's rest slot in a batch
     uint64_t off[LPT];
@@ -320,16 +325,14 @@ k_overlap_lds(const cpx<R> *__restrict__ a, const cpx<R> *__restrict__ b, KeptBi
     const uint64_t drr = deposit_mask((uint64_t)rr, rest_mask);
     const uint64_t dbatch = deposit_mask((uint64_t)RBT, rest_mask);
     const int sub = P >= 256 ? 0 : t / P;
-    int pi[Q], pj[Q];
+    // pair block: rows i0..i0+QI-1, columns j0..j0+QJ-1
+    const int i0 = P >= 256 ? (t / (M / QJ)) * QI : (t % P) / M;
+    const int j0 = P >= 256 ? (t % (M / QJ)) * QJ : (t % P) % M;
+    double2 acc[QI][QJ];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int p = P >= 256 ? t + q * 256 : t % P;
-        pi[q] = p / M;
-        pj[q] = p % M;
-    }
-    double2 acc[Q];
+    for (int x = 0; x < QI; ++x)
 #pragma unroll
-    for (int q = 0; q < Q; ++q) acc[q] = make_double2(0.0, 0.0);
+        for (int y = 0; y < QJ; ++y) acc[x][y] = make_double2(0.0, 0.0);
     const uint64_t r0 = (uint64_t)blockIdx.x * chunk;
     uint64_t xb = deposit_mask(r0, rest_mask);
     const int nb = (int)((chunk + RBT - 1) / RBT);
@@ -341,28 +344,36 @@ k_overlap_lds(const cpx<R> *__restrict__ a, const cpx<R> *__restrict__ b, KeptBi
             const int i = t / RBT + l * (256 / RBT);
             cpx<R> u = live ? a[off[l] | x] : cpx<R>{(R)0, (R)0};
             cpx<R> w = live ? b[off[l] | x] : cpx<R>{(R)0, (R)0};
-            sa[i * RBT + rr] = make_double2((double)u.x, (double)u.y);
-            sb[i * RBT + rr] = make_double2((double)w.x, (double)w.y);
+            sa[i * LD + rr] = make_double2((double)u.x, (double)u.y);
+            sb[i * LD + rr] = make_double2((double)w.x, (double)w.y);
         }
         __syncthreads();
         for (int c = sub; c < RBT; c += T) {
+            double2 u[QI], w[QJ];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const double2 u = sa[pi[q] * RBT + c], w = sb[pj[q] * RBT + c];
-                acc[q].x += u.x * w.x + u.y * w.y;
-                acc[q].y += u.y * w.x - u.x * w.y;
-            }
+            for (int x = 0; x < QI; ++x) u[x] = sa[(i0 + x) * LD + c];
+#pragma unroll
+            for (int y = 0; y < QJ; ++y) w[y] = sb[(j0 + y) * LD + c];
+#pragma unroll
+            for (int x = 0; x < QI; ++x)
+#pragma unroll
+                for (int y = 0; y < QJ; ++y) {
+                    acc[x][y].x += u[x].x * w[y].x + u[x].y * w[y].y;
+                    acc[x][y].y += u[x].y * w[y].x - u[x].x * w[y].y;
+                }
         }
         __syncthreads();
         xb = masked_add(xb, dbatch, rest_mask);
     }
     if (P >= 256) {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) part[(uint64_t)blockIdx.x * P + t + q * 256] = acc[q];
+        for (int x = 0; x < QI; ++x)
+#pragma unroll
+            for (int y = 0; y < QJ; ++y) part[(uint64_t)blockIdx.x * P + (i0 + x) * M + (j0 + y)] = acc[x][y];
         return;
     }
     __shared__ double2 red[256];
-    red[t] = acc[0];
+    red[t] = acc[0][0];
     __syncthreads();
     for (int s2 = T / 2; s2 > 0; s2 >>= 1) {
         if (sub < s2) {

@@ -286,10 +286,11 @@ __device__ __forceinline__ void apply_op(cpx<R> (&v)[1 << RB], cpx<R> (&w)[DUAL 
 
 // Tile-local index of this thread's register 0 in a stage's layout: the thread
 // index with zeros inserted at the (ascending) register bits.
+template <int RB = 4>
 __device__ __forceinline__ int thr_local(const svh_stage_t &S, int tid) {
     int t = tid;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < RB; ++k) {
         const int r = S.rsorted[k];
         t = ((t >> r) << (r + 1)) | (t & ((1 << r) - 1));
     }
@@ -310,10 +311,11 @@ __device__ __forceinline__ uint64_t thr_global(const svh_stage_t &S, int tid) {
 // change and every op is spelled out with constant register indices (no
 // run-time dispatch).  These helpers are the shared prologue / exchange /
 // epilogue; matrices, angles and address tables still come from the program
-// records in device memory, so one compiled program serves any θ.
-template <typename R, int TB>
+// records in device memory, so one compiled program serves any θ.  RB is the
+// number of register bits per thread (2^RB amplitudes in registers).
+template <typename R, int TB, int RB = 4>
 __device__ __forceinline__ uint64_t sp_base(const svh_pass_t *__restrict__ pass, int n) {
-    constexpr int L = TB + 4;
+    constexpr int L = TB + RB;
     uint64_t tmask = 0;
 #pragma unroll
     for (int i = 0; i < L; ++i) tmask |= 1ull << pass->bits[i];
@@ -327,59 +329,59 @@ __device__ __forceinline__ uint64_t sp_base(const svh_pass_t *__restrict__ pass,
     return base;
 }
 
-template <typename R, int TB>
+template <typename R, int TB, int RB = 4>
 __device__ __forceinline__ void sp_load(const cpx<R> *__restrict__ psi, uint64_t base, const svh_stage_t &S,
-                                        cpx<R> (&v)[16]) {
+                                        cpx<R> (&v)[1 << RB]) {
     const cpx<R> *p = psi + (base | thr_global<TB>(S, threadIdx.x));
 #pragma unroll
-    for (int j = 0; j < 16; ++j) v[j] = p[S.goff[j]];
+    for (int j = 0; j < (1 << RB); ++j) v[j] = p[S.goff[j]];
 }
 
-template <typename R, int TB>
+template <typename R, int TB, int RB = 4>
 __device__ __forceinline__ void sp_store(cpx<R> *__restrict__ psi, uint64_t base, const svh_stage_t &S,
-                                         const cpx<R> (&v)[16]) {
+                                         const cpx<R> (&v)[1 << RB]) {
     cpx<R> *p = psi + (base | thr_global<TB>(S, threadIdx.x));
 #pragma unroll
-    for (int j = 0; j < 16; ++j) p[S.goff[j]] = v[j];
+    for (int j = 0; j < (1 << RB); ++j) p[S.goff[j]] = v[j];
 }
 
 // Re-distribute from layout P to layout S through LDS (FIRST: no previous
 // exchange is still being read, so the leading barrier is skipped).
-template <typename R, bool FIRST>
+template <typename R, bool FIRST, int RB = 4>
 __device__ __forceinline__ void sp_exchange(cpx<R> *lds, const svh_stage_t &P, const svh_stage_t &S,
-                                            cpx<R> (&v)[16]) {
-    const int tp = lds_swz(thr_local(P, threadIdx.x));
+                                            cpx<R> (&v)[1 << RB]) {
+    const int tp = lds_swz(thr_local<RB>(P, threadIdx.x));
     if (!FIRST) __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 16; ++j) lds[tp ^ P.lsw[j]] = v[j];
+    for (int j = 0; j < (1 << RB); ++j) lds[tp ^ P.lsw[j]] = v[j];
     __syncthreads();
-    const int tn = lds_swz(thr_local(S, threadIdx.x));
+    const int tn = lds_swz(thr_local<RB>(S, threadIdx.x));
 #pragma unroll
-    for (int j = 0; j < 16; ++j) v[j] = lds[tn ^ S.lsw[j]];
+    for (int j = 0; j < (1 << RB); ++j) v[j] = lds[tn ^ S.lsw[j]];
 }
 
-template <typename R, int A>
-__device__ __forceinline__ void sp_mat1(cpx<R> (&v)[16], const svh_op_t &op) {
-    ap_mat1<R, 4, A>(v, ld_m<R>(op, 0), ld_m<R>(op, 1), ld_m<R>(op, 2), ld_m<R>(op, 3));
+template <typename R, int A, int RB = 4>
+__device__ __forceinline__ void sp_mat1(cpx<R> (&v)[1 << RB], const svh_op_t &op) {
+    ap_mat1<R, RB, A>(v, ld_m<R>(op, 0), ld_m<R>(op, 1), ld_m<R>(op, 2), ld_m<R>(op, 3));
 }
-template <typename R, int H, int L>
-__device__ __forceinline__ void sp_mat2(cpx<R> (&v)[16], const svh_op_t &op) {
+template <typename R, int H, int L, int RB = 4>
+__device__ __forceinline__ void sp_mat2(cpx<R> (&v)[1 << RB], const svh_op_t &op) {
     cpx<R> m[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) m[i] = ld_m<R>(op, i);
-    ap_mat2<R, 4, H, L>(v, m);
+    ap_mat2<R, RB, H, L>(v, m);
 }
-template <typename R, int A, int PAULI>
-__device__ __forceinline__ void sp_rot1(cpx<R> (&v)[16], const svh_op_t &op, const cpx<R> *trig) {
+template <typename R, int A, int PAULI, int RB = 4>
+__device__ __forceinline__ void sp_rot1(cpx<R> (&v)[1 << RB], const svh_op_t &op, const cpx<R> *trig) {
     R cs, sn;
     op_sincos<R>(trig, op, cs, sn);
-    ap_rot1<R, 4, A>(v, PAULI, cs, sn);
+    ap_rot1<R, RB, A>(v, PAULI, cs, sn);
 }
-template <typename R, int H, int L, int PAULI>
-__device__ __forceinline__ void sp_rot2(cpx<R> (&v)[16], const svh_op_t &op, const cpx<R> *trig) {
+template <typename R, int H, int L, int PAULI, int RB = 4>
+__device__ __forceinline__ void sp_rot2(cpx<R> (&v)[1 << RB], const svh_op_t &op, const cpx<R> *trig) {
     R cs, sn;
     op_sincos<R>(trig, op, cs, sn);
-    ap_rot2<R, 4, H, L>(v, PAULI, cs, sn);
+    ap_rot2<R, RB, H, L>(v, PAULI, cs, sn);
 }
 
 // ---- adjoint (ψ and λ together) pieces

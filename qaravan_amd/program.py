@@ -47,14 +47,14 @@ PASS_DT = np.dtype([('kind', '<i4'), ('L', '<i4'), ('tb', '<i4'), ('rb', '<i4'),
                     ('stage_begin', '<i4'), ('n_stages', '<i4'), ('grad_begin', '<i4'),
                     ('n_grad', '<i4'), ('pool_off', '<i8'), ('flags', '<i4'), ('pad', '<i4'),
                     ('bits', '<i4', (MAX_TILE_BITS,))])
-STAGE_DT = np.dtype([('goff', '<i8', (16,)), ('lsw', '<i4', (16,)), ('rbits', '<i4', (8,)),
-                     ('rsorted', '<i4', (4,)), ('tgbit', '<i4', (12,)), ('op_begin', '<i4'),
+STAGE_DT = np.dtype([('goff', '<i8', (32,)), ('lsw', '<i4', (32,)), ('rbits', '<i4', (8,)),
+                     ('rsorted', '<i4', (8,)), ('tgbit', '<i4', (12,)), ('op_begin', '<i4'),
                      ('op_end', '<i4'), ('pad', '<i4', (2,))])
 OP_DT = np.dtype([('kind', '<i4'), ('a', '<i4'), ('b', '<i4'), ('param', '<i4'), ('pauli', '<i4'),
                   ('flags', '<i4'), ('grad_slot', '<i4'), ('pad', '<i4'), ('m', '<f8', (32,)),
                   ('mf', '<f4', (32,))])
 assert HDR_DT.itemsize == 96 and PASS_DT.itemsize == 112
-assert STAGE_DT.itemsize == 304 and OP_DT.itemsize == 416
+assert STAGE_DT.itemsize == 512 and OP_DT.itemsize == 416
 
 
 _ZERO_M = np.zeros(32)
@@ -65,18 +65,21 @@ def lds_swz(li):
     return li ^ ((li >> 5) & 31)
 
 
-_JBITS = np.array([[(j >> k) & 1 for k in range(RB)] for j in range(1 << RB)], dtype=np.int64)
+_JBITS = {r: np.array([[(j >> k) & 1 for k in range(r)] for j in range(1 << r)], dtype=np.int64)
+          for r in (4, 5)}
 
 
 def stage_tables(tile_bits, tb, rbits):
     """Address tables of one stage layout (see svh_stage_t): thread bits fill
-    the non-register tile-local bits in ascending order."""
-    L = tb + RB
+    the non-register tile-local bits in ascending order; len(rbits) register
+    bits give 2^len(rbits) registers."""
+    r = len(rbits)
+    L = tb + r
     rset = set(rbits)
     tgbit = [tile_bits[x] for x in range(L) if x not in rset]
     rb = np.asarray(rbits, dtype=np.int64)
-    loc = _JBITS @ (np.int64(1) << rb)
-    goff = _JBITS @ (np.int64(1) << np.asarray([tile_bits[x] for x in rbits], dtype=np.int64))
+    loc = _JBITS[r] @ (np.int64(1) << rb)
+    goff = _JBITS[r] @ (np.int64(1) << np.asarray([tile_bits[x] for x in rbits], dtype=np.int64))
     return dict(rsorted=sorted(rbits), tgbit=tgbit, lsw=loc ^ ((loc >> 5) & 31), goff=goff, loc=loc)
 
 PAULI_CODE = {'x': 1, 'y': 2, 'z': 3}
@@ -290,6 +293,7 @@ class Pass:
     stages: list = field(default_factory=list)
     mat: np.ndarray = None
     flags: int = 0
+    rb: int = RB
 
 
 @dataclass
@@ -307,7 +311,10 @@ class TileConfig:
     fuse2q: fold 1-qubit gates into neighbouring 2-qubit gates (fuse_2q);
     io_lane_bits: tile-local bits that the first and last stage of a pass keep
               off the registers, so lanes 0..2^io_lane_bits-1 load and store
-              consecutive amplitudes (4 = 128-byte runs for complex64).
+              consecutive amplitudes (4 = 128-byte runs for complex64);
+    rb: register bits per thread (tile L = tb + rb): 4, or 5 for the per-pass
+              compiled forward kernels only (32 amplitudes per thread, fewer
+              LDS re-distributions per pass).
     """
     tb: int = 8
     low_bits: int = 6
@@ -316,6 +323,7 @@ class TileConfig:
     specialize: bool = False
     fuse2q: bool = True
     io_lane_bits: int = 6
+    rb: int = RB
 
 
 def _is_barrier(op):
@@ -324,7 +332,7 @@ def _is_barrier(op):
 
 def partition(ops, n, cfg):
     """Greedy packing of ops into passes (see module docstring, step 3)."""
-    L = cfg.tb + RB
+    L = cfg.tb + cfg.rb
     if L > n:
         raise ValueError(f"tile of {L} bits exceeds n={n}")
     m = min(cfg.low_bits, L)
@@ -370,7 +378,7 @@ def partition(ops, n, cfg):
             x += 1
         tset = set(taken)
         p = Pass(PASS_TILE, sorted(T), [rem[i] for i in taken], tb=cfg.tb,
-                 flags=PASSF_PREFETCH if cfg.prefetch else 0)
+                 flags=PASSF_PREFETCH if cfg.prefetch else 0, rb=cfg.rb)
         passes.append(p)
         rem = [op for i, op in enumerate(rem) if i not in tset]
     return passes
@@ -378,6 +386,7 @@ def partition(ops, n, cfg):
 
 def _stage_ops(p, io_lane_bits=6):
     """Split a tile pass's ops into register stages (tile-local bits)."""
+    RB = p.rb
     L = p.tb + RB
     loc = {g: i for i, g in enumerate(p.bits)}
     pend = [(op, tuple(loc[b] for b in op.bits)) for op in p.ops]
@@ -487,10 +496,10 @@ def encode(n, passes, grad=False, n_params=0):
     for pi, p in enumerate(passes):
         pr = pass_rows[pi]
         pr['kind'] = p.kind
-        pr['rb'] = RB
+        pr['rb'] = p.rb
         pr['flags'] = p.flags
         if p.kind == PASS_TILE:
-            L = p.tb + RB
+            L = p.tb + p.rb
             pr['L'] = L
             pr['tb'] = p.tb
             pr['bits'][:L] = p.bits
@@ -500,12 +509,12 @@ def encode(n, passes, grad=False, n_params=0):
             slot = 0
             for st in p.stages:
                 s = np.zeros((), dtype=STAGE_DT)
-                s['rbits'][:RB] = st.rbits
+                s['rbits'][:p.rb] = st.rbits
                 tab = stage_tables(p.bits, p.tb, st.rbits)
-                s['rsorted'][:] = tab['rsorted']
+                s['rsorted'][:p.rb] = tab['rsorted']
                 s['tgbit'][:p.tb] = tab['tgbit']
-                s['lsw'][:] = tab['lsw']
-                s['goff'][:] = tab['goff']
+                s['lsw'][:1 << p.rb] = tab['lsw']
+                s['goff'][:1 << p.rb] = tab['goff']
                 s['op_begin'] = len(op_rows)
                 ridx = {x: k for k, x in enumerate(st.rbits)}
                 for op, lb in st.ops:
@@ -601,10 +610,15 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
             else:
                 passes.append(Pass(PASS_DENSE, list(_dense_bits(op)), [op], mat=_dense_mat(op)))
         return encode(n, passes, grad, n_params)
-    tb = min(cfg.tb, n - RB)
+    if cfg.rb not in (4, 5):
+        raise ValueError("rb (register bits per thread) must be 4 or 5")
+    if cfg.rb == 5 and (grad or cfg.prefetch or not cfg.specialize):
+        raise ValueError("rb=5 runs only in the per-pass compiled forward kernels (specialize=True, "
+                         "no prefetch, no adjoint)")
+    tb = min(cfg.tb, n - cfg.rb)
     cfg = TileConfig(tb=tb, low_bits=max(1, cfg.low_bits), max_cost=cfg.max_cost,
                      prefetch=cfg.prefetch and not grad, specialize=cfg.specialize, fuse2q=cfg.fuse2q,
-                     io_lane_bits=cfg.io_lane_bits)
+                     io_lane_bits=cfg.io_lane_bits, rb=cfg.rb)
     passes = partition(ops, n, cfg)
     for p in passes:
         if p.kind == PASS_TILE:

@@ -44,7 +44,9 @@ def lds_bytes(p, dtype_code, dual):
 
 
 def eligible(p, dtype_code, dual=False):
-    return (int(p['kind']) == P.PASS_TILE and int(p['rb']) == 4
+    """Forward kernels take 4 or 5 register bits; adjoint kernels 4."""
+    rb = int(p['rb'])
+    return (int(p['kind']) == P.PASS_TILE and (rb == 4 or (rb == 5 and not dual))
             and lds_bytes(p, dtype_code, dual) <= MAX_LDS and (dual or int(p['n_grad']) == 0))
 
 
@@ -117,6 +119,7 @@ def program_source(blob, dtype_code, only=None):
         if not eligible(p, dtype_code) or (only is not None and pi not in only):
             continue
         tb = int(p['tb'])
+        rb = int(p['rb'])
         s0, ns = int(p['stage_begin']), int(p['n_stages'])
         op0 = int(stages[s0]['op_begin'])
         out.append(f'extern "C" __global__ void __launch_bounds__({1 << tb}) '
@@ -125,35 +128,35 @@ def program_source(blob, dtype_code, only=None):
                    f"const cpx<{R}> *__restrict__ trig) {{")
         out.append("    extern __shared__ unsigned char smem_raw[];")
         out.append(f"    cpx<{R}> *lds = reinterpret_cast<cpx<{R}> *>(smem_raw);")
-        out.append(f"    cpx<{R}> v[16];")
-        out.append(f"    const uint64_t base = sp_base<{R}, {tb}>(pass, n);")
-        out.append(f"    sp_load<{R}, {tb}>(psi, base, st[0], v);")
+        out.append(f"    cpx<{R}> v[{1 << rb}];")
+        out.append(f"    const uint64_t base = sp_base<{R}, {tb}, {rb}>(pass, n);")
+        out.append(f"    sp_load<{R}, {tb}, {rb}>(psi, base, st[0], v);")
         for s in range(ns):
             S = stages[s0 + s]
             if s > 0:
                 first = "true" if s == 1 else "false"
-                out.append(f"    sp_exchange<{R}, {first}>(lds, st[{s - 1}], st[{s}], v);")
+                out.append(f"    sp_exchange<{R}, {first}, {rb}>(lds, st[{s - 1}], st[{s}], v);")
             for oi in range(int(S['op_begin']), int(S['op_end'])):
                 o = ops[oi]
                 k = oi - op0
                 kind, a, b, pauli = int(o['kind']), int(o['a']), int(o['b']), int(o['pauli'])
                 if kind == P.OP_MAT1:
-                    out.append(f"    sp_mat1<{R}, {a}>(v, ops[{k}]);")
+                    out.append(f"    sp_mat1<{R}, {a}, {rb}>(v, ops[{k}]);")
                 elif kind == P.OP_MAT2:
-                    out.append(f"    sp_mat2<{R}, {a}, {b}>(v, ops[{k}]);")
+                    out.append(f"    sp_mat2<{R}, {a}, {b}, {rb}>(v, ops[{k}]);")
                 elif kind == P.OP_CX:
                     hi, lo = max(a, b), min(a, b)
                     th = "true" if a > b else "false"
-                    out.append(f"    ap_cx_t<{R}, 4, {hi}, {lo}, {th}>(v);")
+                    out.append(f"    ap_cx_t<{R}, {rb}, {hi}, {lo}, {th}>(v);")
                 elif kind == P.OP_CZ:
-                    out.append(f"    ap_cz<{R}, 4, {max(a, b)}, {min(a, b)}>(v);")
+                    out.append(f"    ap_cz<{R}, {rb}, {max(a, b)}, {min(a, b)}>(v);")
                 elif kind == P.OP_ROT1:
-                    out.append(f"    sp_rot1<{R}, {a}, {pauli}>(v, ops[{k}], trig);")
+                    out.append(f"    sp_rot1<{R}, {a}, {pauli}, {rb}>(v, ops[{k}], trig);")
                 elif kind == P.OP_ROT2:
-                    out.append(f"    sp_rot2<{R}, {a}, {b}, {pauli}>(v, ops[{k}], trig);")
+                    out.append(f"    sp_rot2<{R}, {a}, {b}, {pauli}, {rb}>(v, ops[{k}], trig);")
                 else:
                     raise ValueError(kind)
-        out.append(f"    sp_store<{R}, {tb}>(psi, base, st[{ns - 1}], v);")
+        out.append(f"    sp_store<{R}, {tb}, {rb}>(psi, base, st[{ns - 1}], v);")
         out.append("}")
         out.append("")
     return "\n".join(out)

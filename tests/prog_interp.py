@@ -71,7 +71,7 @@ def run(blob, sv, theta=None, lam=None, grad=None):
             tile_bits = [int(b) for b in p['bits'][:L]]
             for s in range(int(p['stage_begin']), int(p['stage_begin']) + int(p['n_stages'])):
                 st = stages[s]
-                rb = [int(x) for x in st['rbits'][:P.RB]]
+                rb = [int(x) for x in st['rbits'][:int(p['rb'])]]
                 for oi in range(int(st['op_begin']), int(st['op_end'])):
                     o = ops[oi]
                     kind_o = int(o['kind'])

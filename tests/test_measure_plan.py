@@ -186,14 +186,20 @@ def test_overlap_plan_covers_each_rest_once_per_pair(n, kept):
                     assert x == deposit_mask(g * chunk + bt * RBT + rr, rest) or not live
                     for l in range(LPT):
                         i = t // RBT + l * (256 // RBT)
-                        assert i < M and i * RBT + rr < M * RBT
+                        assert i < M and i * (RBT + 1) + rr < M * (RBT + 1)
                         rows.add((i, rr))
                         if live:
                             assert (deposit_kept(i, kept) | x) < 1 << n
                 assert len(rows) == M * RBT             # the batch fills the whole LDS slab
+                QI = 4 if Q >= 16 else (2 if Q >= 4 else 1)
+                QJ = Q // QI
                 for t in range(256):
                     sub = 0 if P >= 256 else t // P
-                    pairs = [t + q * 256 for q in range(Q)] if P >= 256 else [t % P]
+                    if P >= 256:
+                        i0, j0 = (t // (M // QJ)) * QI, (t % (M // QJ)) * QJ
+                        pairs = [(i0 + x) * M + j0 + y for x in range(QI) for y in range(QJ)]
+                    else:
+                        pairs = [t % P]
                     for c in range(sub, RBT, T):
                         r = bt * RBT + c
                         if r >= chunk:

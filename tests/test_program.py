@@ -115,6 +115,38 @@ def test_gate_count_and_fusion_c1():
     assert n1 <= 28 * 40          # RX·RY·RZ per qubit per layer fused into one 2x2
 
 
+@pytest.mark.parametrize("tb,low,cost,io", [(3, 4, 1e9, 4), (4, 4, 48.0, 6), (6, 5, 160.0, 4), (8, 4, 1e9, 4)])
+@pytest.mark.parametrize("seed", range(3))
+def test_five_register_bit_programs_match_oracle(tb, low, cost, io, seed):
+    rng = np.random.default_rng(500 + seed)
+    n = max(tb + 5, 10) + seed
+    circ = random_circuit(n, 70, rng, k3=(seed == 2))
+    cfg = P.TileConfig(tb=tb, low_bits=low, max_cost=cost, specialize=True, io_lane_bits=io, rb=5)
+    prog = compile_circ(circ, cfg)
+    h, passes, stages, ops, _, _ = prog_interp.decode(prog.blob)
+    tile = [p for p in passes if p['kind'] == P.PASS_TILE]
+    assert tile and all(int(p['rb']) == 5 and int(p['L']) == int(p['tb']) + 5 for p in tile)
+    for p in tile:
+        for si in range(int(p['stage_begin']), int(p['stage_begin']) + int(p['n_stages'])):
+            st = stages[si]
+            assert len(set(int(x) for x in st['rbits'][:5])) == 5
+            for o in ops[int(st['op_begin']):int(st['op_end'])]:
+                assert 0 <= int(o['a']) < 5
+    got, _ = prog_interp.run(prog.blob, O.init_state(n))
+    assert np.linalg.norm(got - O.run(circ.gate_list, n)) < 1e-12
+
+
+def test_five_register_bits_only_in_compiled_forward_kernels():
+    circ = brickwork_circuit(12, 2)
+    ops = P.lower_circuit(circ.build(), 12)
+    with pytest.raises(ValueError):
+        P.compile_ops(ops, 12, P.TileConfig(tb=6, rb=5))                      # needs specialize=True
+    with pytest.raises(ValueError):
+        P.compile_ops(ops, 12, P.TileConfig(tb=6, rb=5, specialize=True), grad=True)
+    with pytest.raises(ValueError):
+        P.compile_ops(ops, 12, P.TileConfig(tb=6, rb=3, specialize=True))
+
+
 def _ops_unitary(ops, n):
     """Full 2^n x 2^n matrix of an op list, built column by column from the
     oracle's gate application on basis states (bits -> qubits n-1-b)."""
@@ -199,13 +231,14 @@ def test_stage_tables_cover_each_tile_once():
     offsets (thread part | register part) hit every amplitude of the tile once,
     and the swizzled LDS indices are a permutation of [0, 2^L)."""
     circ = brickwork_circuit(16, 3)
-    for cfg in (P.TileConfig(tb=4, low_bits=4, max_cost=40), P.TileConfig(tb=8, low_bits=6, max_cost=96)):
+    for cfg in (P.TileConfig(tb=4, low_bits=4, max_cost=40), P.TileConfig(tb=8, low_bits=6, max_cost=96),
+                P.TileConfig(tb=6, low_bits=4, max_cost=1e9, io_lane_bits=4, rb=5, specialize=True)):
         prog = compile_circ(circ, cfg)
         h, passes, stages, ops, _, _ = prog_interp.decode(prog.blob)
         for p in passes:
             if p['kind'] != P.PASS_TILE:
                 continue
-            L, tb = int(p['L']), int(p['tb'])
+            L, tb, rb = int(p['L']), int(p['tb']), int(p['rb'])
             tile = set()
             for i in range(1 << L):
                 a = 0
@@ -220,10 +253,10 @@ def test_stage_tables_cover_each_tile_once():
                     for i in range(tb):
                         tg |= ((t >> i) & 1) << int(st['tgbit'][i])
                     tl = t
-                    for r in st['rsorted']:
+                    for r in st['rsorted'][:rb]:
                         r = int(r)
                         tl = ((tl >> r) << (r + 1)) | (tl & ((1 << r) - 1))
-                    for j in range(16):
+                    for j in range(1 << rb):
                         seen_g.add(tg | int(st['goff'][j]))
                         seen_l.add(P.lds_swz(tl) ^ int(st['lsw'][j]))
                 assert seen_g == tile
