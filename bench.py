@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--no-blocks", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-measure", action="store_true")
+    ap.add_argument("--batch-interp", action="store_true", help="batch leg with the op-interpreting kernels")
     ap.add_argument("--profile-forward", action="store_true",
                     help="only the timed forward leg (for rocprofv3 runs)")
     ap.add_argument("--profile-blocks", action="store_true", help="only the MFMA block leg")
@@ -326,7 +327,7 @@ def bench_measure(dev, n=28, reps=5, shots=1_000_000):
     return out
 
 
-def bench_batch(dev, nc=512, seed=20260004, check=6):
+def bench_batch(dev, nc=512, seed=20260004, check=6, specialize=True):
     """512 HEA circuits (10..22 qubits, 4..32 layers) forward + adjoint as one
     HIP-graph launch; latency from device timestamps."""
     import numpy as np
@@ -339,7 +340,8 @@ def bench_batch(dev, nc=512, seed=20260004, check=6):
     t0 = time.perf_counter()
     b = BatchedAdjoint(None, [tfim_terms(int(n)) for n in ns], [len(t) for t in thetas], device=dev,
                        dtype=torch.complex64, keys=list(zip(ns.tolist(), ls.tolist())),
-                       op_lists=[hea_ops(int(n), int(l)) for n, l in zip(ns, ls)], ns=ns.tolist())
+                       op_lists=[hea_ops(int(n), int(l)) for n, l in zip(ns, ls)], ns=ns.tolist(),
+                       specialize=specialize)
     log(f"[bench] batch: planned {nc} circuits in {time.perf_counter() - t0:.1f} s, {len(b.launches)} launches")
     b.set_theta(thetas)
     b.capture()
@@ -365,8 +367,9 @@ def bench_batch(dev, nc=512, seed=20260004, check=6):
         g = g.cpu().numpy()
         worst = max(worst, float(np.linalg.norm(grads[c] - g) / np.linalg.norm(g)))
     amps = float(sum(2.0 ** int(n) for n in ns))
+    kern = f"{len(b.kernels.names)} pass-structure kernels" if specialize else "op-interpreting kernels"
     return dict(workload=f"{nc} HEA circuits, n~U{{10..22}}, layers~U{{4..32}}, TFIM loss, forward+adjoint, "
-                         f"one HIP graph ({b.n_launches} kernels)",
+                         f"one HIP graph ({b.n_launches} launches, {kern})",
                 circuits_per_s=nc / dt, ms_per_batch=dt * 1e3, p50_latency_ms=float(np.percentile(lat, 50)) / 1e3,
                 p99_latency_ms=float(np.percentile(lat, 99)) / 1e3, max_rel_grad_err_vs_c128_sample=worst,
                 checked_circuits=len(idx), total_amplitudes=amps)
@@ -500,7 +503,7 @@ def main():
         log(f"[bench] fused blocks: {blocks}")
     batch = None
     if not args.no_batch:
-        batch = bench_batch(dev)
+        batch = bench_batch(dev, specialize=not args.batch_interp)
         log(f"[bench] batch: {batch}")
     c0 = bench_c0(dev)
     log(f"[bench] c0: {c0}")

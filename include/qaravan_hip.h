@@ -29,7 +29,7 @@
  *                         torch backend); same gradient, computed by the adjoint method
  *   svh_apply_block5      no counterpart: op_action applies a 5-qubit gate with one
  *                         tensordot; here it is an FP32 MFMA GEMM (c64)
- *   svh_batch_*           tensorQ/monte_carlo_sim.py:14-35 (many small StatevectorSim runs,
+ *   svh_batch_*, svh_spb_* tensorQ/monte_carlo_sim.py:14-35 (many small StatevectorSim runs,
  *                         one after another); here a whole batch of parametrised circuits
  *                         with their adjoint gradients runs as one launch sequence
  *   svh_timestamp         no counterpart (device clock for per-circuit latencies)
@@ -277,6 +277,15 @@ int svh_batch_init(void *buf, int64_t total, const int64_t *offsets_dev, int nof
 
 /* (cos θ, sin θ) for a concatenated θ vector, rounded to the state precision. */
 int svh_batch_trig(const double *theta_dev, int64_t np, void *trig_dev, int dtype, void *stream);
+
+/* Batch kernels compiled per pass structure (svh_spb_*): load the named
+ * kernels from code objects (handle freed with svh_sp_free), then launch
+ * kernel ``index`` over an item table as svh_batch_tile does: workgroup b
+ * runs item i with wg_begin[i] <= b < wg_begin[i+1], every item's pass having
+ * the structure the kernel was compiled for. */
+int svh_spb_load(const char *const *paths, int npaths, const char *const *names, int nnames, void **handle);
+int svh_spb_launch(void *handle, int index, const void *items_dev, int nitems, int64_t total_wg, int tb,
+                   size_t lds_bytes, void *stream);
 
 /* out_dev[0] = wall clock (s_memrealtime, 100 MHz ticks), for batch latencies. */
 int svh_timestamp(uint64_t *out_dev, void *stream);

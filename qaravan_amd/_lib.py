@@ -56,6 +56,10 @@ SIGNATURES = {
     "svh_batch_init": (_c_int, [_c_void_p, ctypes.c_int64, _c_void_p, _c_int, _c_int, _c_void_p]),
     "svh_batch_trig": (_c_int, [_c_void_p, ctypes.c_int64, _c_void_p, _c_int, _c_void_p]),
     "svh_timestamp": (_c_int, [_c_void_p, _c_void_p]),
+    "svh_spb_load": (_c_int, [ctypes.POINTER(ctypes.c_char_p), _c_int, ctypes.POINTER(ctypes.c_char_p), _c_int,
+                              ctypes.POINTER(_c_void_p)]),
+    "svh_spb_launch": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_int, ctypes.c_int64, _c_int, _c_size_t,
+                                _c_void_p]),
     "svh_measure_scratch_bytes": (_c_size_t, [_c_int, _c_int, _c_int]),
     "svh_marginal_probs": (_c_int, [_c_void_p, _c_int, _c_int, ctypes.POINTER(_c_int), _c_int, _c_void_p,
                                     _c_void_p, _c_size_t, _c_void_p]),

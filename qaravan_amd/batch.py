@@ -55,11 +55,13 @@ class BatchedAdjoint:
     circuits (rotations carry ``param_index`` into their own θ_c)."""
 
     def __init__(self, circuits, terms_list, n_params_list, device="cuda", dtype=torch.complex64,
-                 tb_of_n=None, max_cost=1e9, keys=None, op_lists=None, ns=None):
+                 tb_of_n=None, max_cost=1e9, keys=None, op_lists=None, ns=None, specialize=False):
         """``circuits``: qaravan-style circuits (rotations carry ``param_index``),
         or None with ``op_lists`` (elementary ops) and ``ns``.  Circuits with equal
         ``keys`` share compiled programs (θ lives on the device, so programs
-        depend on the structure only)."""
+        depend on the structure only).  ``specialize``: run every pass with a
+        kernel compiled for its structure (one launch per structure and round)
+        instead of the op-interpreting batch kernel."""
         _lib.require_gpu()
         self.device = torch.device(device)
         self.dtype = dtype
@@ -134,10 +136,32 @@ class BatchedAdjoint:
         tt = [_term_table(self.ns[c], terms_list[c]) for c in range(nc)]
         self.t_off = np.concatenate([[0], np.cumsum([len(t) for t in tt])]).astype(np.int64)
         self.terms = torch.from_numpy(np.concatenate(tt).view(np.uint8).copy()).to(dev)
+        self.kernels = self._specialise() if specialize else None
         self._plan(es)
         self.graph = None
 
     # ---------------------------------------------------------------- plan
+    def _specialise(self):
+        """Kernel name of every (program, pass): passes with the same body text
+        share one compiled batch kernel."""
+        import hashlib
+
+        from .specialize import BatchKernels, batch_body
+        self.kname = {}
+        kernels = {}
+        for pp in self.pps:
+            for dual, prog in ((False, pp.fwd), (True, pp.adj)):
+                if id(prog) in self.kname:
+                    continue
+                names = []
+                for pi, pa in enumerate(prog.passes):
+                    body = batch_body(prog.blob, pi, self.code, dual)
+                    nm = ("svh_spbd_" if dual else "svh_spb_") + hashlib.sha1(body.encode()).hexdigest()[:16]
+                    kernels.setdefault(nm, (body, dual, pa.tb))
+                    names.append(nm)
+                self.kname[id(prog)] = names
+        return BatchKernels(kernels, self.code)
+
     def _pass_ptrs(self, prog_addr, prog, pi):
         h = np.frombuffer(prog.blob[:HDR_DT.itemsize].tobytes(), dtype=HDR_DT)[0]
         return (prog_addr + int(h['off_passes']) + pi * PASS_DT.itemsize,
@@ -172,7 +196,8 @@ class BatchedAdjoint:
                     n = self.ns[c]
                     ppass, pst, pops, pgrad = self._pass_ptrs(self.prog_addr[c][which], prog, r)
                     ng = sum(1 for st in p.stages for op, _ in st.ops if op.flags & 2) if which else 0
-                    groups.setdefault(tb, []).append(dict(
+                    gkey = (tb, self.kname[id(prog)][r]) if self.kernels is not None else (tb, None)
+                    groups.setdefault(gkey, []).append(dict(
                         psi=psi0 + es * int(self.amp_off[c]), lam=(lam0 + es * int(self.amp_off[c])) if which else 0,
                         pass_=ppass, stages=pst, ops=pops, trig=trig0 + es * int(self.par_off[c]),
                         gpart=gp0 + 8 * int(self.gp_off[c]), nwg=2 ** (n - L), n=n, ng=ng))
@@ -188,7 +213,7 @@ class BatchedAdjoint:
                                          out=self.grad.data_ptr() + 8 * int(self.par_off[c]),
                                          tstamp=(self.t_end.data_ptr() + 8 * c) if last else 0,
                                          nblocks=2 ** (n - L), ng=ng))
-                for tb, items in sorted(groups.items()):
+                for (tb, kname), items in sorted(groups.items(), key=lambda kv: (kv[0][0], kv[0][1] or "")):
                     arr = np.zeros(len(items), dtype=ITEM_DT)
                     wg = 0
                     for i, it in enumerate(items):
@@ -196,7 +221,14 @@ class BatchedAdjoint:
                                   it['gpart'], wg, it['n'], 0)
                         wg += it['nwg']
                     mng = max(it['ng'] for it in items)
-                    self.launches.append(('tile', add_table(arr), len(items), wg, tb, which, mng))
+                    if kname is None:
+                        self.launches.append(('tile', add_table(arr), len(items), wg, tb, which, mng))
+                    else:
+                        L = tb + RB
+                        nw = max(1, 2 ** tb // 64)
+                        lds = (2 if which else 1) * es * 2 ** L + (8 * nw * mng if which else 0)
+                        self.launches.append(('spb', add_table(arr), len(items), wg, tb, lds,
+                                              self.kernels.index[kname]))
                 if reds:
                     arr = np.zeros(len(reds), dtype=RED_DT)
                     sb = 0
@@ -249,6 +281,10 @@ class BatchedAdjoint:
             if L[0] == 'tile':
                 _, off, ni, wg, tb, dual, mng = L
                 _lib.check(lib.svh_batch_tile(tb0 + off, ni, wg, self.code, tb, dual, mng, st), "svh_batch_tile")
+            elif L[0] == 'spb':
+                _, off, ni, wg, tb, lds, k = L
+                _lib.check(lib.svh_spb_launch(self.kernels.handle, k, tb0 + off, ni, wg, tb, lds, st),
+                           "svh_spb_launch")
             elif L[0] == 'reduce':
                 _, off, ni, nb = L
                 _lib.check(lib.svh_batch_reduce(tb0 + off, ni, nb, st), "svh_batch_reduce")

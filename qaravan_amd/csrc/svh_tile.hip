@@ -735,6 +735,54 @@ int svh_sp_load(const char *const *paths, int npaths, int n_passes, void **handl
     return 0;
 }
 
+int svh_spb_load(const char *const *paths, int npaths, const char *const *names, int nnames, void **handle) {
+    if (!paths || npaths <= 0 || !names || nnames <= 0 || !handle) return fail(-1, "bad arguments");
+    auto *H = new SpHandle();
+    H->fn.assign(nnames, nullptr);
+    for (int i = 0; i < npaths; ++i) {
+        hipModule_t m = nullptr;
+        hipError_t e = hipModuleLoad(&m, paths[i]);
+        if (e != hipSuccess) {
+            for (auto mm : H->modules) (void)hipModuleUnload(mm);
+            delete H;
+            return hip_status(e, "hipModuleLoad");
+        }
+        H->modules.push_back(m);
+        for (int k = 0; k < nnames; ++k) {
+            if (H->fn[k] || !names[k]) continue;
+            hipFunction_t f = nullptr;
+            if (hipModuleGetFunction(&f, m, names[k]) == hipSuccess) H->fn[k] = f;
+        }
+    }
+    (void)hipGetLastError();  // lookups of kernels in the other modules are expected to fail
+    for (int k = 0; k < nnames; ++k) {
+        if (!H->fn[k]) {
+            const std::string nm = names[k] ? names[k] : "(null)";
+            for (auto mm : H->modules) (void)hipModuleUnload(mm);
+            delete H;
+            return fail(-9, "batch kernel " + nm + " not found in the code objects");
+        }
+    }
+    *handle = H;
+    return 0;
+}
+
+int svh_spb_launch(void *handle, int index, const void *items_dev, int nitems, int64_t total_wg, int tb,
+                   size_t lds_bytes, void *stream) {
+    const auto *H = static_cast<const SpHandle *>(handle);
+    if (!H || !items_dev || nitems <= 0) return fail(-1, "bad arguments");
+    if (index < 0 || index >= (int)H->fn.size()) return fail(-4, "batch kernel index out of range");
+    if (tb < 1 || tb > 10) return fail(-4, "bad workgroup size");
+    if (total_wg <= 0 || total_wg > 0x7fffffff) return fail(-4, "bad workgroup count");
+    if (lds_bytes > kMaxLds) return fail(-7, "batch kernel needs more than 160 KiB of LDS");
+    const void *it = items_dev;
+    int ni = nitems;
+    void *args[] = {(void *)&it, &ni};
+    return hip_status(hipModuleLaunchKernel(H->fn[index], (unsigned)total_wg, 1, 1, 1u << tb, 1, 1, (unsigned)lds_bytes,
+                                            (hipStream_t)stream, args, nullptr),
+                      "hipModuleLaunchKernel(svh_spb)");
+}
+
 int svh_sp_free(void *handle) {
     auto *H = static_cast<SpHandle *>(handle);
     if (!H) return 0;

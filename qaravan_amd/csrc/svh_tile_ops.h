@@ -314,12 +314,12 @@ __device__ __forceinline__ uint64_t thr_global(const svh_stage_t &S, int tid) {
 // records in device memory, so one compiled program serves any θ.  RB is the
 // number of register bits per thread (2^RB amplitudes in registers).
 template <typename R, int TB, int RB = 4>
-__device__ __forceinline__ uint64_t sp_base(const svh_pass_t *__restrict__ pass, int n) {
+__device__ __forceinline__ uint64_t sp_base_wg(const svh_pass_t *__restrict__ pass, int n, uint64_t wg) {
     constexpr int L = TB + RB;
     uint64_t tmask = 0;
 #pragma unroll
     for (int i = 0; i < L; ++i) tmask |= 1ull << pass->bits[i];
-    uint64_t base = 0, g = blockIdx.x;
+    uint64_t base = 0, g = wg;
     for (int b = 0; b < n && g; ++b) {
         if (!((tmask >> b) & 1)) {
             base |= (g & 1ull) << b;
@@ -327,6 +327,19 @@ __device__ __forceinline__ uint64_t sp_base(const svh_pass_t *__restrict__ pass,
         }
     }
     return base;
+}
+
+// Index of this workgroup's item in a batch table: the last item whose
+// wg_begin is <= blockIdx.x (wg_begin ascending, first item at 0).
+__device__ __forceinline__ int sp_find_item(const svh_batch_item_t *__restrict__ items, int nitems) {
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].wg_begin <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 template <typename R, int TB, int RB = 4>
@@ -459,15 +472,15 @@ __device__ __forceinline__ void spd_rot2(cpx<R> (&v)[16], cpx<R> (&w)[16], const
     ap_rot2<R, 4, H, L>(v, PAULI, cs, sn);
 }
 
-// Sum the waves' gradient slots of this workgroup into its partial row.
+// Sum the waves' gradient slots of this workgroup into its partial row wg.
 template <int TB>
-__device__ __forceinline__ void sp_grad_flush(const double *gacc, int ng, double *gpart) {
+__device__ __forceinline__ void sp_grad_flush_wg(const double *gacc, int ng, double *gpart, uint64_t wg) {
     constexpr int NW = ((1 << TB) + 63) / 64;
     __syncthreads();
     for (int sl = threadIdx.x; sl < ng; sl += (1 << TB)) {
         double acc = 0.0;
         for (int wv = 0; wv < NW; ++wv) acc += gacc[wv * ng + sl];
-        gpart[(uint64_t)blockIdx.x * ng + sl] = acc;
+        gpart[wg * ng + sl] = acc;
     }
 }
 

@@ -50,6 +50,120 @@ def eligible(p, dtype_code, dual=False):
             and lds_bytes(p, dtype_code, dual) <= MAX_LDS and (dual or int(p['n_grad']) == 0))
 
 
+def _body(R, p, stages, ops, dual):
+    """Statements of one pass after the prologue has defined psi (and lam),
+    n, pass, st (this pass's stages), ops (this pass's ops), trig, gpart and
+    wg (the tile index)."""
+    tb, rb = int(p['tb']), int(p['rb'])
+    s0, ns = int(p['stage_begin']), int(p['n_stages'])
+    op0 = int(stages[s0]['op_begin'])
+    ng = int(p['n_grad'])
+    L = int(p['L'])
+    out = ["    extern __shared__ unsigned char smem_raw[];",
+           f"    cpx<{R}> *lds = reinterpret_cast<cpx<{R}> *>(smem_raw);"]
+    if dual:
+        out += [f"    cpx<{R}> *lds2 = lds + {2 ** L};",
+                f"    double *gacc = reinterpret_cast<double *>(lds2 + {2 ** L});",
+                f"    cpx<{R}> v[16], w[16];",
+                f"    const uint64_t base = sp_base_wg<{R}, {tb}>(pass, n, wg);",
+                f"    sp_load2<{R}, {tb}>(psi, lam, base, st[0], v, w);"]
+    else:
+        out += [f"    cpx<{R}> v[{1 << rb}];",
+                f"    const uint64_t base = sp_base_wg<{R}, {tb}, {rb}>(pass, n, wg);",
+                f"    sp_load<{R}, {tb}, {rb}>(psi, base, st[0], v);"]
+    for s in range(ns):
+        S = stages[s0 + s]
+        if s > 0:
+            first = "true" if s == 1 else "false"
+            out.append(f"    sp_exchange2<{R}, {first}>(lds, lds2, st[{s - 1}], st[{s}], v, w);" if dual else
+                       f"    sp_exchange<{R}, {first}, {rb}>(lds, st[{s - 1}], st[{s}], v);")
+        for oi in range(int(S['op_begin']), int(S['op_end'])):
+            o = ops[oi]
+            k = oi - op0
+            kind, a, b, pauli = int(o['kind']), int(o['a']), int(o['b']), int(o['pauli'])
+            hi, lo = max(a, b), min(a, b)
+            if dual:
+                g = "true" if int(o['flags']) & P.OPF_GRAD else "false"
+                if kind == P.OP_MAT1:
+                    out.append(f"    sp_mat1<{R}, {a}>(v, ops[{k}]); sp_mat1<{R}, {a}>(w, ops[{k}]);")
+                elif kind == P.OP_MAT2:
+                    out.append(f"    sp_mat2<{R}, {a}, {b}>(v, ops[{k}]); sp_mat2<{R}, {a}, {b}>(w, ops[{k}]);")
+                elif kind == P.OP_CX:
+                    th = "true" if a > b else "false"
+                    out.append(f"    ap_cx_t<{R}, 4, {hi}, {lo}, {th}>(v); ap_cx_t<{R}, 4, {hi}, {lo}, {th}>(w);")
+                elif kind == P.OP_CZ:
+                    out.append(f"    ap_cz<{R}, 4, {hi}, {lo}>(v); ap_cz<{R}, 4, {hi}, {lo}>(w);")
+                elif kind == P.OP_ROT1:
+                    out.append(f"    spd_rot1<{R}, {tb}, {a}, {pauli}, {g}>(v, w, ops[{k}], trig, gacc, {ng});")
+                elif kind == P.OP_ROT2:
+                    out.append(f"    spd_rot2<{R}, {tb}, {a}, {b}, {pauli}, {g}>(v, w, ops[{k}], trig, gacc, {ng});")
+                else:
+                    raise ValueError(kind)
+            else:
+                if kind == P.OP_MAT1:
+                    out.append(f"    sp_mat1<{R}, {a}, {rb}>(v, ops[{k}]);")
+                elif kind == P.OP_MAT2:
+                    out.append(f"    sp_mat2<{R}, {a}, {b}, {rb}>(v, ops[{k}]);")
+                elif kind == P.OP_CX:
+                    th = "true" if a > b else "false"
+                    out.append(f"    ap_cx_t<{R}, {rb}, {hi}, {lo}, {th}>(v);")
+                elif kind == P.OP_CZ:
+                    out.append(f"    ap_cz<{R}, {rb}, {hi}, {lo}>(v);")
+                elif kind == P.OP_ROT1:
+                    out.append(f"    sp_rot1<{R}, {a}, {pauli}, {rb}>(v, ops[{k}], trig);")
+                elif kind == P.OP_ROT2:
+                    out.append(f"    sp_rot2<{R}, {a}, {b}, {pauli}, {rb}>(v, ops[{k}], trig);")
+                else:
+                    raise ValueError(kind)
+    if dual:
+        out.append(f"    sp_store2<{R}, {tb}>(psi, lam, base, st[{ns - 1}], v, w);")
+        if ng:
+            out.append(f"    sp_grad_flush_wg<{tb}>(gacc, {ng}, gpart, wg);")
+    else:
+        out.append(f"    sp_store<{R}, {tb}, {rb}>(psi, base, st[{ns - 1}], v);")
+    return out
+
+
+def _pass_kernel(R, p, stages, ops, dual, name):
+    """A kernel for one pass of one program: tile index = blockIdx.x."""
+    tb = int(p['tb'])
+    lam = f"cpx<{R}> *__restrict__ lam, " if dual else ""
+    gp = ", double *__restrict__ gpart" if dual else ""
+    head = [f'extern "C" __global__ void __launch_bounds__({1 << tb}) '
+            f"{name}(cpx<{R}> *__restrict__ psi, {lam}int n, const svh_pass_t *__restrict__ pass, "
+            f"const svh_stage_t *__restrict__ st, const svh_op_t *__restrict__ ops, "
+            f"const cpx<{R}> *__restrict__ trig{gp}) {{",
+            "    const uint64_t wg = blockIdx.x;"]
+    return head + _body(R, p, stages, ops, dual) + ["}", ""]
+
+
+def batch_body(blob, pi, dtype_code, dual):
+    """The pass-independent text of pass ``pi``'s batch kernel (equal texts run
+    in one launch for every circuit whose pass has that structure)."""
+    h, passes, stages, ops = _decode(blob)
+    R = "float" if dtype_code == 0 else "double"
+    return "\n".join(_body(R, passes[pi], stages, ops, dual))
+
+
+def batch_kernel(body, dtype_code, dual, tb, name):
+    """Kernel ``name`` over a batch item table (svh_batch_item_t): workgroup b
+    finds its item (last wg_begin <= b) and runs ``body`` on that circuit."""
+    R = "float" if dtype_code == 0 else "double"
+    lam = (f"    cpx<{R}> *lam = reinterpret_cast<cpx<{R}> *>(it.lam);\n"
+           f"    double *gpart = reinterpret_cast<double *>(it.gpart);\n") if dual else ""
+    return (f'extern "C" __global__ void __launch_bounds__({1 << tb}) '
+            f"{name}(const svh_batch_item_t *__restrict__ items, int nitems) {{\n"
+            f"    const svh_batch_item_t &it = items[sp_find_item(items, nitems)];\n"
+            f"    cpx<{R}> *psi = reinterpret_cast<cpx<{R}> *>(it.psi);\n" + lam +
+            f"    const int n = it.n;\n"
+            f"    const svh_pass_t *pass = reinterpret_cast<const svh_pass_t *>(it.pass);\n"
+            f"    const svh_stage_t *st = reinterpret_cast<const svh_stage_t *>(it.stages) + pass->stage_begin;\n"
+            f"    const svh_op_t *ops = reinterpret_cast<const svh_op_t *>(it.ops) + st[0].op_begin;\n"
+            f"    const cpx<{R}> *trig = reinterpret_cast<const cpx<{R}> *>(it.trig);\n"
+            f"    const uint64_t wg = (uint64_t)((int64_t)blockIdx.x - it.wg_begin);\n"
+            + body + "\n}\n")
+
+
 def adjoint_source(blob, dtype_code, only=None):
     """Adjoint (ψ, λ) kernels ``svh_spd_<p>`` for the tile passes of an adjoint
     program: gradient ops accumulate 2·Im⟨λ|P|ψ⟩ before applying U† to both."""
@@ -59,53 +173,7 @@ def adjoint_source(blob, dtype_code, only=None):
     for pi, p in enumerate(passes):
         if not eligible(p, dtype_code, True) or (only is not None and pi not in only):
             continue
-        tb = int(p['tb'])
-        s0, ns = int(p['stage_begin']), int(p['n_stages'])
-        op0 = int(stages[s0]['op_begin'])
-        ng = int(p['n_grad'])
-        out.append(f'extern "C" __global__ void __launch_bounds__({1 << tb}) '
-                   f"svh_spd_{pi}(cpx<{R}> *__restrict__ psi, cpx<{R}> *__restrict__ lam, int n, "
-                   f"const svh_pass_t *__restrict__ pass, const svh_stage_t *__restrict__ st, "
-                   f"const svh_op_t *__restrict__ ops, const cpx<{R}> *__restrict__ trig, "
-                   f"double *__restrict__ gpart) {{")
-        out.append("    extern __shared__ unsigned char smem_raw[];")
-        out.append(f"    cpx<{R}> *lds = reinterpret_cast<cpx<{R}> *>(smem_raw);")
-        out.append(f"    cpx<{R}> *lds2 = lds + {2 ** int(p['L'])};")
-        out.append(f"    double *gacc = reinterpret_cast<double *>(lds2 + {2 ** int(p['L'])});")
-        out.append(f"    cpx<{R}> v[16], w[16];")
-        out.append(f"    const uint64_t base = sp_base<{R}, {tb}>(pass, n);")
-        out.append(f"    sp_load2<{R}, {tb}>(psi, lam, base, st[0], v, w);")
-        for s in range(ns):
-            S = stages[s0 + s]
-            if s > 0:
-                first = "true" if s == 1 else "false"
-                out.append(f"    sp_exchange2<{R}, {first}>(lds, lds2, st[{s - 1}], st[{s}], v, w);")
-            for oi in range(int(S['op_begin']), int(S['op_end'])):
-                o = ops[oi]
-                k = oi - op0
-                kind, a, b, pauli = int(o['kind']), int(o['a']), int(o['b']), int(o['pauli'])
-                g = "true" if int(o['flags']) & P.OPF_GRAD else "false"
-                if kind == P.OP_MAT1:
-                    out.append(f"    sp_mat1<{R}, {a}>(v, ops[{k}]); sp_mat1<{R}, {a}>(w, ops[{k}]);")
-                elif kind == P.OP_MAT2:
-                    out.append(f"    sp_mat2<{R}, {a}, {b}>(v, ops[{k}]); sp_mat2<{R}, {a}, {b}>(w, ops[{k}]);")
-                elif kind == P.OP_CX:
-                    hi, lo = max(a, b), min(a, b)
-                    th = "true" if a > b else "false"
-                    out.append(f"    ap_cx_t<{R}, 4, {hi}, {lo}, {th}>(v); ap_cx_t<{R}, 4, {hi}, {lo}, {th}>(w);")
-                elif kind == P.OP_CZ:
-                    out.append(f"    ap_cz<{R}, 4, {max(a, b)}, {min(a, b)}>(v); ap_cz<{R}, 4, {max(a, b)}, {min(a, b)}>(w);")
-                elif kind == P.OP_ROT1:
-                    out.append(f"    spd_rot1<{R}, {tb}, {a}, {pauli}, {g}>(v, w, ops[{k}], trig, gacc, {ng});")
-                elif kind == P.OP_ROT2:
-                    out.append(f"    spd_rot2<{R}, {tb}, {a}, {b}, {pauli}, {g}>(v, w, ops[{k}], trig, gacc, {ng});")
-                else:
-                    raise ValueError(kind)
-        out.append(f"    sp_store2<{R}, {tb}>(psi, lam, base, st[{ns - 1}], v, w);")
-        if ng:
-            out.append(f"    sp_grad_flush<{tb}>(gacc, {ng}, gpart);")
-        out.append("}")
-        out.append("")
+        out += _pass_kernel(R, p, stages, ops, True, f"svh_spd_{pi}")
     return "\n".join(out)
 
 
@@ -118,47 +186,7 @@ def program_source(blob, dtype_code, only=None):
     for pi, p in enumerate(passes):
         if not eligible(p, dtype_code) or (only is not None and pi not in only):
             continue
-        tb = int(p['tb'])
-        rb = int(p['rb'])
-        s0, ns = int(p['stage_begin']), int(p['n_stages'])
-        op0 = int(stages[s0]['op_begin'])
-        out.append(f'extern "C" __global__ void __launch_bounds__({1 << tb}) '
-                   f"svh_sp_{pi}(cpx<{R}> *__restrict__ psi, int n, const svh_pass_t *__restrict__ pass, "
-                   f"const svh_stage_t *__restrict__ st, const svh_op_t *__restrict__ ops, "
-                   f"const cpx<{R}> *__restrict__ trig) {{")
-        out.append("    extern __shared__ unsigned char smem_raw[];")
-        out.append(f"    cpx<{R}> *lds = reinterpret_cast<cpx<{R}> *>(smem_raw);")
-        out.append(f"    cpx<{R}> v[{1 << rb}];")
-        out.append(f"    const uint64_t base = sp_base<{R}, {tb}, {rb}>(pass, n);")
-        out.append(f"    sp_load<{R}, {tb}, {rb}>(psi, base, st[0], v);")
-        for s in range(ns):
-            S = stages[s0 + s]
-            if s > 0:
-                first = "true" if s == 1 else "false"
-                out.append(f"    sp_exchange<{R}, {first}, {rb}>(lds, st[{s - 1}], st[{s}], v);")
-            for oi in range(int(S['op_begin']), int(S['op_end'])):
-                o = ops[oi]
-                k = oi - op0
-                kind, a, b, pauli = int(o['kind']), int(o['a']), int(o['b']), int(o['pauli'])
-                if kind == P.OP_MAT1:
-                    out.append(f"    sp_mat1<{R}, {a}, {rb}>(v, ops[{k}]);")
-                elif kind == P.OP_MAT2:
-                    out.append(f"    sp_mat2<{R}, {a}, {b}, {rb}>(v, ops[{k}]);")
-                elif kind == P.OP_CX:
-                    hi, lo = max(a, b), min(a, b)
-                    th = "true" if a > b else "false"
-                    out.append(f"    ap_cx_t<{R}, {rb}, {hi}, {lo}, {th}>(v);")
-                elif kind == P.OP_CZ:
-                    out.append(f"    ap_cz<{R}, {rb}, {max(a, b)}, {min(a, b)}>(v);")
-                elif kind == P.OP_ROT1:
-                    out.append(f"    sp_rot1<{R}, {a}, {pauli}, {rb}>(v, ops[{k}], trig);")
-                elif kind == P.OP_ROT2:
-                    out.append(f"    sp_rot2<{R}, {a}, {b}, {pauli}, {rb}>(v, ops[{k}], trig);")
-                else:
-                    raise ValueError(kind)
-        out.append(f"    sp_store<{R}, {tb}, {rb}>(psi, base, st[{ns - 1}], v);")
-        out.append("}")
-        out.append("")
+        out += _pass_kernel(R, p, stages, ops, False, f"svh_sp_{pi}")
     return "\n".join(out)
 
 
@@ -233,4 +261,60 @@ class Specialized:
         try:
             self.close()
         except Exception:
+            pass
+
+
+def compile_batch(kernels, dtype_code, chunks=None):
+    """Code objects for batch kernels: ``kernels`` maps kernel name ->
+    (body, dual, tb).  Compiled in ``chunks`` parallel hipcc runs, cached by
+    content hash like the per-pass kernels."""
+    from concurrent.futures import ThreadPoolExecutor
+    names = sorted(kernels)
+    if not names:
+        return []
+    os.makedirs(CACHE_DIR, exist_ok=True)
+    k = chunks or min(8, os.cpu_count() or 4, len(names))
+    groups = [names[i::k] for i in range(k)]
+    srcs = []
+    for g in groups:
+        if not g:
+            continue
+        text = ['#include "svh_tile_ops.h"', "using namespace svh;", ""]
+        for nm in g:
+            body, dual, tb = kernels[nm]
+            text.append(batch_kernel(body, dtype_code, dual, tb, nm))
+        srcs.append("\n".join(text))
+    with ThreadPoolExecutor(max_workers=len(srcs)) as ex:
+        return list(ex.map(lambda a: _compile_one(a[1], "b%d" % a[0]), enumerate(srcs)))
+
+
+class BatchKernels:
+    """Loaded batch kernels (svh_spb_load); ``index[name]`` is the launch index."""
+
+    def __init__(self, kernels, dtype_code):
+        import ctypes
+
+        from . import _lib
+        lib = _lib.load()
+        self.names = sorted(kernels)
+        self.index = {nm: i for i, nm in enumerate(self.names)}
+        self.paths = compile_batch(kernels, dtype_code)
+        parr = (ctypes.c_char_p * len(self.paths))(*[x.encode() for x in self.paths])
+        narr = (ctypes.c_char_p * len(self.names))(*[x.encode() for x in self.names])
+        h = ctypes.c_void_p()
+        _lib.check(lib.svh_spb_load(parr, len(self.paths), narr, len(self.names), ctypes.byref(h)), "svh_spb_load")
+        self.handle = h
+
+    def close(self):
+        import ctypes
+
+        from . import _lib
+        if getattr(self, "handle", None) and self.handle.value:
+            _lib.load().svh_sp_free(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # pragma: no cover - interpreter shutdown
             pass

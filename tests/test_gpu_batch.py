@@ -23,12 +23,13 @@ def make_batch(nc, seed, nmin=10, nmax=14, lmax=5):
     return circs, terms, thetas
 
 
+@pytest.mark.parametrize("spz", [False, True])
 @pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
-def test_batch_matches_single_circuit_adjoint(gpu, dtype):
+def test_batch_matches_single_circuit_adjoint(gpu, dtype, spz):
     from qaravan_amd.adjoint import AdjointModel
     from qaravan_amd.batch import BatchedAdjoint
     circs, terms, thetas = make_batch(9, 5)
-    b = BatchedAdjoint(circs, terms, [len(t) for t in thetas], device=gpu, dtype=dtype)
+    b = BatchedAdjoint(circs, terms, [len(t) for t in thetas], device=gpu, dtype=dtype, specialize=spz)
     b.set_theta(thetas)
     e, _ = b.run()
     e = e.cpu().numpy()
@@ -44,10 +45,12 @@ def test_batch_matches_single_circuit_adjoint(gpu, dtype):
         assert np.linalg.norm(grads[c] - gc) / np.linalg.norm(gc) < tol
 
 
-def test_batch_vs_oracle_c128(gpu):
+@pytest.mark.parametrize("spz", [False, True])
+def test_batch_vs_oracle_c128(gpu, spz):
     from qaravan_amd.batch import BatchedAdjoint
     circs, terms, thetas = make_batch(4, 11, nmin=10, nmax=11, lmax=2)
-    b = BatchedAdjoint(circs, terms, [len(t) for t in thetas], device=gpu, dtype=torch.complex128)
+    b = BatchedAdjoint(circs, terms, [len(t) for t in thetas], device=gpu, dtype=torch.complex128,
+                       specialize=spz)
     b.set_theta(thetas)
     e, _ = b.run()
     grads = b.gradients()
@@ -64,10 +67,11 @@ def test_batch_vs_oracle_c128(gpu):
         assert np.linalg.norm(grads[c] - gr) / np.linalg.norm(gr) < 1e-10
 
 
-def test_batch_graph_replays_match_eager(gpu):
+@pytest.mark.parametrize("spz", [False, True])
+def test_batch_graph_replays_match_eager(gpu, spz):
     from qaravan_amd.batch import BatchedAdjoint
     circs, terms, thetas = make_batch(7, 23)
-    b = BatchedAdjoint(circs, terms, [len(t) for t in thetas], device=gpu, dtype=torch.complex64)
+    b = BatchedAdjoint(circs, terms, [len(t) for t in thetas], device=gpu, dtype=torch.complex64, specialize=spz)
     b.set_theta(thetas)
     e0, g0 = b.run()
     e0, g0 = e0.clone(), g0.clone()
