@@ -102,8 +102,24 @@ k_marg_reg(const cpx<R> *__restrict__ psi, uint64_t N, KeptBits kb, int k, doubl
     double acc[M];
 #pragma unroll
     for (int m = 0; m < M; ++m) acc[m] = 0.0;
+    // four independent loads in flight per thread per step
     const uint64_t stride = (uint64_t)gridDim.x * 256;
-    for (uint64_t a = (uint64_t)blockIdx.x * 256 + threadIdx.x; a < N; a += stride) {
+    uint64_t a = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    for (; a + 3 * stride < N; a += 4 * stride) {
+        cpx<R> z[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) z[u] = psi[a + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t au = a + u * stride;
+            const double p = abs2(z[u]);
+            int bin = 0;
+            for (int i = 0; i < k; ++i) bin = (bin << 1) | (int)((au >> kb.bit[i]) & 1);
+#pragma unroll
+            for (int m = 0; m < M; ++m) acc[m] += bin == m ? p : 0.0;
+        }
+    }
+    for (; a < N; a += stride) {
         const double p = abs2(psi[a]);
         int bin = 0;
         for (int i = 0; i < k; ++i) bin = (bin << 1) | (int)((a >> kb.bit[i]) & 1);
@@ -122,6 +138,14 @@ k_marg_reg(const cpx<R> *__restrict__ psi, uint64_t N, KeptBits kb, int k, doubl
         if (threadIdx.x == 0) part[(uint64_t)blockIdx.x * M + m] = red[0];
         __syncthreads();
     }
+}
+
+// Every qubit kept in address order (kept[i] = n-1-i): p[m] = |ψ[m]|².
+template <typename R>
+__global__ void __launch_bounds__(256) k_marg_full(const cpx<R> *__restrict__ psi, uint64_t N,
+                                                   double *__restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t a = (uint64_t)blockIdx.x * 256 + threadIdx.x; a < N; a += stride) out[a] = abs2(psi[a]);
 }
 
 // out[m] = Σ_c part[c*M + m], c ascending.
@@ -519,6 +543,18 @@ int svh_marginal_probs(const void *psi, int n, int dtype, const int *kept, int k
     const uint64_t rest_mask = ((1ull << n) - 1) & ~kmask;
     hipStream_t st = (hipStream_t)stream;
     double *part = staged ? (double *)scratch : probs;
+    bool natural = k == n;
+    for (int i = 0; i < k && natural; ++i) natural = kb.bit[i] == n - 1 - i;
+    if (natural && k > 4) {
+        const uint64_t N = 1ull << n;
+        uint64_t g = (N + 255) / 256;
+        if (g > 65536) g = 65536;
+        if (dtype == SVH_C64)
+            k_marg_full<float><<<(unsigned)g, 256, 0, st>>>((const cpx<float> *)psi, N, probs);
+        else
+            k_marg_full<double><<<(unsigned)g, 256, 0, st>>>((const cpx<double> *)psi, N, probs);
+        return checked_launch("svh_marginal_probs");
+    }
     if (p.kind == 0) {
         const unsigned g = (unsigned)p.C;
         const uint64_t N = 1ull << n;
