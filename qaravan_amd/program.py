@@ -18,7 +18,9 @@ the library runs as a handful of fused passes:
    owns L address bits (always the lowest ``low_bits`` ones, so every tile
    load is a run of contiguous amplitudes) and takes every op whose bits fit,
    stopping at an ALU-cost cap.  An op that does not fit blocks its bits for
-   the rest of the pass, so the order of overlapping gates is preserved.
+   the rest of the pass, so the order of overlapping gates is preserved,
+   except between two diagonal ops (CZ, Z-type rotations, diagonal matrices),
+   which commute and may pass each other.
 4. Stages.  Inside a pass, ops are grouped into stages of 4 register bits;
    each stage change is one LDS re-distribution in the kernel.
 
@@ -330,6 +332,36 @@ def _is_barrier(op):
     return op.kind in ('dense', 'norm')
 
 
+def is_diagonal(op):
+    """Diagonal in the computational basis: such ops commute with each other,
+    so a diagonal op may move past another diagonal op on shared bits."""
+    if op.kind == 'cz':
+        return True
+    if op.kind in ('rot1', 'rot2'):
+        return op.pauli == PAULI_CODE['z']
+    if op.kind in ('mat1', 'mat2'):
+        m = np.asarray(op.mat)
+        return not np.any(m - np.diag(np.diag(m)))
+    return False
+
+
+class _Blocker:
+    """Bits held back by ops that were skipped: a later op must wait if it
+    shares a bit with a skipped op, unless both are diagonal (they commute)."""
+
+    def __init__(self):
+        self.diag, self.other = set(), set()
+
+    def blocks(self, op, bits):
+        return bool(bits & self.other) if is_diagonal(op) else bool(bits & (self.other | self.diag))
+
+    def add(self, op, bits):
+        (self.diag if is_diagonal(op) else self.other).update(bits)
+
+    def all(self):
+        return self.diag | self.other
+
+
 def partition(ops, n, cfg):
     """Greedy packing of ops into passes (see module docstring, step 3)."""
     L = cfg.tb + cfg.rb
@@ -350,15 +382,15 @@ def partition(ops, n, cfg):
             rem.pop(0)
             continue
         T = set(low)
-        blocked = set()
+        blocked = _Blocker()
         taken = []
         cost = 0.0
         for i, op in enumerate(rem):
             if _is_barrier(op):
                 break
             b = set(op.bits)
-            if b & blocked:
-                blocked |= b
+            if blocked.blocks(op, b):
+                blocked.add(op, b)
             else:
                 need = b - T
                 c = OP_COST[op.kind]
@@ -367,8 +399,8 @@ def partition(ops, n, cfg):
                     taken.append(i)
                     cost += c
                 else:
-                    blocked |= b
-            if len(blocked) == n or cost + 0.25 > cfg.max_cost:
+                    blocked.add(op, b)
+            if len(blocked.other) == n or cost + 0.25 > cfg.max_cost:
                 break   # nothing later can join this pass
         # complete the tile with the lowest unused bits
         x = 0
@@ -396,29 +428,29 @@ def _stage_ops(p, io_lane_bits=6):
     cur = None
     while pend:
         if cur is not None:
-            took, blocked, keep = [], set(), []
+            took, blocked, keep = [], _Blocker(), []
             for op, lb in pend:
                 s = set(lb)
-                if not (s & blocked) and s <= set(cur.rbits):
+                if not blocked.blocks(op, s) and s <= set(cur.rbits):
                     took.append((op, lb))
                 else:
-                    blocked |= s
+                    blocked.add(op, s)
                     keep.append((op, lb))
             if took:
                 cur.ops.extend(took)
                 pend = keep
                 continue
-        R, blocked = [], set()
+        R, blocked = [], _Blocker()
         for op, lb in pend:
             s = set(lb)
-            if s & blocked:
-                blocked |= s
+            if blocked.blocks(op, s):
+                blocked.add(op, s)
                 continue
             new = [x for x in lb if x not in R]
             if len(R) + len(new) <= RB:
                 R += new
             else:
-                blocked |= s
+                blocked.add(op, s)
             if len(R) == RB:
                 break
         for x in reversed(range(L)):            # fill with high tile bits

@@ -264,17 +264,17 @@ class Specialized:
             pass
 
 
-def compile_batch(kernels, dtype_code, chunks=None):
+def compile_batch(kernels, dtype_code, per_file=8, workers=8):
     """Code objects for batch kernels: ``kernels`` maps kernel name ->
-    (body, dual, tb).  Compiled in ``chunks`` parallel hipcc runs, cached by
-    content hash like the per-pass kernels."""
+    (body, dual, tb).  ``per_file`` kernels per code object (so the cache key
+    of a file does not depend on the machine), ``workers`` hipcc runs at a
+    time; cached by content hash like the per-pass kernels."""
     from concurrent.futures import ThreadPoolExecutor
     names = sorted(kernels)
     if not names:
         return []
     os.makedirs(CACHE_DIR, exist_ok=True)
-    k = chunks or min(8, os.cpu_count() or 4, len(names))
-    groups = [names[i::k] for i in range(k)]
+    groups = [names[i:i + per_file] for i in range(0, len(names), per_file)]
     srcs = []
     for g in groups:
         if not g:
@@ -284,7 +284,7 @@ def compile_batch(kernels, dtype_code, chunks=None):
             body, dual, tb = kernels[nm]
             text.append(batch_kernel(body, dtype_code, dual, tb, nm))
         srcs.append("\n".join(text))
-    with ThreadPoolExecutor(max_workers=len(srcs)) as ex:
+    with ThreadPoolExecutor(max_workers=max(1, min(workers, len(srcs)))) as ex:
         return list(ex.map(lambda a: _compile_one(a[1], "b%d" % a[0]), enumerate(srcs)))
 
 

@@ -205,16 +205,17 @@ def _param_ops(circ, n):
     return P.lower_circuit(circ.build(), n, param_mode=True)
 
 
-@pytest.mark.parametrize("tb", [4, 6])
-def test_param_forward_and_adjoint_programs(tb):
+@pytest.mark.parametrize("tb,layers,cost", [(4, 3, 40), (6, 3, 40), (6, 6, 1e9), (4, 5, 1e9)])
+def test_param_forward_and_adjoint_programs(tb, layers, cost):
     """Forward program with device-θ rotations, and the adjoint program's
-    gradient, interpreted on the CPU, against the oracle's adjoint."""
+    gradient, interpreted on the CPU, against the oracle's adjoint (RZ and CZ
+    are diagonal, so the scheduler may reorder them)."""
     from qaravan_amd.adjoint import ParamProgram
-    n, layers = 10, 3
+    n = 10
     theta = hea_theta(n, layers)
     circ = hea_circuit(n, layers, theta)
     terms = tfim_terms(n)
-    pp = ParamProgram(circ, terms, len(theta), cfg=P.TileConfig(tb=tb, low_bits=6, max_cost=40))
+    pp = ParamProgram(circ, terms, len(theta), cfg=P.TileConfig(tb=tb, low_bits=6, max_cost=cost))
     psi, _ = prog_interp.run(pp.fwd.blob, O.init_state(n), theta=theta)
     ref = O.run(circ.gate_list, n)
     assert np.linalg.norm(psi - ref) < 1e-12
@@ -224,6 +225,54 @@ def test_param_forward_and_adjoint_programs(tb):
     from test_oracle import hea_spec
     _, gref = O.adjoint_gradient(hea_spec(n, layers), theta, n, terms)
     assert np.allclose(grad, gref, atol=1e-10)
+
+
+def _diag_rich_circuit(n, n_gates, rng):
+    """Random gates where most are diagonal (CZ, RZ, diagonal 1q/2q matrices)."""
+    from qaravan_amd import RZ
+    gates = []
+    for _ in range(n_gates):
+        r = rng.random()
+        q = int(rng.integers(n))
+        if r < 0.25:
+            qs = [int(x) for x in rng.choice(n, size=2, replace=False)]
+            gates.append(CZ(qs))
+        elif r < 0.45:
+            gates.append(RZ([q], float(rng.uniform(0, 6))))
+        elif r < 0.6:
+            gates.append(Gate("D", [q], np.diag(np.exp(1j * rng.uniform(0, 6, size=2)))))
+        elif r < 0.72:
+            qs = [int(x) for x in rng.choice(n, size=2, replace=False)]
+            gates.append(Gate("D2", qs, np.diag(np.exp(1j * rng.uniform(0, 6, size=4)))))
+        elif r < 0.86:
+            gates.append(Gate("U", [q], random_unitary(2, rng)))
+        else:
+            i = int(rng.integers(n - 1))
+            gates.append(CNOT([i, i + 1], n))
+    return Circuit(gates, n=n)
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("tb,rb", [(4, 4), (6, 4), (5, 5)])
+def test_diagonal_reordering_matches_oracle(seed, tb, rb):
+    rng = np.random.default_rng(700 + seed)
+    n = 11
+    circ = _diag_rich_circuit(n, 150, rng)
+    cfg = P.TileConfig(tb=tb, low_bits=4, max_cost=1e9, specialize=rb == 5, rb=rb)
+    prog = compile_circ(circ, cfg)
+    got, _ = prog_interp.run(prog.blob, O.init_state(n))
+    assert np.linalg.norm(got - O.run(circ.gate_list, n)) < 1e-12
+
+
+def test_diagonal_ops_cut_hea_passes():
+    """The 20-qubit HEA of the gradient workload: RZ layers and the CZ ladder
+    commute, so far fewer passes are needed than with strict gate order."""
+    from qaravan_amd.adjoint import ParamProgram
+    n, layers = 20, 16
+    theta = hea_theta(n, layers)
+    pp = ParamProgram(hea_circuit(n, layers, theta), tfim_terms(n), len(theta),
+                      cfg=P.TileConfig(tb=8, low_bits=4, max_cost=1e9))
+    assert pp.fwd.n_tile_passes() <= 24 and pp.adj.n_tile_passes() <= 24
 
 
 def test_stage_tables_cover_each_tile_once():
