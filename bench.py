@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--profile-forward", action="store_true",
                     help="only the timed forward leg (for rocprofv3 runs)")
     ap.add_argument("--profile-blocks", action="store_true", help="only the MFMA block leg")
+    ap.add_argument("--profile-adjoint", action="store_true", help="only 20 adjoint gradients (no torch tapes)")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only a 2 GiB device copy (calibrates the HBM byte counters)")
     ap.add_argument("--tape-host", action="store_true",
@@ -205,6 +206,30 @@ def bench_adjoint(dev, iters=10, tape_host=False):
         torch.cuda.synchronize(dev)
         out.update(torch_autograd_host_theta_grads_per_s=1.0 / (time.perf_counter() - t0))
     return out
+
+
+def profile_adjoint(dev, reps=20):
+    """The 20-qubit gradient alone, for kernel traces."""
+    import torch
+
+    from qaravan_amd.adjoint import AdjointModel
+    from qaravan_amd.autotune import best_adjoint_config
+    from qaravan_amd.workloads import hea_circuit, hea_theta, tfim_terms
+    n, layers = 20, 16
+    theta = hea_theta(n, layers)
+    cfg = best_adjoint_config(n, torch.complex64)
+    model = AdjointModel(hea_circuit(n, layers, theta), tfim_terms(n), len(theta), device=dev,
+                         dtype=torch.complex64, cfg=cfg)
+    th = torch.tensor(theta, device=dev)
+    for _ in range(2):
+        model.energy_and_grad(th)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        model.energy_and_grad(th)
+    torch.cuda.synchronize(dev)
+    return dict(ms=(time.perf_counter() - t0) / reps * 1e3, fwd_passes=model.pp.fwd.n_tile_passes(),
+                adj_passes=model.pp.adj.n_tile_passes())
 
 
 def bench_blocks(dev, n=30, layers=24, reps=3):
@@ -483,6 +508,9 @@ def main():
         return
     if args.profile_blocks:
         print(json.dumps({"profile": "blocks", **bench_blocks(dev, reps=1)}))
+        return
+    if args.profile_adjoint:
+        print(json.dumps({"profile": "adjoint", **profile_adjoint(dev)}))
         return
     fwd = bench_forward(args, dev)
     if args.profile_forward:
