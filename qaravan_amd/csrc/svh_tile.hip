@@ -274,22 +274,47 @@ __global__ void k_trig(const double *__restrict__ theta, int64_t np, cpx<R> *__r
     }
 }
 
-// Sum the per-workgroup partials of one pass (fixed order) into grad[param].
+// Gradient partials of up to kGradChunk passes, reduced after the sweep: slot
+// s of pass entry e has its nb rows at part + off[e] + row·ng + (s - slot0).
+constexpr int kGradChunk = 128;
+struct GradChunk {
+    int64_t off[kGradChunk];
+    int32_t nb[kGradChunk];
+    int32_t ng[kGradChunk];
+    int32_t slot0[kGradChunk];
+    int32_t npass;
+    int32_t pad;
+};
+
+// slotsum[s] = Σ_rows partials of slot s (fixed-order tree); grid = the
+// chunk's slots, contiguous from chunk.slot0[0].
 __global__ void __launch_bounds__(256)
-k_grad_reduce(const double *__restrict__ gpart, int nblocks, int ng, const int32_t *__restrict__ slot_param,
-              double *__restrict__ grad) {
-    const int sl = blockIdx.x;
+k_grad_slots(const double *__restrict__ part, GradChunk C, double *__restrict__ slotsum) {
+    const int s = C.slot0[0] + (int)blockIdx.x;
+    int e = 0;
+    while (e + 1 < C.npass && C.slot0[e + 1] <= s) ++e;
+    const int ng = C.ng[e], l = s - C.slot0[e];
+    const double *p = part + C.off[e];
     double acc = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) acc += gpart[(uint64_t)b * ng + sl];
+    for (int b = threadIdx.x; b < C.nb[e]; b += blockDim.x) acc += p[(uint64_t)b * ng + l];
     __shared__ double red[256];
     red[threadIdx.x] = acc;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(&grad[slot_param[sl]], red[0]);
+    if (threadIdx.x == 0) slotsum[s] = red[0];
 }
+
+// grad[param(s)] += slotsum[s] for s ascending (one thread: a fixed order even
+// when several gates share a parameter).
+__global__ void k_grad_scatter(const double *__restrict__ slotsum, int nslots,
+                               const int32_t *__restrict__ slot_param, double *__restrict__ grad) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int s = 0; s < nslots; ++s) grad[slot_param[s]] += slotsum[s];
+}
+
 
 // ------------------------------------------------ host launch
 constexpr size_t kMaxLds = 160 * 1024;
@@ -500,6 +525,19 @@ static size_t trig_bytes(const svh_prog_header_t *h, int dtype) {
     return ((size_t)h->n_params * es + 255) & ~(size_t)255;
 }
 
+// Doubles of gradient partials of all tile passes of an adjoint program (each
+// pass has its own region: nb·ng rows), plus one sum per gradient slot.
+static size_t grad_doubles(const svh_prog_header_t *h) {
+    const char *b = reinterpret_cast<const char *>(h);
+    const auto *passes = reinterpret_cast<const svh_pass_t *>(b + h->off_passes);
+    size_t d = (size_t)h->n_grad_slots;
+    for (int p = 0; p < h->n_passes; ++p) {
+        const svh_pass_t &P = passes[p];
+        if (P.kind == SVH_PASS_TILE && P.n_grad > 0) d += ((size_t)1 << (h->n_qubits - P.L)) * P.n_grad;
+    }
+    return d;
+}
+
 static size_t scratch_need(const svh_prog_header_t *h, int dtype, bool dual) {
     const char *b = reinterpret_cast<const char *>(h);
     const auto *passes = reinterpret_cast<const svh_pass_t *>(b + h->off_passes);
@@ -509,9 +547,11 @@ static size_t scratch_need(const svh_prog_header_t *h, int dtype, bool dual) {
         const svh_pass_t &P = passes[p];
         size_t s = (kReduceBlocks + 8) * sizeof(double);
         if (P.kind == SVH_PASS_DENSE) s += es * ((size_t)1 << (2 * P.L));
-        if (P.kind == SVH_PASS_TILE && dual && P.n_grad > 0)
-            s = sizeof(double) * (((size_t)1 << (h->n_qubits - P.L)) * P.n_grad);
         if (s > need) need = s;
+    }
+    if (dual) {
+        const size_t g = sizeof(double) * grad_doubles(h);
+        if (g > need) need = g;
     }
     return trig_bytes(h, dtype) + ((need + 255) & ~(size_t)255);
 }
@@ -582,33 +622,47 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
 
     const auto *hstage = reinterpret_cast<const svh_stage_t *>(hb + h->off_stages);
     if (sp && (int)sp->fn.size() != h->n_passes) return fail(-8, "specialised kernels do not match the program");
+    // adjoint: every pass writes its gradient partials to its own region of sd;
+    // they are reduced once after the sweep (two launches per chunk of passes)
+    size_t goff = 0;
+    GradChunk chunk{};
+    double *slotsum = sd + (grad_doubles(h) - (size_t)h->n_grad_slots);
+    auto flush_chunk = [&]() -> int {
+        if (chunk.npass == 0) return 0;
+        const int last = chunk.npass - 1;
+        const int nslots = chunk.slot0[last] + chunk.ng[last] - chunk.slot0[0];
+        k_grad_slots<<<nslots, 256, 0, st>>>(sd, chunk, slotsum);
+        chunk = GradChunk{};
+        return checked_launch("k_grad_slots");
+    };
     for (int p = 0; p < h->n_passes; ++p) {
         const svh_pass_t &P = hpass[p];
         int e = 0;
+        double *gp = sd;
+        if (dual && P.kind == SVH_PASS_TILE && P.n_grad > 0) {
+            gp = sd + goff;
+            if (chunk.npass == kGradChunk && (e = flush_chunk())) return e;
+            chunk.off[chunk.npass] = (int64_t)goff;
+            chunk.nb[chunk.npass] = (int32_t)(1ull << (n - P.L));
+            chunk.ng[chunk.npass] = P.n_grad;
+            chunk.slot0[chunk.npass] = P.grad_begin;
+            chunk.npass++;
+            goff += ((size_t)1 << (n - P.L)) * P.n_grad;
+        }
         if (P.kind == SVH_PASS_TILE && sp && sp->fn[p]) {
             e = launch_sp(sp->fn[p], psi, dual ? lam : nullptr, n, dtype, P, dpass + p, dstage, dop, hstage, trig,
-                          sd, st);
-            if (!e && dual && P.n_grad > 0) {
-                const int nb = (int)(1ull << (n - P.L));
-                k_grad_reduce<<<P.n_grad, 256, 0, st>>>(sd, nb, P.n_grad, dgrad + P.grad_begin, grad_dev);
-                e = checked_launch("k_grad_reduce");
-            }
+                          gp, st);
         } else if (P.kind == SVH_PASS_TILE) {
             if (dtype == SVH_C64)
                 e = dual ? launch_tile<float, true>((cpx<float> *)psi, (cpx<float> *)lam, n, P, dpass + p, dstage,
-                                                    dop, (const cpx<float> *)trig, sd, st)
+                                                    dop, (const cpx<float> *)trig, gp, st)
                          : launch_tile<float, false>((cpx<float> *)psi, nullptr, n, P, dpass + p, dstage, dop,
                                                      (const cpx<float> *)trig, sd, st);
             else
                 e = dual ? launch_tile<double, true>((cpx<double> *)psi, (cpx<double> *)lam, n, P, dpass + p,
-                                                     dstage, dop, (const cpx<double> *)trig, sd, st)
+                                                     dstage, dop, (const cpx<double> *)trig, gp, st)
                          : launch_tile<double, false>((cpx<double> *)psi, nullptr, n, P, dpass + p, dstage, dop,
                                                       (const cpx<double> *)trig, sd, st);
-            if (!e && dual && P.n_grad > 0) {
-                const int nb = (int)(1ull << (n - P.L));
-                k_grad_reduce<<<P.n_grad, 256, 0, st>>>(sd, nb, P.n_grad, dgrad + P.grad_begin, grad_dev);
-                e = checked_launch("k_grad_reduce");
-            }
         } else if (P.kind == SVH_PASS_DENSE) {
             if (dual) return fail(-8, "dense passes are not differentiable in the adjoint sweep");
             const int K = 1 << P.L;
@@ -633,6 +687,11 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
             if (!e) e = svh_scale_by_inv_norm(psi, n, dtype, sd + kReduceBlocks, stream);
         }
         if (e) return e;
+    }
+    if (dual && h->n_grad_slots > 0) {
+        if (int e = flush_chunk()) return e;
+        k_grad_scatter<<<1, 64, 0, st>>>(slotsum, h->n_grad_slots, dgrad, grad_dev);
+        return checked_launch("k_grad_scatter");
     }
     return 0;
 }
