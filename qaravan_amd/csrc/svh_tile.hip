@@ -286,10 +286,13 @@ struct GradChunk {
     int32_t pad;
 };
 
-// slotsum[s] = Σ_rows partials of slot s (fixed-order tree); grid = the
-// chunk's slots, contiguous from chunk.slot0[0].
+// Σ_rows partials of slot s (fixed-order tree); grid = the chunk's slots,
+// contiguous from chunk.slot0[0].  With grad != nullptr every slot has its own
+// parameter and the sum is added to grad[slot_param[s]] directly; otherwise it
+// goes to slotsum[s] for k_grad_scatter.
 __global__ void __launch_bounds__(256)
-k_grad_slots(const double *__restrict__ part, GradChunk C, double *__restrict__ slotsum) {
+k_grad_slots(const double *__restrict__ part, GradChunk C, double *__restrict__ slotsum,
+             const int32_t *__restrict__ slot_param, double *__restrict__ grad) {
     const int s = C.slot0[0] + (int)blockIdx.x;
     int e = 0;
     while (e + 1 < C.npass && C.slot0[e + 1] <= s) ++e;
@@ -304,7 +307,10 @@ k_grad_slots(const double *__restrict__ part, GradChunk C, double *__restrict__ 
         if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) slotsum[s] = red[0];
+    if (threadIdx.x == 0) {
+        if (grad) grad[slot_param[s]] += red[0];
+        else slotsum[s] = red[0];
+    }
 }
 
 // grad[param(s)] += slotsum[s] for s ascending (one thread: a fixed order even
@@ -627,11 +633,21 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
     size_t goff = 0;
     GradChunk chunk{};
     double *slotsum = sd + (grad_doubles(h) - (size_t)h->n_grad_slots);
+    bool distinct = true;  // every gradient slot has its own parameter
+    if (dual) {
+        const auto *hslot = reinterpret_cast<const int32_t *>(hb + h->off_grad);
+        std::vector<char> seen((size_t)h->n_params, 0);
+        for (int sl = 0; sl < h->n_grad_slots && distinct; ++sl) {
+            const int q = hslot[sl];
+            if (q < 0 || q >= h->n_params || seen[q]) distinct = false;
+            else seen[q] = 1;
+        }
+    }
     auto flush_chunk = [&]() -> int {
         if (chunk.npass == 0) return 0;
         const int last = chunk.npass - 1;
         const int nslots = chunk.slot0[last] + chunk.ng[last] - chunk.slot0[0];
-        k_grad_slots<<<nslots, 256, 0, st>>>(sd, chunk, slotsum);
+        k_grad_slots<<<nslots, 256, 0, st>>>(sd, chunk, slotsum, dgrad, distinct ? grad_dev : nullptr);
         chunk = GradChunk{};
         return checked_launch("k_grad_slots");
     };
@@ -690,6 +706,7 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
     }
     if (dual && h->n_grad_slots > 0) {
         if (int e = flush_chunk()) return e;
+        if (distinct) return 0;
         k_grad_scatter<<<1, 64, 0, st>>>(slotsum, h->n_grad_slots, dgrad, grad_dev);
         return checked_launch("k_grad_scatter");
     }
