@@ -27,11 +27,14 @@ from .statevector_sim import DeviceScratch, inner, run_program
 
 
 def _adjoint_cfg(n, dtype, cfg):
-    """The adjoint pass holds ψ and λ: cap the workgroup size so both fit in
-    registers (c64: ≤ 512 threads, c128: ≤ 256)."""
-    tb_max = 9 if dtype == torch.complex64 else 8
-    tb = min(cfg.tb, tb_max, n - 4)
-    return replace(cfg, tb=tb, prefetch=False)
+    """The adjoint pass holds ψ and λ: cap the tile so both fit in LDS and the
+    workgroup so the op-interpreting kernel fits in registers (c64: ≤ 512
+    threads, c128: ≤ 256; one more thread bit with 3 register bits).  Five
+    register bits are forward-only."""
+    rb = min(cfg.rb, 4)
+    tb_max = (9 if dtype == torch.complex64 else 8) + (4 - rb)
+    tb = min(cfg.tb, tb_max, n - rb)
+    return replace(cfg, tb=tb, rb=rb, prefetch=False)
 
 
 class ParamProgram:
@@ -55,7 +58,7 @@ class ParamProgram:
                 raise NotImplementedError("gates on > 2 qubits and resets are not differentiable here")
             if op.kind in ('rot1', 'rot2') and not 0 <= op.param < self.n_params:
                 raise ValueError(f"param index {op.param} outside [0, {self.n_params})")
-        self.fwd = compile_ops(ops, self.n, replace(cfg, tb=min(cfg.tb, self.n - 4), prefetch=False),
+        self.fwd = compile_ops(ops, self.n, replace(cfg, tb=min(cfg.tb, self.n - cfg.rb), prefetch=False),
                                n_params=self.n_params)
         back = []
         for op in reversed(ops):

@@ -121,8 +121,9 @@ def best_adjoint_config(n, dtype):
         cfg = ADJOINT_DEFAULT[name]
     else:
         cfg = TileConfig(tb=e["tb"], low_bits=e["low_bits"], max_cost=e["max_cost"],
-                         specialize=e.get("specialize", False), io_lane_bits=e.get("io_lane_bits", 6))
-    return replace(cfg, tb=min(cfg.tb, n - 4))
+                         specialize=e.get("specialize", False), io_lane_bits=e.get("io_lane_bits", 6),
+                         rb=e.get("rb", 4))
+    return replace(cfg, tb=min(cfg.tb, n - cfg.rb))
 
 
 def tune_adjoint(n, layers, dtype, device, space=None, repeats=5, log=print, out_path=None):
@@ -137,11 +138,11 @@ def tune_adjoint(n, layers, dtype, device, space=None, repeats=5, log=print, out
     circ = hea_circuit(n, layers, theta)
     th = torch.tensor(theta, device=device)
     results = []
-    for tb, mc, spz, io in itertools.product(space["tb"], space["max_cost"], space["specialize"],
-                                             space.get("io_lane_bits", [6])):
-        if tb + 4 > n:
+    for tb, mc, spz, io, rb in itertools.product(space["tb"], space["max_cost"], space["specialize"],
+                                                 space.get("io_lane_bits", [6]), space.get("rb", [4])):
+        if tb + rb > n or (rb != 4 and not spz):
             continue
-        cfg = TileConfig(tb=tb, low_bits=4, max_cost=mc, specialize=spz, io_lane_bits=io)
+        cfg = TileConfig(tb=tb, low_bits=4, max_cost=mc, specialize=spz, io_lane_bits=io, rb=rb)
         m = AdjointModel(circ, tfim_terms(n), len(theta), device=device, dtype=dtype, cfg=cfg)
         for _ in range(2):
             m.energy_and_grad(th)
@@ -152,14 +153,15 @@ def tune_adjoint(n, layers, dtype, device, space=None, repeats=5, log=print, out
         torch.cuda.synchronize(device)
         t = (time.perf_counter() - t0) / repeats
         results.append((t, cfg))
-        log(f"[tune-adjoint] n={n} tb={tb} cap={mc:g} specialize={spz} io={io}: {t * 1e3:.3f} ms")
+        log(f"[tune-adjoint] n={n} tb={tb} rb={rb} cap={mc:g} specialize={spz} io={io}: {t * 1e3:.3f} ms")
     results.sort(key=lambda x: x[0])
     t, cfg = results[0]
     cache = _load()
     cache[f"adjoint:{str(dtype).replace('torch.', '')}:{n}"] = dict(tb=cfg.tb, low_bits=cfg.low_bits,
                                                                      max_cost=cfg.max_cost,
                                                                      specialize=cfg.specialize,
-                                                                     io_lane_bits=cfg.io_lane_bits, ms=t * 1e3)
+                                                                     io_lane_bits=cfg.io_lane_bits, rb=cfg.rb,
+                                                                     ms=t * 1e3)
     with open(out_path or CACHE, "w") as f:
         json.dump(cache, f, indent=1, sort_keys=True)
     return cfg, results

@@ -433,8 +433,8 @@ static int check_program(const void *prog_host, int n, const svh_prog_header_t *
     for (int p = 0; p < h->n_passes; ++p) {
         const svh_pass_t &P = passes[p];
         if (P.kind == SVH_PASS_TILE) {
-            if (P.rb < 4 || P.rb > SVH_MAX_REG_BITS)
-                return fail(-8, "tile pass " + std::to_string(p) + ": register bits must be 4 or 5");
+            if (P.rb < 3 || P.rb > SVH_MAX_REG_BITS)
+                return fail(-8, "tile pass " + std::to_string(p) + ": register bits must be 3, 4 or 5");
             if (P.L != P.tb + P.rb || P.L > h->n_qubits || P.L > SVH_MAX_TILE_BITS)
                 return fail(-8, "tile pass " + std::to_string(p) + ": bad L");
             if (P.n_stages < 1 || P.stage_begin < 0 || P.stage_begin + P.n_stages > h->n_stages)

@@ -398,46 +398,48 @@ __device__ __forceinline__ void sp_rot2(cpx<R> (&v)[1 << RB], const svh_op_t &op
 }
 
 // ---- adjoint (ψ and λ together) pieces
-template <typename R, int TB>
+template <typename R, int TB, int RB = 4>
 __device__ __forceinline__ void sp_load2(const cpx<R> *__restrict__ psi, const cpx<R> *__restrict__ lam,
-                                         uint64_t base, const svh_stage_t &S, cpx<R> (&v)[16], cpx<R> (&w)[16]) {
+                                         uint64_t base, const svh_stage_t &S, cpx<R> (&v)[1 << RB],
+                                         cpx<R> (&w)[1 << RB]) {
     const uint64_t t = base | thr_global<TB>(S, threadIdx.x);
     const cpx<R> *p = psi + t;
     const cpx<R> *q = lam + t;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
+    for (int j = 0; j < (1 << RB); ++j) {
         v[j] = p[S.goff[j]];
         w[j] = q[S.goff[j]];
     }
 }
 
-template <typename R, int TB>
+template <typename R, int TB, int RB = 4>
 __device__ __forceinline__ void sp_store2(cpx<R> *__restrict__ psi, cpx<R> *__restrict__ lam, uint64_t base,
-                                          const svh_stage_t &S, const cpx<R> (&v)[16], const cpx<R> (&w)[16]) {
+                                          const svh_stage_t &S, const cpx<R> (&v)[1 << RB],
+                                          const cpx<R> (&w)[1 << RB]) {
     const uint64_t t = base | thr_global<TB>(S, threadIdx.x);
     cpx<R> *p = psi + t;
     cpx<R> *q = lam + t;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
+    for (int j = 0; j < (1 << RB); ++j) {
         p[S.goff[j]] = v[j];
         q[S.goff[j]] = w[j];
     }
 }
 
-template <typename R, bool FIRST>
+template <typename R, bool FIRST, int RB = 4>
 __device__ __forceinline__ void sp_exchange2(cpx<R> *lds, cpx<R> *lds2, const svh_stage_t &P,
-                                             const svh_stage_t &S, cpx<R> (&v)[16], cpx<R> (&w)[16]) {
-    const int tp = lds_swz(thr_local(P, threadIdx.x));
+                                             const svh_stage_t &S, cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB]) {
+    const int tp = lds_swz(thr_local<RB>(P, threadIdx.x));
     if (!FIRST) __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
+    for (int j = 0; j < (1 << RB); ++j) {
         lds[tp ^ P.lsw[j]] = v[j];
         lds2[tp ^ P.lsw[j]] = w[j];
     }
     __syncthreads();
-    const int tn = lds_swz(thr_local(S, threadIdx.x));
+    const int tn = lds_swz(thr_local<RB>(S, threadIdx.x));
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
+    for (int j = 0; j < (1 << RB); ++j) {
         v[j] = lds[tn ^ S.lsw[j]];
         w[j] = lds2[tn ^ S.lsw[j]];
     }
@@ -450,26 +452,26 @@ __device__ __forceinline__ void sp_grad_store(double g, double *gacc, int ng, in
     if ((threadIdx.x & 63) == 0) gacc[(threadIdx.x >> 6) * ng + slot] = 2.0 * g;
 }
 
-template <typename R, int TB, int A, int PAULI, bool GRAD>
-__device__ __forceinline__ void spd_rot1(cpx<R> (&v)[16], cpx<R> (&w)[16], const svh_op_t &op,
+template <typename R, int TB, int A, int PAULI, bool GRAD, int RB = 4>
+__device__ __forceinline__ void spd_rot1(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t &op,
                                          const cpx<R> *trig, double *gacc, int ng) {
-    if constexpr (GRAD) sp_grad_store<((1 << TB) < 64 ? (1 << TB) : 64)>(gr_rot1<R, 4, A>(v, w, PAULI), gacc, ng,
+    if constexpr (GRAD) sp_grad_store<((1 << TB) < 64 ? (1 << TB) : 64)>(gr_rot1<R, RB, A>(v, w, PAULI), gacc, ng,
                                                                        op.grad_slot);
     R cs, sn;
     op_sincos<R>(trig, op, cs, sn);
-    ap_rot1<R, 4, A>(w, PAULI, cs, sn);
-    ap_rot1<R, 4, A>(v, PAULI, cs, sn);
+    ap_rot1<R, RB, A>(w, PAULI, cs, sn);
+    ap_rot1<R, RB, A>(v, PAULI, cs, sn);
 }
 
-template <typename R, int TB, int H, int L, int PAULI, bool GRAD>
-__device__ __forceinline__ void spd_rot2(cpx<R> (&v)[16], cpx<R> (&w)[16], const svh_op_t &op,
+template <typename R, int TB, int H, int L, int PAULI, bool GRAD, int RB = 4>
+__device__ __forceinline__ void spd_rot2(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t &op,
                                          const cpx<R> *trig, double *gacc, int ng) {
-    if constexpr (GRAD) sp_grad_store<((1 << TB) < 64 ? (1 << TB) : 64)>(gr_rot2<R, 4, H, L>(v, w, PAULI), gacc,
+    if constexpr (GRAD) sp_grad_store<((1 << TB) < 64 ? (1 << TB) : 64)>(gr_rot2<R, RB, H, L>(v, w, PAULI), gacc,
                                                                        ng, op.grad_slot);
     R cs, sn;
     op_sincos<R>(trig, op, cs, sn);
-    ap_rot2<R, 4, H, L>(w, PAULI, cs, sn);
-    ap_rot2<R, 4, H, L>(v, PAULI, cs, sn);
+    ap_rot2<R, RB, H, L>(w, PAULI, cs, sn);
+    ap_rot2<R, RB, H, L>(v, PAULI, cs, sn);
 }
 
 // Sum the waves' gradient slots of this workgroup into its partial row wg.

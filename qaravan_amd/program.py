@@ -68,7 +68,7 @@ def lds_swz(li):
 
 
 _JBITS = {r: np.array([[(j >> k) & 1 for k in range(r)] for j in range(1 << r)], dtype=np.int64)
-          for r in (4, 5)}
+          for r in (3, 4, 5)}
 
 
 def stage_tables(tile_bits, tb, rbits):
@@ -314,9 +314,11 @@ class TileConfig:
     io_lane_bits: tile-local bits that the first and last stage of a pass keep
               off the registers, so lanes 0..2^io_lane_bits-1 load and store
               consecutive amplitudes (4 = 128-byte runs for complex64);
-    rb: register bits per thread (tile L = tb + rb): 4, or 5 for the per-pass
-              compiled forward kernels only (32 amplitudes per thread, fewer
-              LDS re-distributions per pass).
+    rb: register bits per thread (tile L = tb + rb): 4 everywhere; 5 in the
+              per-pass compiled forward kernels (32 amplitudes per thread,
+              fewer LDS re-distributions per pass); 3 in the per-pass compiled
+              forward and adjoint kernels (8 amplitudes per thread: twice the
+              waves for small states).
     """
     tb: int = 8
     low_bits: int = 6
@@ -642,11 +644,13 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
             else:
                 passes.append(Pass(PASS_DENSE, list(_dense_bits(op)), [op], mat=_dense_mat(op)))
         return encode(n, passes, grad, n_params)
-    if cfg.rb not in (4, 5):
-        raise ValueError("rb (register bits per thread) must be 4 or 5")
+    if cfg.rb not in (3, 4, 5):
+        raise ValueError("rb (register bits per thread) must be 3, 4 or 5")
     if cfg.rb == 5 and (grad or cfg.prefetch or not cfg.specialize):
         raise ValueError("rb=5 runs only in the per-pass compiled forward kernels (specialize=True, "
                          "no prefetch, no adjoint)")
+    if cfg.rb == 3 and (cfg.prefetch or not cfg.specialize):
+        raise ValueError("rb=3 runs only in the per-pass compiled kernels (specialize=True, no prefetch)")
     tb = min(cfg.tb, n - cfg.rb)
     cfg = TileConfig(tb=tb, low_bits=max(1, cfg.low_bits), max_cost=cfg.max_cost,
                      prefetch=cfg.prefetch and not grad, specialize=cfg.specialize, fuse2q=cfg.fuse2q,

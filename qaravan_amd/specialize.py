@@ -44,9 +44,9 @@ def lds_bytes(p, dtype_code, dual):
 
 
 def eligible(p, dtype_code, dual=False):
-    """Forward kernels take 4 or 5 register bits; adjoint kernels 4."""
+    """Forward kernels take 3 to 5 register bits; adjoint kernels 3 or 4."""
     rb = int(p['rb'])
-    return (int(p['kind']) == P.PASS_TILE and (rb == 4 or (rb == 5 and not dual))
+    return (int(p['kind']) == P.PASS_TILE and (rb in (3, 4) or (rb == 5 and not dual))
             and lds_bytes(p, dtype_code, dual) <= MAX_LDS and (dual or int(p['n_grad']) == 0))
 
 
@@ -64,9 +64,9 @@ def _body(R, p, stages, ops, dual):
     if dual:
         out += [f"    cpx<{R}> *lds2 = lds + {2 ** L};",
                 f"    double *gacc = reinterpret_cast<double *>(lds2 + {2 ** L});",
-                f"    cpx<{R}> v[16], w[16];",
-                f"    const uint64_t base = sp_base_wg<{R}, {tb}>(pass, n, wg);",
-                f"    sp_load2<{R}, {tb}>(psi, lam, base, st[0], v, w);"]
+                f"    cpx<{R}> v[{1 << rb}], w[{1 << rb}];",
+                f"    const uint64_t base = sp_base_wg<{R}, {tb}, {rb}>(pass, n, wg);",
+                f"    sp_load2<{R}, {tb}, {rb}>(psi, lam, base, st[0], v, w);"]
     else:
         out += [f"    cpx<{R}> v[{1 << rb}];",
                 f"    const uint64_t base = sp_base_wg<{R}, {tb}, {rb}>(pass, n, wg);",
@@ -75,7 +75,7 @@ def _body(R, p, stages, ops, dual):
         S = stages[s0 + s]
         if s > 0:
             first = "true" if s == 1 else "false"
-            out.append(f"    sp_exchange2<{R}, {first}>(lds, lds2, st[{s - 1}], st[{s}], v, w);" if dual else
+            out.append(f"    sp_exchange2<{R}, {first}, {rb}>(lds, lds2, st[{s - 1}], st[{s}], v, w);" if dual else
                        f"    sp_exchange<{R}, {first}, {rb}>(lds, st[{s - 1}], st[{s}], v);")
         for oi in range(int(S['op_begin']), int(S['op_end'])):
             o = ops[oi]
@@ -85,18 +85,20 @@ def _body(R, p, stages, ops, dual):
             if dual:
                 g = "true" if int(o['flags']) & P.OPF_GRAD else "false"
                 if kind == P.OP_MAT1:
-                    out.append(f"    sp_mat1<{R}, {a}>(v, ops[{k}]); sp_mat1<{R}, {a}>(w, ops[{k}]);")
+                    out.append(f"    sp_mat1<{R}, {a}, {rb}>(v, ops[{k}]); sp_mat1<{R}, {a}, {rb}>(w, ops[{k}]);")
                 elif kind == P.OP_MAT2:
-                    out.append(f"    sp_mat2<{R}, {a}, {b}>(v, ops[{k}]); sp_mat2<{R}, {a}, {b}>(w, ops[{k}]);")
+                    out.append(f"    sp_mat2<{R}, {a}, {b}, {rb}>(v, ops[{k}]); "
+                               f"sp_mat2<{R}, {a}, {b}, {rb}>(w, ops[{k}]);")
                 elif kind == P.OP_CX:
                     th = "true" if a > b else "false"
-                    out.append(f"    ap_cx_t<{R}, 4, {hi}, {lo}, {th}>(v); ap_cx_t<{R}, 4, {hi}, {lo}, {th}>(w);")
+                    out.append(f"    ap_cx_t<{R}, {rb}, {hi}, {lo}, {th}>(v); ap_cx_t<{R}, {rb}, {hi}, {lo}, {th}>(w);")
                 elif kind == P.OP_CZ:
-                    out.append(f"    ap_cz<{R}, 4, {hi}, {lo}>(v); ap_cz<{R}, 4, {hi}, {lo}>(w);")
+                    out.append(f"    ap_cz<{R}, {rb}, {hi}, {lo}>(v); ap_cz<{R}, {rb}, {hi}, {lo}>(w);")
                 elif kind == P.OP_ROT1:
-                    out.append(f"    spd_rot1<{R}, {tb}, {a}, {pauli}, {g}>(v, w, ops[{k}], trig, gacc, {ng});")
+                    out.append(f"    spd_rot1<{R}, {tb}, {a}, {pauli}, {g}, {rb}>(v, w, ops[{k}], trig, gacc, {ng});")
                 elif kind == P.OP_ROT2:
-                    out.append(f"    spd_rot2<{R}, {tb}, {a}, {b}, {pauli}, {g}>(v, w, ops[{k}], trig, gacc, {ng});")
+                    out.append(f"    spd_rot2<{R}, {tb}, {a}, {b}, {pauli}, {g}, {rb}>(v, w, ops[{k}], trig, gacc, "
+                               f"{ng});")
                 else:
                     raise ValueError(kind)
             else:
@@ -116,7 +118,7 @@ def _body(R, p, stages, ops, dual):
                 else:
                     raise ValueError(kind)
     if dual:
-        out.append(f"    sp_store2<{R}, {tb}>(psi, lam, base, st[{ns - 1}], v, w);")
+        out.append(f"    sp_store2<{R}, {tb}, {rb}>(psi, lam, base, st[{ns - 1}], v, w);")
         if ng:
             out.append(f"    sp_grad_flush_wg<{tb}>(gacc, {ng}, gpart, wg);")
     else:

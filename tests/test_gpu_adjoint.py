@@ -24,17 +24,18 @@ def hea_spec(n, layers):
     return spec
 
 
-@pytest.mark.parametrize("n,layers,tb,spz", [(8, 2, 4, False), (10, 3, 6, False), (12, 2, 8, False),
-                                           (10, 3, 6, True), (12, 2, 5, True)])
+@pytest.mark.parametrize("n,layers,tb,spz,rb", [(8, 2, 4, False, 4), (10, 3, 6, False, 4), (12, 2, 8, False, 4),
+                                              (10, 3, 6, True, 4), (12, 2, 5, True, 4), (11, 3, 7, True, 3),
+                                              (12, 4, 9, True, 3)])
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
-def test_adjoint_vs_oracle(gpu, n, layers, tb, spz, dtype):
+def test_adjoint_vs_oracle(gpu, n, layers, tb, spz, rb, dtype):
     from qaravan_amd.adjoint import AdjointModel
     theta = hea_theta(n, layers)
     circ = hea_circuit(n, layers, theta)
     terms = tfim_terms(n)
-    print(f"adjoint n={n} layers={layers} tb={tb} specialize={spz} {dtype}", flush=True)
+    print(f"adjoint n={n} layers={layers} tb={tb} rb={rb} specialize={spz} {dtype}", flush=True)
     model = AdjointModel(circ, terms, len(theta), device=gpu, dtype=dtype,
-                         cfg=P.TileConfig(tb=tb, low_bits=6, max_cost=48, specialize=spz))
+                         cfg=P.TileConfig(tb=tb, low_bits=6, max_cost=48, specialize=spz, rb=rb))
     th = torch.tensor(theta, device=gpu, requires_grad=True)
     e = model(th)
     e.backward()
@@ -93,12 +94,19 @@ def test_adjoint_20q_c64_vs_c128_and_fd(gpu):
     m64 = AdjointModel(circ, terms, len(theta), device=gpu, dtype=torch.complex64)
     m128 = AdjointModel(circ, terms, len(theta), device=gpu, dtype=torch.complex128)
     th = torch.tensor(theta, device=gpu)
+    # the tuned configuration (specialised kernels, 3 or 4 register bits) agrees too
+    from qaravan_amd.autotune import best_adjoint_config
+    mt = AdjointModel(circ, terms, len(theta), device=gpu, dtype=torch.complex64,
+                      cfg=best_adjoint_config(n, torch.complex64))
+    _, gt = mt.energy_and_grad(th)
     e64, g64 = m64.energy_and_grad(th)
     e128, g128 = m128.energy_and_grad(th)
     g64, g128 = g64.cpu().numpy(), g128.cpu().numpy()
     rel = np.linalg.norm(g64 - g128) / np.linalg.norm(g128)
-    print(f"20q: E64={e64.item():.9f} E128={e128.item():.12f} rel grad err c64={rel:.2e}", flush=True)
-    assert rel < 1e-5
+    rel_t = np.linalg.norm(gt.cpu().numpy() - g128) / np.linalg.norm(g128)
+    print(f"20q: E64={e64.item():.9f} E128={e128.item():.12f} rel grad err c64={rel:.2e} tuned={rel_t:.2e}",
+          flush=True)
+    assert rel < 1e-5 and rel_t < 1e-5
     eps = 1e-3
     bias = np.sin(2 * eps) / (2 * eps)
 

@@ -136,7 +136,7 @@ def test_five_register_bit_programs_match_oracle(tb, low, cost, io, seed):
     assert np.linalg.norm(got - O.run(circ.gate_list, n)) < 1e-12
 
 
-def test_five_register_bits_only_in_compiled_forward_kernels():
+def test_register_bit_options_are_checked():
     circ = brickwork_circuit(12, 2)
     ops = P.lower_circuit(circ.build(), 12)
     with pytest.raises(ValueError):
@@ -144,7 +144,9 @@ def test_five_register_bits_only_in_compiled_forward_kernels():
     with pytest.raises(ValueError):
         P.compile_ops(ops, 12, P.TileConfig(tb=6, rb=5, specialize=True), grad=True)
     with pytest.raises(ValueError):
-        P.compile_ops(ops, 12, P.TileConfig(tb=6, rb=3, specialize=True))
+        P.compile_ops(ops, 12, P.TileConfig(tb=6, rb=2, specialize=True))
+    with pytest.raises(ValueError):
+        P.compile_ops(ops, 12, P.TileConfig(tb=6, rb=3))                      # compiled kernels only
 
 
 def _ops_unitary(ops, n):
@@ -205,8 +207,9 @@ def _param_ops(circ, n):
     return P.lower_circuit(circ.build(), n, param_mode=True)
 
 
-@pytest.mark.parametrize("tb,layers,cost", [(4, 3, 40), (6, 3, 40), (6, 6, 1e9), (4, 5, 1e9)])
-def test_param_forward_and_adjoint_programs(tb, layers, cost):
+@pytest.mark.parametrize("tb,layers,cost,rb", [(4, 3, 40, 4), (6, 3, 40, 4), (6, 6, 1e9, 4), (4, 5, 1e9, 4),
+                                               (5, 4, 1e9, 3), (7, 3, 40, 3)])
+def test_param_forward_and_adjoint_programs(tb, layers, cost, rb):
     """Forward program with device-θ rotations, and the adjoint program's
     gradient, interpreted on the CPU, against the oracle's adjoint (RZ and CZ
     are diagonal, so the scheduler may reorder them)."""
@@ -215,7 +218,9 @@ def test_param_forward_and_adjoint_programs(tb, layers, cost):
     theta = hea_theta(n, layers)
     circ = hea_circuit(n, layers, theta)
     terms = tfim_terms(n)
-    pp = ParamProgram(circ, terms, len(theta), cfg=P.TileConfig(tb=tb, low_bits=6, max_cost=cost))
+    pp = ParamProgram(circ, terms, len(theta), cfg=P.TileConfig(tb=tb, low_bits=6, max_cost=cost, rb=rb,
+                                                                specialize=rb != 4))
+    assert all(p.rb == rb for p in pp.adj.passes if p.kind == P.PASS_TILE)
     psi, _ = prog_interp.run(pp.fwd.blob, O.init_state(n), theta=theta)
     ref = O.run(circ.gate_list, n)
     assert np.linalg.norm(psi - ref) < 1e-12
@@ -253,12 +258,12 @@ def _diag_rich_circuit(n, n_gates, rng):
 
 
 @pytest.mark.parametrize("seed", range(4))
-@pytest.mark.parametrize("tb,rb", [(4, 4), (6, 4), (5, 5)])
+@pytest.mark.parametrize("tb,rb", [(4, 4), (6, 4), (5, 5), (7, 3)])
 def test_diagonal_reordering_matches_oracle(seed, tb, rb):
     rng = np.random.default_rng(700 + seed)
     n = 11
     circ = _diag_rich_circuit(n, 150, rng)
-    cfg = P.TileConfig(tb=tb, low_bits=4, max_cost=1e9, specialize=rb == 5, rb=rb)
+    cfg = P.TileConfig(tb=tb, low_bits=4, max_cost=1e9, specialize=rb != 4, rb=rb)
     prog = compile_circ(circ, cfg)
     got, _ = prog_interp.run(prog.blob, O.init_state(n))
     assert np.linalg.norm(got - O.run(circ.gate_list, n)) < 1e-12
@@ -281,7 +286,8 @@ def test_stage_tables_cover_each_tile_once():
     and the swizzled LDS indices are a permutation of [0, 2^L)."""
     circ = brickwork_circuit(16, 3)
     for cfg in (P.TileConfig(tb=4, low_bits=4, max_cost=40), P.TileConfig(tb=8, low_bits=6, max_cost=96),
-                P.TileConfig(tb=6, low_bits=4, max_cost=1e9, io_lane_bits=4, rb=5, specialize=True)):
+                P.TileConfig(tb=6, low_bits=4, max_cost=1e9, io_lane_bits=4, rb=5, specialize=True),
+                P.TileConfig(tb=7, low_bits=4, max_cost=1e9, io_lane_bits=3, rb=3, specialize=True)):
         prog = compile_circ(circ, cfg)
         h, passes, stages, ops, _, _ = prog_interp.decode(prog.blob)
         for p in passes:
