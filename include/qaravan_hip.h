@@ -29,6 +29,8 @@
  *                         torch backend); same gradient, computed by the adjoint method
  *   svh_apply_block5      no counterpart: op_action applies a 5-qubit gate with one
  *                         tensordot; here it is an FP32 MFMA GEMM (c64)
+ *   svh_apply_block5x2    two such op_action calls (statevector_sim.py:50-52) on disjoint
+ *                         qubits of one layer, fused into one sweep over the state
  *   svh_batch_*, svh_spb_* tensorQ/monte_carlo_sim.py:14-35 (many small StatevectorSim runs,
  *                         one after another); here a whole batch of parametrised circuits
  *                         with their adjoint gradients runs as one launch sequence
@@ -200,6 +202,13 @@ int svh_apply_dense(void *psi, int n, int dtype, const int *bits, int k,
  * u_dev = device 32x32 complex64 matrix, row-major.  Programs route their
  * 5-qubit dense passes here for complex64 states. */
 int svh_apply_block5(void *psi, int n, int dtype, const int *bits, const void *u_dev, void *stream);
+
+/* Apply two dense 5-qubit gates on disjoint bits (they commute) in one sweep,
+ * both on the FP32 matrix cores (SVH_C64 only, n >= 12).  bits_a / bits_b and
+ * ua_dev / ub_dev as for svh_apply_block5.  Programs route two consecutive
+ * complex64 5-qubit dense passes on disjoint bits here. */
+int svh_apply_block5x2(void *psi, int n, int dtype, const int *bits_a, const void *ua_dev,
+                       const int *bits_b, const void *ub_dev, void *stream);
 
 /* Bytes of device scratch svh_run_program / svh_run_adjoint need for `prog`. */
 int svh_program_scratch_bytes(const void *prog_host, int dtype, size_t *bytes);

@@ -110,6 +110,149 @@ k_block5(cpx<float> *__restrict__ psi, const cpx<float> *__restrict__ U, Block5A
     }
 }
 
+// ------------------------------------------------ two blocks per sweep
+// Two 5-qubit unitaries A and B on disjoint bits (they commute) applied to
+// one 4096-amplitude tile in one HBM sweep: twice the MFMA work per byte.
+// Tile = A's 5 bits, B's 5 bits and the 2 lowest other bits; LDS complex index
+// (before the swizzle) = rA·128 + rB·4 + o.  GEMM A runs over the 128 columns
+// (rB, o), GEMM B over the 128 columns (rA, o).  The swizzle XORs rB's low
+// three bits with rA's so the strided GEMM-B reads hit distinct banks while
+// GEMM-A reads stay contiguous.
+//
+// When the tile lacks the 2 address bits above its lowest 2, four tiles share
+// every 128-byte line.  The persistent grid then makes the four sibling tiles
+// (those non-tile bits = 0..3) run at the same step on workgroups of the same
+// XCD (workgroups are placed on the 8 XCDs round-robin), so each line is
+// fetched into that XCD's L2 once.
+struct Block5x2Args {
+    int64_t gk[16];   // global offset of tile element 256·k (k < 16)
+    int32_t lk[16];   // LDS complex index of tile element 256·k
+    int32_t tg[8];    // global bit of tile-order bit i (the thread bits)
+    int32_t tl[8];    // LDS index bit of tile-order bit i
+    int64_t tmask;    // global bits inside the tile
+    int64_t ntiles;   // 2^(n-12)
+    int32_t n;
+    int32_t siblings; // 4: group sibling tiles per XCD (see above), 1: plain order
+};
+
+__device__ __forceinline__ int swz2(int idx) { return idx ^ (((idx >> 7) & 7) << 2); }
+
+__global__ void __launch_bounds__(256, 2)
+k_block5x2(cpx<float> *__restrict__ psi, const cpx<float> *__restrict__ UA, const cpx<float> *__restrict__ UB,
+           Block5x2Args A) {
+    __shared__ cpx<float> tile[4096];
+    // B's W fragments in LDS, wb[s][lane] = (W_B[j32][2s+hi], W_B[32+j32][2s+hi])
+    // (lane = j32 + 32 hi): every wave reads the same 16 KB, conflict-free.
+    __shared__ float2 wb[32 * 64];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int j32 = lane & 31;
+    const int hi = lane >> 5;
+
+    float Fa[2][32];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+        const int i = 32 * mt + j32;
+        const int r = i >> 1, a = i & 1;
+#pragma unroll
+        for (int s = 0; s < 32; ++s) {
+            const cpx<float> u = UA[r * 32 + s];
+            Fa[mt][s] = a == 0 ? (hi == 0 ? u.x : -u.y) : (hi == 0 ? u.y : u.x);
+        }
+    }
+    for (int e = tid; e < 32 * 64; e += 256) {
+        const int s = e >> 6, l = e & 63, jj = l & 31, hh = l >> 5;
+        float f[2];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            const int i = 32 * mt + jj;
+            const int r = i >> 1, a = i & 1;
+            const cpx<float> w = UB[r * 32 + s];
+            f[mt] = a == 0 ? (hh == 0 ? w.x : -w.y) : (hh == 0 ? w.y : w.x);
+        }
+        wb[e] = make_float2(f[0], f[1]);
+    }
+    uint64_t gt = 0;
+    int lt = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        gt |= (uint64_t)((tid >> i) & 1) << A.tg[i];
+        lt |= ((tid >> i) & 1) << A.tl[i];
+    }
+    const float *tf = reinterpret_cast<const float *>(tile);
+    const int col = 32 * wave + j32;
+
+    // tile order: with siblings = 4, step k of workgroup w handles
+    // (group g, sibling m) with m = (w / 8) % 4 and g enumerating the rest.
+    const int64_t G = gridDim.x;
+    const int64_t w = blockIdx.x;
+    const int64_t S = A.siblings;
+    const int64_t m = S == 4 ? (w >> 3) & 3 : 0;
+    const int64_t gw = S == 4 ? ((w >> 5) << 3) | (w & 7) : w;   // this workgroup's group slot
+    const int64_t gstride = S == 4 ? G / 4 : G;
+    for (int64_t g = gw; g < A.ntiles / S; g += gstride) {
+        // tile base: tile index deposited into the non-tile bits
+        uint64_t base = 0;
+        {
+            uint64_t q = (uint64_t)(g * S + m);
+            for (int b = 0; b < A.n && q; ++b) {
+                if (!((A.tmask >> b) & 1)) {
+                    base |= (q & 1ull) << b;
+                    q >>= 1;
+                }
+            }
+        }
+        const cpx<float> *src = psi + (base | gt);
+        cpx<float> vals[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) vals[k] = src[A.gk[k]];
+        __syncthreads();  // the previous tile's stores have read LDS
+#pragma unroll
+        for (int k = 0; k < 16; ++k) tile[swz2(lt | A.lk[k])] = vals[k];
+        __syncthreads();
+
+        // GEMM A: rows rA, column col = (rB, o)
+        floatx16 acc0 = {0}, acc1 = {0};
+#pragma unroll
+        for (int s = 0; s < 32; ++s) {
+            const float b = tf[swz2(s * 128 + col) * 2 + hi];
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Fa[0][s], b, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Fa[1][s], b, acc1, 0, 0, 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 16; v += 2) {
+            const int i = (v & 3) + 4 * hi + 8 * (v >> 2);
+            tile[swz2((i >> 1) * 128 + col)] = cpx<float>{acc0[v], acc0[v + 1]};
+            tile[swz2(((32 + i) >> 1) * 128 + col)] = cpx<float>{acc1[v], acc1[v + 1]};
+        }
+        __syncthreads();
+
+        // GEMM B: rows rB, column col = (rA, o): complex index rA·128 + rB·4 + o
+        const int cb = (col >> 2) * 128 + (col & 3);
+        floatx16 bcc0 = {0}, bcc1 = {0};
+#pragma unroll
+        for (int s = 0; s < 32; ++s) {
+            const float b = tf[swz2(cb + s * 4) * 2 + hi];
+            const float2 wv = wb[s * 64 + lane];
+            bcc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(wv.x, b, bcc0, 0, 0, 0);
+            bcc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(wv.y, b, bcc1, 0, 0, 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 16; v += 2) {
+            const int i = (v & 3) + 4 * hi + 8 * (v >> 2);
+            tile[swz2(cb + (i >> 1) * 4)] = cpx<float>{bcc0[v], bcc0[v + 1]};
+            tile[swz2(cb + ((32 + i) >> 1) * 4)] = cpx<float>{bcc1[v], bcc1[v + 1]};
+        }
+        __syncthreads();
+        cpx<float> *dst = psi + (base | gt);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) dst[A.gk[k]] = tile[swz2(lt | A.lk[k])];
+    }
+}
+
 }  // namespace svh
 
 using namespace svh;
@@ -181,4 +324,88 @@ extern "C" int svh_apply_block5(void *psi, int n, int dtype, const int *bits, co
     const int64_t grid = A.ntiles < cap ? A.ntiles : cap;
     k_block5<<<(unsigned)grid, 256, 0, (hipStream_t)stream>>>((cpx<float> *)psi, (const cpx<float> *)u_dev, A);
     return checked_launch("k_block5");
+}
+
+
+// Tile geometry for blocks on address bits ba[0..5) and bb[0..5) (row bit j
+// of each matrix): tile = both blocks + the 2 lowest other bits.
+static int block5x2_args(int n, const int *ba, const int *bb, Block5x2Args *A) {
+    if (n < 12) return fail(-2, "MFMA block path needs n >= 12");
+    uint64_t ma = 0, mb = 0;
+    for (int j = 0; j < 5; ++j) {
+        if (ba[j] < 0 || ba[j] >= n || ((ma >> ba[j]) & 1)) return fail(-4, "bad block bits");
+        if (bb[j] < 0 || bb[j] >= n || ((mb >> bb[j]) & 1)) return fail(-4, "bad block bits");
+        ma |= 1ull << ba[j];
+        mb |= 1ull << bb[j];
+    }
+    if (ma & mb) return fail(-4, "the two blocks share a bit");
+    int other[2], no = 0;
+    for (int b = 0; b < n && no < 2; ++b)
+        if (!(((ma | mb) >> b) & 1)) other[no++] = b;
+    const uint64_t tmask = ma | mb | (1ull << other[0]) | (1ull << other[1]);
+    int tbits[12], nt = 0;
+    for (int b = 0; b < n; ++b)
+        if ((tmask >> b) & 1) tbits[nt++] = b;
+    // LDS bit of tile-order bit p: A row bit j -> 7 + j, B row bit j -> 2 + j, other i -> i
+    int lpos[12];
+    for (int p = 0; p < 12; ++p) {
+        const int g = tbits[p];
+        lpos[p] = -1;
+        for (int j = 0; j < 5; ++j) {
+            if (ba[j] == g) lpos[p] = 7 + j;
+            if (bb[j] == g) lpos[p] = 2 + j;
+        }
+        for (int i = 0; i < 2; ++i)
+            if (other[i] == g) lpos[p] = i;
+    }
+    for (int i = 0; i < 8; ++i) {
+        A->tg[i] = tbits[i];
+        A->tl[i] = lpos[i];
+    }
+    for (int k = 0; k < 16; ++k) {
+        int64_t go = 0;
+        int lo = 0;
+        for (int i = 0; i < 4; ++i) {
+            if ((k >> i) & 1) {
+                go |= 1ll << tbits[8 + i];
+                lo |= 1 << lpos[8 + i];
+            }
+        }
+        A->gk[k] = go;
+        A->lk[k] = lo;
+    }
+    A->tmask = (int64_t)tmask;
+    A->ntiles = 1ll << (n - 12);
+    A->n = n;
+    // siblings: the lowest two non-tile bits are bits 2 and 3 exactly when the
+    // tile holds bits 0 and 1 but not 2 and 3; then 4 tiles share each line
+    const bool low01 = (tmask & 3ull) == 3ull;
+    const bool has23 = (tmask & 12ull) != 0;
+    A->siblings = (low01 && !has23 && A->ntiles >= 32) ? 4 : 1;
+    return 0;
+}
+
+extern "C" int svh_apply_block5x2(void *psi, int n, int dtype, const int *bits_a, const void *ua_dev,
+                                  const int *bits_b, const void *ub_dev, void *stream) {
+    if (!psi || !bits_a || !bits_b || !ua_dev || !ub_dev) return fail(-1, "NULL pointer");
+    if (dtype != SVH_C64) return fail(-3, "the MFMA block path is complex64 only");
+    if (n < 12 || n > kMaxQubits) return fail(-2, "MFMA block path needs 12 <= n <= 36");
+    Block5x2Args A;
+    if (int e = block5x2_args(n, bits_a, bits_b, &A)) return e;
+    int dev = 0, cus = 0;
+    if (int e = hip_status(hipGetDevice(&dev), "hipGetDevice")) return e;
+    if (int e = hip_status(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                           "hipDeviceGetAttribute"))
+        return e;
+    // two resident workgroups per CU; with sibling groups the grid is a
+    // multiple of 32 (8 XCDs x 4 siblings)
+    int64_t grid = 2 * (int64_t)(cus > 0 ? cus : 256);
+    if (A.siblings == 4) grid = (grid / 32) * 32;
+    const int64_t units = A.ntiles / A.siblings;
+    if (A.siblings == 4 && grid / 4 > units) grid = units * 4;
+    if (A.siblings == 1 && grid > A.ntiles) grid = A.ntiles;
+    if (grid < 1) grid = 1;
+    k_block5x2<<<(unsigned)grid, 256, 0, (hipStream_t)stream>>>((cpx<float> *)psi, (const cpx<float> *)ua_dev,
+                                                                (const cpx<float> *)ub_dev, A);
+    return checked_launch("k_block5x2");
 }

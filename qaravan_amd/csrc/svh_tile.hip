@@ -13,6 +13,7 @@
 // U† is applied to both vectors.
 #include "svh_tile_ops.h"
 
+#include <algorithm>
 #include <vector>
 
 namespace svh {
@@ -522,6 +523,54 @@ extern "C" int svh_norm2(const void *psi, int n, int dtype, double *out_dev, voi
                          size_t scratch_bytes, void *stream);
 extern "C" int svh_scale_by_inv_norm(void *psi, int n, int dtype, const double *norm2_dev, void *stream);
 extern "C" int svh_apply_block5(void *psi, int n, int dtype, const int *bits, const void *u_dev, void *stream);
+extern "C" int svh_apply_block5x2(void *psi, int n, int dtype, const int *bits_a, const void *ua_dev,
+                                  const int *bits_b, const void *ub_dev, void *stream);
+
+static bool dense_bits_disjoint(const svh_pass_t &a, const svh_pass_t &b) {
+    uint64_t ma = 0, mb = 0;
+    for (int j = 0; j < a.L; ++j) ma |= 1ull << a.bits[j];
+    for (int j = 0; j < b.L; ++j) mb |= 1ull << b.bits[j];
+    return (ma & mb) == 0;
+}
+
+// A run of consecutive complex64 5-qubit dense passes on pairwise disjoint
+// bits (so they commute) on the matrix cores.  They are ordered by their
+// highest bit and the k-th highest is paired with the k-th lowest: a pair's
+// tile then takes its two spare bits from the low end of the address, which
+// keeps the tile's HBM runs long and its span over pages small (pairing the
+// two highest blocks would spread each tile over 2^10 distant pages).  An odd
+// one out runs alone.  mat = scratch for two 32x32 complex64 matrices.
+static int run_dense5_group(const svh_pass_t *g, int count, void *psi, int n, const double *dpool, void *mat,
+                            hipStream_t st) {
+    int order[8], top[8];
+    for (int i = 0; i < count; ++i) {
+        order[i] = i;
+        top[i] = 0;
+        for (int j = 0; j < 5; ++j) top[i] = g[i].bits[j] > top[i] ? g[i].bits[j] : top[i];
+    }
+    std::sort(order, order + count, [&](int a, int b) { return top[a] > top[b]; });
+    void *mat2 = static_cast<char *>(mat) + 8 * 1024;
+    for (int lo = 0, hi = count - 1; lo <= hi; ++lo, --hi) {
+        const svh_pass_t &A = g[order[lo]];
+        int ab[5];
+        for (int j = 0; j < 5; ++j) ab[j] = A.bits[4 - j];  // row bit j = matrix qubit 4-j
+        k_pool_to<float><<<4, 256, 0, st>>>(dpool + A.pool_off, (cpx<float> *)mat, 1024);
+        if (int e = checked_launch("k_pool_to")) return e;
+        int e = 0;
+        if (lo == hi) {
+            e = svh_apply_block5(psi, n, SVH_C64, ab, mat, st);
+        } else {
+            const svh_pass_t &B = g[order[hi]];
+            int bb[5];
+            for (int j = 0; j < 5; ++j) bb[j] = B.bits[4 - j];
+            k_pool_to<float><<<4, 256, 0, st>>>(dpool + B.pool_off, (cpx<float> *)mat2, 1024);
+            if ((e = checked_launch("k_pool_to"))) return e;
+            e = svh_apply_block5x2(psi, n, SVH_C64, ab, mat, bb, mat2, st);
+        }
+        if (e) return e;
+    }
+    return 0;
+}
 
 // Scratch layout: [trig table: n_params (cos, sin) pairs][work area], the work
 // area holding FP64 reduction partials, a dense pass's matrix, or the adjoint's
@@ -552,7 +601,7 @@ static size_t scratch_need(const svh_prog_header_t *h, int dtype, bool dual) {
     for (int p = 0; p < h->n_passes; ++p) {
         const svh_pass_t &P = passes[p];
         size_t s = (kReduceBlocks + 8) * sizeof(double);
-        if (P.kind == SVH_PASS_DENSE) s += es * ((size_t)1 << (2 * P.L));
+        if (P.kind == SVH_PASS_DENSE) s += 2 * es * ((size_t)1 << (2 * P.L));  // room for a pair
         if (s > need) need = s;
     }
     if (dual) {
@@ -679,6 +728,18 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
                                                      dstage, dop, (const cpx<double> *)trig, gp, st)
                          : launch_tile<double, false>((cpx<double> *)psi, nullptr, n, P, dpass + p, dstage, dop,
                                                       (const cpx<double> *)trig, sd, st);
+        } else if (P.kind == SVH_PASS_DENSE && P.L == 5 && dtype == SVH_C64 && n >= 12) {
+            if (dual) return fail(-8, "dense passes are not differentiable in the adjoint sweep");
+            // complex64 5-qubit gates go to the matrix cores, two per sweep
+            int q = p + 1;
+            while (q < h->n_passes && hpass[q].kind == SVH_PASS_DENSE && hpass[q].L == 5 && q - p < 8) {
+                bool ok = true;
+                for (int r = p; r < q && ok; ++r) ok = dense_bits_disjoint(hpass[r], hpass[q]);
+                if (!ok) break;
+                ++q;
+            }
+            e = run_dense5_group(hpass + p, q - p, psi, n, dpool, sd + kReduceBlocks + 8, st);
+            p = q - 1;
         } else if (P.kind == SVH_PASS_DENSE) {
             if (dual) return fail(-8, "dense passes are not differentiable in the adjoint sweep");
             const int K = 1 << P.L;
@@ -689,14 +750,7 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
                 k_pool_to<double><<<(K * K + 255) / 256, 256, 0, st>>>(dpool + P.pool_off, (cpx<double> *)mat,
                                                                        K * K);
             e = checked_launch("k_pool_to");
-            if (!e && P.L == 5 && dtype == SVH_C64 && n >= 12) {
-                // complex64 5-qubit gates go to the matrix cores
-                int rb[5];
-                for (int j = 0; j < 5; ++j) rb[j] = P.bits[4 - j];
-                e = svh_apply_block5(psi, n, dtype, rb, mat, stream);
-            } else if (!e) {
-                e = svh_apply_dense(psi, n, dtype, P.bits, P.L, mat, stream);
-            }
+            if (!e) e = svh_apply_dense(psi, n, dtype, P.bits, P.L, mat, stream);
         } else if (P.kind == SVH_PASS_NORMALIZE) {
             if (dual) return fail(-8, "normalisation is not differentiable in the adjoint sweep");
             e = svh_norm2(psi, n, dtype, sd + kReduceBlocks, sd, kReduceBlocks * sizeof(double), stream);

@@ -36,7 +36,48 @@ def test_block5_direct_vs_oracle(gpu, bits):
     assert err < 1e-5
 
 
-@pytest.mark.parametrize("n,layers", [(12, 3), (15, 6)])
+@pytest.mark.parametrize("n,ba,bb", [
+    (14, (0, 1, 2, 3, 4), (5, 6, 7, 8, 9)),          # one contiguous 4096-amplitude tile
+    (20, (10, 11, 12, 13, 14), (15, 16, 17, 18, 19)),  # 32-byte runs, sibling tiles grouped per XCD
+    (20, (15, 16, 17, 18, 19), (2, 3, 4, 5, 6)),      # lowest other bits 0, 1; no sibling grouping
+    (17, (3, 9, 1, 13, 6), (16, 0, 11, 2, 8)),        # scrambled bits
+    (12, (11, 10, 9, 8, 7), (6, 5, 4, 3, 2)),         # single tile
+])
+def test_block5x2_direct_vs_oracle(gpu, n, ba, bb):
+    from qaravan_amd import _lib
+    rng = np.random.default_rng(sum(ba) + 7 * sum(bb))
+    ua, ub = haar_unitary(32, rng), haar_unitary(32, rng)
+    sv = rng.standard_normal(2 ** n) + 1j * rng.standard_normal(2 ** n)
+    sv /= np.linalg.norm(sv)
+    psi = torch.tensor(sv, dtype=torch.complex64, device=gpu)
+    uad = torch.tensor(ua, dtype=torch.complex64, device=gpu)
+    ubd = torch.tensor(ub, dtype=torch.complex64, device=gpu)
+    lib = _lib.load()
+    print(f"block5x2 n={n} a={ba} b={bb}", flush=True)
+    _lib.check(lib.svh_apply_block5x2(_lib.ptr(psi), n, 0, (ctypes.c_int * 5)(*ba), _lib.ptr(uad),
+                                      (ctypes.c_int * 5)(*bb), _lib.ptr(ubd), _lib.stream_ptr(gpu)), "block5x2")
+    ref = O.op_action(ua, [n - 1 - ba[4 - a] for a in range(5)], sv)
+    ref = O.op_action(ub, [n - 1 - bb[4 - a] for a in range(5)], ref)
+    err = np.linalg.norm(psi.cpu().numpy() - ref)
+    print(f"  err={err:.2e}", flush=True)
+    assert err < 1e-5
+
+
+def test_block5x2_rejects_bad_arguments(gpu):
+    from qaravan_amd import _lib
+    lib = _lib.load()
+    psi = torch.zeros(2 ** 12, dtype=torch.complex64, device=gpu)
+    ud = torch.zeros(32 * 32, dtype=torch.complex64, device=gpu)
+    a = (ctypes.c_int * 5)(0, 1, 2, 3, 4)
+    assert lib.svh_apply_block5x2(_lib.ptr(psi), 12, 0, a, _lib.ptr(ud), (ctypes.c_int * 5)(4, 5, 6, 7, 8),
+                                  _lib.ptr(ud), None) == -4
+    assert lib.svh_apply_block5x2(_lib.ptr(psi), 12, 1, a, _lib.ptr(ud), (ctypes.c_int * 5)(5, 6, 7, 8, 9),
+                                  _lib.ptr(ud), None) == -3
+    assert lib.svh_apply_block5x2(_lib.ptr(psi), 11, 0, a, _lib.ptr(ud), (ctypes.c_int * 5)(5, 6, 7, 8, 9),
+                                  _lib.ptr(ud), None) == -2
+
+
+@pytest.mark.parametrize("n,layers", [(12, 3), (15, 6), (20, 5)])
 def test_haar_block_circuit_vs_oracle(gpu, n, layers):
     from qaravan_amd import StatevectorSim
     circ = haar_block_circuit(n, layers)
