@@ -257,6 +257,7 @@ def bench_blocks(dev, n=30, layers=24, reps=3):
             times.append(time.perf_counter() - t0)
     dt = min(times)
     flops = nb * (64 * 64 * 2) * 2 ** (n - 5)
+    sweeps = _block_sweeps(prog)
     psi128 = torch.empty(2 ** n, dtype=torch.complex128, device=dev)
     _lib.check(lib.svh_init_basis(_lib.ptr(psi128), n, 1, 0, _lib.stream_ptr(dev)), "init")
     run_program(psi128, prog, sc)
@@ -265,8 +266,24 @@ def bench_blocks(dev, n=30, layers=24, reps=3):
     torch.cuda.empty_cache()
     return dict(workload=f"{n}q x {layers} layers of Haar 5-qubit blocks ({nb} blocks), complex64",
                 ms=dt * 1e3, ms_per_block=dt * 1e3 / nb, blocks_per_s=nb / dt,
-                mfma_tflops=flops / dt / 1e12, mfma_util=flops / dt / 157.3e12,
-                hbm_gbps=nb * 2 * 2 ** n * 8 / dt / 1e9, l2_err_vs_c128=err)
+                mfma_tflops=flops / dt / 1e12, mfma_util=flops / dt / 157.3e12, sweeps=sweeps,
+                hbm_gbps=sweeps * 2 * 2 ** n * 8 / dt / 1e9, l2_err_vs_c128=err)
+
+
+def _block_sweeps(prog):
+    """State sweeps the library makes for a program of complex64 5-qubit dense
+    passes: runs of consecutive passes on pairwise disjoint qubits (at most 8)
+    go two per sweep, an odd one alone."""
+    from qaravan_amd.program import PASS_DENSE
+    sweeps, group = 0, []
+    for p in prog.passes + [None]:
+        bits = set(p.bits) if p is not None and p.kind == PASS_DENSE and len(p.bits) == 5 else None
+        if bits is not None and len(group) < 8 and all(not (bits & g) for g in group):
+            group.append(bits)
+            continue
+        sweeps += (len(group) + 1) // 2
+        group = [bits] if bits is not None else []
+    return sweeps
 
 
 def bench_c0(dev, reps=20):

@@ -1,15 +1,18 @@
 // Dense 5-qubit blocks on the FP32 matrix cores (complex64 states).
 //
-// A 32x32 complex unitary U acting on 5 address bits is a real GEMM
-//   Y(64 x C) = W(64 x 64) · X(64 x C),  W[2r+a][2c+b] = [[Re U, -Im U], [Im U, Re U]]
-// where X's rows interleave (re, im) of the 32 block amplitudes and its C
-// columns run over the other bits.  One workgroup (4 waves) owns a tile of
-// 4096 amplitudes: the 5 block bits and the 7 lowest other bits, so every tile
-// is a set of >= 128-amplitude contiguous runs in HBM.  The tile is staged in
-// LDS with coalesced loads, each wave multiplies a 64 x 32-column slab with
-// v_mfma_f32_32x32x2_f32 (W lives in 64 VGPRs per lane for the whole launch),
-// writes its result back to LDS, and the tile is stored coalesced.  The grid
-// is persistent (workgroups stride over tiles) so W is built once per WG.
+// A 32x32 complex unitary U = a + i·b acting on 5 address bits multiplies a
+// 32 x C complex matrix X = x + i·y whose rows are the 32 block amplitudes and
+// whose C columns run over the other bits.  It is done as three real FP32
+// GEMMs (the 3M form): P1 = a·x, P2 = b·y, P3 = (a + b)·(x + y), then
+// Re = P1 − P2 and Im = P3 − P1 − P2 — 3/4 of the MFMA work of the 64x64 real
+// form [[a, −b], [b, a]].  One workgroup (4 waves) owns a tile of 4096
+// amplitudes: the 5 block bits and the 7 lowest other bits, so every tile is a
+// set of >= 128-amplitude contiguous runs in HBM.  The tile is staged in LDS
+// with coalesced loads, each wave multiplies a 32-column slab with
+// v_mfma_f32_32x32x2_f32 (a, b and a + b live in 48 VGPRs per lane for the
+// whole launch), writes its result back to LDS, and the tile is stored
+// coalesced.  The grid is persistent (workgroups stride over tiles) so the
+// fragments are built once per WG.
 //
 // Operand layouts of v_mfma_f32_32x32x2_f32 (wave64):
 //   A[i][k]: lane i + 32k;  B[k][j]: lane j + 32k;
@@ -41,19 +44,17 @@ k_block5(cpx<float> *__restrict__ psi, const cpx<float> *__restrict__ U, Block5A
     const int j32 = lane & 31;
     const int hi = lane >> 5;  // 0: real rows / k even, 1: imaginary rows / k odd
 
-    // W fragments: Af[mt][s] = W[32 mt + j32][2 s + hi]
-    //   row i = 32 mt + j32 -> amplitude r = i / 2, part a = i % 2
-    //   col k = 2 s + hi    -> amplitude c = s,     part b = hi
-    float Af[2][32];
+    // 3M complex product (U = a + i·b, X = x + i·y):
+    //   Re(UX) = a·x − b·y,  Im(UX) = (a + b)·(x + y) − a·x − b·y.
+    // MFMA step t (k = 2t + hi): lane (j32, hi) holds U[j32][2t + hi] as the A
+    // operand and amplitude 2t + hi of its column as the B operand.
+    float Fr[16], Fi[16], Fs[16];
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
-        const int i = 32 * mt + j32;
-        const int r = i >> 1, a = i & 1;
-#pragma unroll
-        for (int s = 0; s < 32; ++s) {
-            const cpx<float> u = U[r * 32 + s];
-            Af[mt][s] = a == 0 ? (hi == 0 ? u.x : -u.y) : (hi == 0 ? u.y : u.x);
-        }
+    for (int t = 0; t < 16; ++t) {
+        const cpx<float> u = U[j32 * 32 + 2 * t + hi];
+        Fr[t] = u.x;
+        Fi[t] = u.y;
+        Fs[t] = u.x + u.y;
     }
 
     // this thread's share of the tile-order index: t = tid (8 bits)
@@ -64,7 +65,6 @@ k_block5(cpx<float> *__restrict__ psi, const cpx<float> *__restrict__ U, Block5A
         gt |= (uint64_t)((tid >> i) & 1) << A.tg[i];
         lt |= ((tid >> i) & 1) << A.tl[i];
     }
-    const float *tf = reinterpret_cast<const float *>(tile);
     const int col = 32 * wave + j32;
 
     for (int64_t t = blockIdx.x; t < A.ntiles; t += gridDim.x) {
@@ -88,20 +88,21 @@ k_block5(cpx<float> *__restrict__ psi, const cpx<float> *__restrict__ U, Block5A
         for (int k = 0; k < 16; ++k) tile[lt | A.lk[k]] = vals[k];
         __syncthreads();
 
-        floatx16 acc0 = {0}, acc1 = {0};
+        floatx16 p1 = {0}, p2 = {0}, p3 = {0};
 #pragma unroll
-        for (int s = 0; s < 32; ++s) {
-            const float b = tf[(s * 128 + col) * 2 + hi];
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Af[0][s], b, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Af[1][s], b, acc1, 0, 0, 0);
+        for (int t = 0; t < 16; ++t) {
+            const cpx<float> x = tile[(2 * t + hi) * 128 + col];
+            p1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Fr[t], x.x, p1, 0, 0, 0);
+            p2 = __builtin_amdgcn_mfma_f32_32x32x2f32(Fi[t], x.y, p2, 0, 0, 0);
+            p3 = __builtin_amdgcn_mfma_f32_32x32x2f32(Fs[t], x.x + x.y, p3, 0, 0, 0);
         }
         // every wave has read its slab before any wave overwrites LDS
         __syncthreads();
+        // D layout: row i = (v & 3) + 4 hi + 8 (v >> 2) of register v
 #pragma unroll
-        for (int v = 0; v < 16; v += 2) {
-            const int i = (v & 3) + 4 * hi + 8 * (v >> 2);  // even row -> (re, im) in v, v+1
-            tile[(i >> 1) * 128 + col] = cpx<float>{acc0[v], acc0[v + 1]};
-            tile[((32 + i) >> 1) * 128 + col] = cpx<float>{acc1[v], acc1[v + 1]};
+        for (int v = 0; v < 16; ++v) {
+            const int i = (v & 3) + 4 * hi + 8 * (v >> 2);
+            tile[i * 128 + col] = cpx<float>{p1[v] - p2[v], p3[v] - p1[v] - p2[v]};
         }
         __syncthreads();
         cpx<float> *dst = psi + (base | gt);
@@ -141,37 +142,31 @@ __global__ void __launch_bounds__(256, 2)
 k_block5x2(cpx<float> *__restrict__ psi, const cpx<float> *__restrict__ UA, const cpx<float> *__restrict__ UB,
            Block5x2Args A) {
     __shared__ cpx<float> tile[4096];
-    // B's W fragments in LDS, wb[s][lane] = (W_B[j32][2s+hi], W_B[32+j32][2s+hi])
-    // (lane = j32 + 32 hi): every wave reads the same 16 KB, conflict-free.
-    __shared__ float2 wb[32 * 64];
+    // 3M complex products: with U = a + i·b and X = x + i·y,
+    //   Re(UX) = a·x − b·y,  Im(UX) = (a + b)·(x + y) − a·x − b·y,
+    // three real 32x32 GEMMs instead of the four of the real 64x64 form.
+    // MFMA step t (k = 2t + hi): lane (j32, hi) holds U[j32][2t + hi] as the A
+    // operand and amplitude 2t + hi of its column as the B operand.
+    // A's fragments live in registers, B's in LDS: wb[t][lane] = (b_re, b_im, b_re + b_im).
+    __shared__ float4 wb[16 * 64];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int j32 = lane & 31;
     const int hi = lane >> 5;
 
-    float Fa[2][32];
+    float Fr[16], Fi[16], Fs[16];
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
-        const int i = 32 * mt + j32;
-        const int r = i >> 1, a = i & 1;
-#pragma unroll
-        for (int s = 0; s < 32; ++s) {
-            const cpx<float> u = UA[r * 32 + s];
-            Fa[mt][s] = a == 0 ? (hi == 0 ? u.x : -u.y) : (hi == 0 ? u.y : u.x);
-        }
+    for (int t = 0; t < 16; ++t) {
+        const cpx<float> u = UA[j32 * 32 + 2 * t + hi];
+        Fr[t] = u.x;
+        Fi[t] = u.y;
+        Fs[t] = u.x + u.y;
     }
-    for (int e = tid; e < 32 * 64; e += 256) {
-        const int s = e >> 6, l = e & 63, jj = l & 31, hh = l >> 5;
-        float f[2];
-#pragma unroll
-        for (int mt = 0; mt < 2; ++mt) {
-            const int i = 32 * mt + jj;
-            const int r = i >> 1, a = i & 1;
-            const cpx<float> w = UB[r * 32 + s];
-            f[mt] = a == 0 ? (hh == 0 ? w.x : -w.y) : (hh == 0 ? w.y : w.x);
-        }
-        wb[e] = make_float2(f[0], f[1]);
+    for (int e = tid; e < 16 * 64; e += 256) {
+        const int t = e >> 6, l = e & 63;
+        const cpx<float> w = UB[(l & 31) * 32 + 2 * t + (l >> 5)];
+        wb[e] = make_float4(w.x, w.y, w.x + w.y, 0.f);
     }
     uint64_t gt = 0;
     int lt = 0;
@@ -180,7 +175,6 @@ k_block5x2(cpx<float> *__restrict__ psi, const cpx<float> *__restrict__ UA, cons
         gt |= (uint64_t)((tid >> i) & 1) << A.tg[i];
         lt |= ((tid >> i) & 1) << A.tl[i];
     }
-    const float *tf = reinterpret_cast<const float *>(tile);
     const int col = 32 * wave + j32;
 
     // tile order: with siblings = 4, step k of workgroup w handles
@@ -213,38 +207,39 @@ k_block5x2(cpx<float> *__restrict__ psi, const cpx<float> *__restrict__ UA, cons
         __syncthreads();
 
         // GEMM A: rows rA, column col = (rB, o)
-        floatx16 acc0 = {0}, acc1 = {0};
+        floatx16 p1 = {0}, p2 = {0}, p3 = {0};
 #pragma unroll
-        for (int s = 0; s < 32; ++s) {
-            const float b = tf[swz2(s * 128 + col) * 2 + hi];
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Fa[0][s], b, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Fa[1][s], b, acc1, 0, 0, 0);
+        for (int t = 0; t < 16; ++t) {
+            const cpx<float> x = tile[swz2((2 * t + hi) * 128 + col)];
+            p1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Fr[t], x.x, p1, 0, 0, 0);
+            p2 = __builtin_amdgcn_mfma_f32_32x32x2f32(Fi[t], x.y, p2, 0, 0, 0);
+            p3 = __builtin_amdgcn_mfma_f32_32x32x2f32(Fs[t], x.x + x.y, p3, 0, 0, 0);
         }
         __syncthreads();
+        // D layout: row i = (v & 3) + 4 hi + 8 (v >> 2) of register v
 #pragma unroll
-        for (int v = 0; v < 16; v += 2) {
+        for (int v = 0; v < 16; ++v) {
             const int i = (v & 3) + 4 * hi + 8 * (v >> 2);
-            tile[swz2((i >> 1) * 128 + col)] = cpx<float>{acc0[v], acc0[v + 1]};
-            tile[swz2(((32 + i) >> 1) * 128 + col)] = cpx<float>{acc1[v], acc1[v + 1]};
+            tile[swz2(i * 128 + col)] = cpx<float>{p1[v] - p2[v], p3[v] - p1[v] - p2[v]};
         }
         __syncthreads();
 
         // GEMM B: rows rB, column col = (rA, o): complex index rA·128 + rB·4 + o
         const int cb = (col >> 2) * 128 + (col & 3);
-        floatx16 bcc0 = {0}, bcc1 = {0};
+        floatx16 q1 = {0}, q2 = {0}, q3 = {0};
 #pragma unroll
-        for (int s = 0; s < 32; ++s) {
-            const float b = tf[swz2(cb + s * 4) * 2 + hi];
-            const float2 wv = wb[s * 64 + lane];
-            bcc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(wv.x, b, bcc0, 0, 0, 0);
-            bcc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(wv.y, b, bcc1, 0, 0, 0);
+        for (int t = 0; t < 16; ++t) {
+            const cpx<float> x = tile[swz2(cb + (2 * t + hi) * 4)];
+            const float4 w = wb[t * 64 + lane];
+            q1 = __builtin_amdgcn_mfma_f32_32x32x2f32(w.x, x.x, q1, 0, 0, 0);
+            q2 = __builtin_amdgcn_mfma_f32_32x32x2f32(w.y, x.y, q2, 0, 0, 0);
+            q3 = __builtin_amdgcn_mfma_f32_32x32x2f32(w.z, x.x + x.y, q3, 0, 0, 0);
         }
         __syncthreads();
 #pragma unroll
-        for (int v = 0; v < 16; v += 2) {
+        for (int v = 0; v < 16; ++v) {
             const int i = (v & 3) + 4 * hi + 8 * (v >> 2);
-            tile[swz2(cb + (i >> 1) * 4)] = cpx<float>{bcc0[v], bcc0[v + 1]};
-            tile[swz2(cb + ((32 + i) >> 1) * 4)] = cpx<float>{bcc1[v], bcc1[v + 1]};
+            tile[swz2(cb + i * 4)] = cpx<float>{q1[v] - q2[v], q3[v] - q1[v] - q2[v]};
         }
         __syncthreads();
         cpx<float> *dst = psi + (base | gt);
