@@ -120,11 +120,10 @@ k_block5(cpx<float> *__restrict__ psi, const cpx<float> *__restrict__ U, Block5A
 // three bits with rA's so the strided GEMM-B reads hit distinct banks while
 // GEMM-A reads stay contiguous.
 //
-// When the tile lacks the 2 address bits above its lowest 2, four tiles share
-// every 128-byte line.  The persistent grid then makes the four sibling tiles
-// (those non-tile bits = 0..3) run at the same step on workgroups of the same
-// XCD (workgroups are placed on the 8 XCDs round-robin), so each line is
-// fetched into that XCD's L2 once.
+// When s of the address bits 0..3 lie outside the tile, 2^s tiles share every
+// 128-byte line.  The persistent grid then makes those sibling tiles run at
+// the same step on workgroups of the same XCD (workgroups are placed on the 8
+// XCDs round-robin), so each line is fetched into that XCD's L2 once.
 struct Block5x2Args {
     int64_t gk[16];   // global offset of tile element 256·k (k < 16)
     int32_t lk[16];   // LDS complex index of tile element 256·k
@@ -133,7 +132,7 @@ struct Block5x2Args {
     int64_t tmask;    // global bits inside the tile
     int64_t ntiles;   // 2^(n-12)
     int32_t n;
-    int32_t siblings; // 4: group sibling tiles per XCD (see above), 1: plain order
+    int32_t sib_bits; // s: sibling tiles grouped per XCD (see above), 0: plain order
 };
 
 __device__ __forceinline__ int swz2(int idx) { return idx ^ (((idx >> 7) & 7) << 2); }
@@ -177,15 +176,18 @@ k_block5x2(cpx<float> *__restrict__ psi, const cpx<float> *__restrict__ UA, cons
     }
     const int col = 32 * wave + j32;
 
-    // tile order: with siblings = 4, step k of workgroup w handles
-    // (group g, sibling m) with m = (w / 8) % 4 and g enumerating the rest.
+    // tile order: tile = g·S + m with S = 2^s siblings; workgroup w takes
+    // sibling m = (w / 8) mod S of the groups g = gw, gw + G/S, ... (gw = w
+    // with those bits removed), so the S workgroups of one group are w, w + 8,
+    // ..., on one XCD.  The grid is a multiple of 8·S.
     const int64_t G = gridDim.x;
     const int64_t w = blockIdx.x;
-    const int64_t S = A.siblings;
-    const int64_t m = S == 4 ? (w >> 3) & 3 : 0;
-    const int64_t gw = S == 4 ? ((w >> 5) << 3) | (w & 7) : w;   // this workgroup's group slot
-    const int64_t gstride = S == 4 ? G / 4 : G;
-    for (int64_t g = gw; g < A.ntiles / S; g += gstride) {
+    const int sb = A.sib_bits;
+    const int64_t S = 1ll << sb;
+    const int64_t m = (w >> 3) & (S - 1);
+    const int64_t gw = ((w >> (3 + sb)) << 3) | (w & 7);
+    const int64_t gstride = G >> sb;
+    for (int64_t g = gw; g < (A.ntiles >> sb); g += gstride) {
         // tile base: tile index deposited into the non-tile bits
         uint64_t base = 0;
         {
@@ -372,11 +374,12 @@ static int block5x2_args(int n, const int *ba, const int *bb, Block5x2Args *A) {
     A->tmask = (int64_t)tmask;
     A->ntiles = 1ll << (n - 12);
     A->n = n;
-    // siblings: the lowest two non-tile bits are bits 2 and 3 exactly when the
-    // tile holds bits 0 and 1 but not 2 and 3; then 4 tiles share each line
-    const bool low01 = (tmask & 3ull) == 3ull;
-    const bool has23 = (tmask & 12ull) != 0;
-    A->siblings = (low01 && !has23 && A->ntiles >= 32) ? 4 : 1;
+    // siblings: the non-tile bits among 0..3 (at most two, as the tile holds
+    // the two lowest non-block bits) are the lowest non-tile bits, so they are
+    // the low bits of the tile index
+    int sb = 0;
+    for (int b = 0; b < 4; ++b) sb += !((tmask >> b) & 1);
+    A->sib_bits = A->ntiles >= (8ll << sb) ? sb : 0;
     return 0;
 }
 
@@ -392,13 +395,12 @@ extern "C" int svh_apply_block5x2(void *psi, int n, int dtype, const int *bits_a
     if (int e = hip_status(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
                            "hipDeviceGetAttribute"))
         return e;
-    // two resident workgroups per CU; with sibling groups the grid is a
-    // multiple of 32 (8 XCDs x 4 siblings)
+    // two resident workgroups per CU; with 2^s siblings the grid is a
+    // multiple of 8·2^s (8 XCDs) and at most the tile count
+    const int64_t quant = A.sib_bits ? 8ll << A.sib_bits : 1;
     int64_t grid = 2 * (int64_t)(cus > 0 ? cus : 256);
-    if (A.siblings == 4) grid = (grid / 32) * 32;
-    const int64_t units = A.ntiles / A.siblings;
-    if (A.siblings == 4 && grid / 4 > units) grid = units * 4;
-    if (A.siblings == 1 && grid > A.ntiles) grid = A.ntiles;
+    grid = grid / quant * quant;
+    if (grid > A.ntiles) grid = A.ntiles;  // ntiles >= quant and both are powers of two
     if (grid < 1) grid = 1;
     k_block5x2<<<(unsigned)grid, 256, 0, (hipStream_t)stream>>>((cpx<float> *)psi, (const cpx<float> *)ua_dev,
                                                                 (const cpx<float> *)ub_dev, A);
