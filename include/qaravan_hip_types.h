@@ -1,0 +1,153 @@
+/* Types and constants of the qaravan HIP statevector library's C ABI: the
+ * gate-program layout, pass/stage/op records and batch item tables.  The
+ * entry points are declared in qaravan_hip.h, which includes this file.
+ * Code generated for specialised passes depends on this file only (not on
+ * the entry-point declarations), so its content keys the kernel cache. */
+#ifndef QARAVAN_HIP_TYPES_H
+#define QARAVAN_HIP_TYPES_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVH_ABI_VERSION 1
+
+enum { SVH_C64 = 0, SVH_C128 = 1 };
+
+/* ---- gate program ---------------------------------------------------------
+ * A program is one contiguous byte buffer: svh_prog_header_t, then the pass,
+ * stage, op, grad-slot (int32) and matrix-pool (double) arrays at the byte
+ * offsets the header gives.  The caller builds it on the host, copies it to
+ * device memory, and passes both copies to svh_run_program / svh_run_adjoint:
+ * the host copy drives the launches, the device copy is read by the kernels.
+ */
+#define SVH_PROG_MAGIC 0x53564850u /* 'SVHP' */
+#define SVH_MAX_TILE_BITS 16
+#define SVH_MAX_REG_BITS 5
+
+enum {
+    SVH_PASS_TILE = 1,      /* fused pass over tiles of 2^L amplitudes            */
+    SVH_PASS_DENSE = 2,     /* one dense k-qubit gate (k <= 6) from the pool        */
+    SVH_PASS_NORMALIZE = 3  /* ψ /= ‖ψ‖₂ (FP64 norm), used after Reset projectors   */
+};
+
+enum {
+    SVH_OP_MAT1 = 1, /* dense 2x2 on register bit a                               */
+    SVH_OP_MAT2 = 2, /* dense 4x4 on register bits (a, b), a > b, a = row MSB       */
+    SVH_OP_CX = 3,   /* controlled-X: a = target bit, b = control bit              */
+    SVH_OP_CZ = 4,   /* diag(1,1,1,-1) on (a, b)                                   */
+    SVH_OP_ROT1 = 5, /* exp(-i·s·θ[param]·P), P = pauli (1 X, 2 Y, 3 Z), on a       */
+    SVH_OP_ROT2 = 6  /* exp(-i·s·θ[param]·P⊗P) on (a, b)                           */
+};
+
+enum {
+    SVH_OPF_NEG = 1, /* s = -1 (the adjoint sweep applies U†)                     */
+    SVH_OPF_GRAD = 2 /* before applying: grad_slot += 2·Im⟨λ|P|ψ⟩ (adjoint only)   */
+};
+
+typedef struct {
+    uint32_t magic;
+    int32_t version;
+    int32_t n_qubits;
+    int32_t n_passes;
+    int32_t n_stages;
+    int32_t n_ops;
+    int32_t n_grad_slots;
+    int32_t n_pool; /* doubles in the matrix pool */
+    int64_t off_passes, off_stages, off_ops, off_grad, off_pool;
+    int64_t n_params; /* length of θ read by ROT ops (0 if none)                  */
+    int64_t reserved[2];
+} svh_prog_header_t; /* 96 bytes */
+
+typedef struct {
+    int32_t kind;        /* SVH_PASS_*                                              */
+    int32_t L;           /* tile bits (TILE) or gate qubits k (DENSE)              */
+    int32_t tb;          /* log2 threads per workgroup (TILE)                       */
+    int32_t rb;          /* register bits per thread (TILE); L = tb + rb            */
+    int32_t stage_begin; /* first stage (TILE)                                       */
+    int32_t n_stages;
+    int32_t grad_begin;  /* first grad slot of this pass (adjoint)                  */
+    int32_t n_grad;
+    int64_t pool_off;    /* DENSE: offset (doubles) of the 2^k x 2^k complex matrix  */
+    int32_t flags;       /* SVH_PASSF_*                                              */
+    int32_t pad;
+    int32_t bits[SVH_MAX_TILE_BITS]; /* TILE: global bit of tile-local bit i (i < L);
+                                        DENSE: global bit of matrix qubit i (i = 0 is
+                                        the most significant row bit)                */
+} svh_pass_t; /* 112 bytes */
+
+enum {
+    /* TILE (forward only): persistent workgroups; the next tile is copied
+     * HBM -> LDS asynchronously while the current one is computed.  Needs
+     * tile-local bit 0 = address bit 0. */
+    SVH_PASSF_PREFETCH = 1
+};
+
+/* A stage's layout: thread t holds registers j = 0..2^rb-1 at tile-local index
+ * insert-zeros(t at rsorted) | loc(j), i.e. at global amplitude
+ * base | Σ_i bit_i(t)·2^tgbit[i] | goff[j].  The tables are precomputed by the
+ * host so the kernel does no bit arithmetic per amplitude.  The op-interpreting
+ * kernels run rb = 4; the per-pass compiled kernels (svh_sp_*) run rb = 4 or 5. */
+typedef struct {
+    int64_t goff[32];    /* global amplitude offset of register j                 */
+    int32_t lsw[32];     /* swizzled tile-local (LDS) offset of register j         */
+    int32_t rbits[8];    /* tile-local bit held in register-index bit k (k < rb)   */
+    int32_t rsorted[8];  /* rbits[0..rb) in ascending order                         */
+    int32_t tgbit[12];   /* global bit of thread-index bit i (i < tb)              */
+    int32_t op_begin, op_end;
+    int32_t pad[2];
+} svh_stage_t; /* 512 bytes */
+
+typedef struct {
+    int32_t kind;        /* SVH_OP_*                                                */
+    int32_t a, b;        /* register-index bits                                      */
+    int32_t param;       /* θ index (ROT*)                                           */
+    int32_t pauli;       /* 1 X, 2 Y, 3 Z (ROT*)                                     */
+    int32_t flags;       /* SVH_OPF_*                                                */
+    int32_t grad_slot;   /* pass-local gradient slot (SVH_OPF_GRAD)                  */
+    int32_t pad;
+    double m[32];        /* complex row-major matrix (re, im pairs)                  */
+    float mf[32];        /* the same matrix rounded to float (used by SVH_C64)       */
+} svh_op_t; /* 416 bytes */
+
+/* ---- batches of circuits ----------------------------------------------------
+ * Many independent circuits (different n, depth, θ) advance together: round r
+ * of a batch runs pass r of every circuit that has one, in ONE launch per
+ * (tile shape, mode), so small circuits fill the GPU together.  The caller
+ * (host planner) builds the item tables once, in device memory; a whole batch
+ * (init, trig, forward rounds, H·ψ and energies, adjoint rounds with gradient
+ * reduction) is a fixed launch sequence that can be captured in a HIP graph. */
+typedef struct {
+    uint64_t psi, lam;          /* device state / λ pointers (lam = 0 forward)      */
+    uint64_t pass, stages, ops; /* device program pointers: this circuit's pass r   */
+    uint64_t trig;              /* device (cos, sin) table of the circuit          */
+    uint64_t gpart;             /* device per-workgroup gradient partials           */
+    int64_t wg_begin;           /* first workgroup of the item in the launch         */
+    int32_t n, pad;
+} svh_batch_item_t; /* 72 bytes */
+
+typedef struct {
+    uint64_t part;              /* device partial sums: row b of ng values          */
+    uint64_t slot_param;        /* device int32 slot -> output index                 */
+    uint64_t out;               /* device float64 outputs (grad or energy)           */
+    uint64_t tstamp;            /* device uint64 completion timestamp or 0           */
+    int32_t nblocks, ng;        /* partial rows, slots                                */
+    int64_t slot_begin;         /* first launch block of the item                     */
+} svh_batch_red_t; /* 48 bytes */
+
+typedef struct {
+    uint64_t psi, lam;          /* device ψ (read) and λ = Hψ (written)             */
+    uint64_t terms;             /* device term table (32 B per term: x, z, re, im)    */
+    uint64_t epart;             /* device FP64 per-block partials of Re⟨ψ|λ⟩         */
+    int64_t block_begin;        /* first launch block of the item                     */
+    int32_t n, nterms;
+} svh_batch_ham_t; /* 48 bytes */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QARAVAN_HIP_TYPES_H */

@@ -11,6 +11,8 @@ launches those kernels for the passes they cover (``svh_run_program_sp``).
 """
 import hashlib
 import os
+import sys
+import time
 import subprocess
 
 import numpy as np
@@ -197,7 +199,9 @@ def _headers_digest():
     for f in ("svh_common.h", "svh_tile_ops.h"):
         with open(os.path.join(CSRC, f), "rb") as fh:
             m.update(fh.read())
-    with open(os.path.join(HERE, "..", "include", "qaravan_hip.h"), "rb") as fh:
+    # the kernels use the ABI's types; the entry-point declarations do not
+    # enter generated code
+    with open(os.path.join(HERE, "..", "include", "qaravan_hip_types.h"), "rb") as fh:
         m.update(fh.read())
     return m.hexdigest()
 
@@ -212,11 +216,14 @@ def _compile_one(src, tag):
     with open(srcf, "w") as f:
         f.write(src)
     tmp = co + f".tmp{os.getpid()}_{tag}"
+    t0 = time.time()
     r = subprocess.run([HIPCC, *FLAGS, srcf, "-o", tmp], capture_output=True, text=True)
     if r.returncode != 0:
         err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
         raise RuntimeError("specialised program compile failed:\n" + err[-4000:])
     os.replace(tmp, co)
+    # a cache miss costs seconds to tens of seconds: say so, one line each
+    print(f"[specialize] compiled {os.path.basename(co)} in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     return co
 
 
