@@ -35,6 +35,7 @@ MAGIC = 0x53564850
 ABI_VERSION = 1
 PASS_TILE, PASS_DENSE, PASS_NORMALIZE = 1, 2, 3
 PASSF_PREFETCH = 1
+PASSF_HALF_LDS = 2
 OP_MAT1, OP_MAT2, OP_CX, OP_CZ, OP_ROT1, OP_ROT2 = 1, 2, 3, 4, 5, 6
 OPF_NEG, OPF_GRAD = 1, 2
 RB = 4
@@ -319,6 +320,10 @@ class TileConfig:
               fewer LDS re-distributions per pass); 3 in the per-pass compiled
               forward and adjoint kernels (8 amplitudes per thread: twice the
               waves for small states).
+    half_lds: per-pass compiled forward kernels move the tile through LDS in
+              two halves, split on a register bit that the two layouts of a
+              re-distribution share: half the LDS per workgroup, so more
+              workgroups per CU (the stage planner keeps such a bit).
     """
     tb: int = 8
     low_bits: int = 6
@@ -328,6 +333,7 @@ class TileConfig:
     fuse2q: bool = True
     io_lane_bits: int = 6
     rb: int = RB
+    half_lds: bool = False
 
 
 def _is_barrier(op):
@@ -418,8 +424,10 @@ def partition(ops, n, cfg):
     return passes
 
 
-def _stage_ops(p, io_lane_bits=6):
-    """Split a tile pass's ops into register stages (tile-local bits)."""
+def _stage_ops(p, io_lane_bits=6, keep_shared=False):
+    """Split a tile pass's ops into register stages (tile-local bits).
+    keep_shared: every stage keeps at least one register bit of the stage
+    before it where the ops leave a free register slot (half-LDS exchanges)."""
     RB = p.rb
     L = p.tb + RB
     loc = {g: i for i, g in enumerate(p.bits)}
@@ -449,12 +457,16 @@ def _stage_ops(p, io_lane_bits=6):
                 blocked.add(op, s)
                 continue
             new = [x for x in lb if x not in R]
-            if len(R) + len(new) <= RB:
+            # keep_shared: the last free slot is for a bit of the previous layout
+            room = RB - (1 if keep_shared and cur is not None and not set(R + new) & set(cur.rbits) else 0)
+            if len(R) + len(new) <= room:
                 R += new
             else:
                 blocked.add(op, s)
             if len(R) == RB:
                 break
+        if keep_shared and cur is not None and len(R) < RB and not set(R) & set(cur.rbits):
+            R.append(max(cur.rbits))            # a bit both layouts hold
         for x in reversed(range(L)):            # fill with high tile bits
             if len(R) == RB:
                 break
@@ -466,11 +478,26 @@ def _stage_ops(p, io_lane_bits=6):
         stages.append(Stage(list(io_bits), []))
     # coalesced load/store: the first/last layout must keep lanes on low bits
     if set(stages[0].rbits) & lane_bits:
-        stages.insert(0, Stage(list(io_bits), []))
+        stages.insert(0, Stage(_io_layout(stages[0].rbits, io_bits, lane_bits, keep_shared), []))
     if set(stages[-1].rbits) & lane_bits:
-        stages.append(Stage(list(io_bits), []))
+        stages.append(Stage(_io_layout(stages[-1].rbits, io_bits, lane_bits, keep_shared), []))
     p.stages = stages
     return p
+
+
+def _io_layout(nb, io_bits, lane_bits, keep_shared):
+    """Register bits of a coalesced first/last layout next to the stage
+    holding ``nb``: the high tile bits, or (keep_shared) the neighbour's
+    non-lane register bits topped up with high tile bits."""
+    if not keep_shared:
+        return list(io_bits)
+    R = [x for x in nb if x not in lane_bits]
+    for x in range(max(io_bits), -1, -1):
+        if len(R) == len(io_bits):
+            break
+        if x not in R and x not in lane_bits:
+            R.append(x)
+    return R
 
 
 # ----------------------------------------------------------------- encoding
@@ -651,14 +678,19 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
                          "no prefetch, no adjoint)")
     if cfg.rb == 3 and (cfg.prefetch or not cfg.specialize):
         raise ValueError("rb=3 runs only in the per-pass compiled kernels (specialize=True, no prefetch)")
+    if cfg.half_lds and (cfg.prefetch or not cfg.specialize):
+        raise ValueError("half_lds runs only in the per-pass compiled kernels (specialize=True, no prefetch)")
     tb = min(cfg.tb, n - cfg.rb)
+    half = cfg.half_lds and not grad
     cfg = TileConfig(tb=tb, low_bits=max(1, cfg.low_bits), max_cost=cfg.max_cost,
                      prefetch=cfg.prefetch and not grad, specialize=cfg.specialize, fuse2q=cfg.fuse2q,
-                     io_lane_bits=cfg.io_lane_bits, rb=cfg.rb)
+                     io_lane_bits=cfg.io_lane_bits, rb=cfg.rb, half_lds=half)
     passes = partition(ops, n, cfg)
     for p in passes:
         if p.kind == PASS_TILE:
-            _stage_ops(p, cfg.io_lane_bits)
+            _stage_ops(p, cfg.io_lane_bits, keep_shared=half)
+            if half and all(set(a.rbits) & set(b.rbits) for a, b in zip(p.stages, p.stages[1:])):
+                p.flags |= PASSF_HALF_LDS
     prog = encode(n, passes, grad, n_params)
     prog.specialize = bool(cfg.specialize)
     return prog

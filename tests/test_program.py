@@ -316,3 +316,97 @@ def test_stage_tables_cover_each_tile_once():
                         seen_l.add(P.lds_swz(tl) ^ int(st['lsw'][j]))
                 assert seen_g == tile
                 assert seen_l == set(range(1 << L))
+
+
+def _thr_local(t, rsorted):
+    for r in rsorted:
+        r = int(r)
+        t = ((t >> r) << (r + 1)) | (t & ((1 << r) - 1))
+    return t
+
+
+def _run_half_exchange(lines, tb, rb, Pst, Sst, L):
+    """Executes one generated two-half exchange on the CPU for every thread:
+    thread t starts with register j = tile element thr_local_P(t) ^ loc_P(j)
+    and must end with element thr_local_S(t) ^ loc_S(j)."""
+    import re
+    h = None
+    for ln in lines:
+        m = re.search(r"& (\d+)\) \| \(\(thr_local<\d+>\(st\[\d+\], threadIdx.x\) >> (\d+)\)", ln)
+        if m:
+            h = int(m.group(2)) - 1
+    mask = (1 << h) - 1
+
+    def slot_thread(tl):
+        return P.lds_swz((tl & mask) | ((tl >> (h + 1)) << h))
+
+    def loc(rbits, j):
+        return sum(((j >> k) & 1) << int(rbits[k]) for k in range(rb))
+
+    nt = 1 << tb
+    v = [[_thr_local(t, Pst['rsorted'][:rb]) ^ loc(Pst['rbits'], j) for j in range(1 << rb)] for t in range(nt)]
+    w = [[None] * (1 << rb) for _ in range(nt)]
+    lds = [None] * (1 << (L - 1))
+    for ln in lines:
+        m = re.match(r"\s+lds\[tp \^ (\d+)\] = v\[(\d+)\];", ln)
+        if m:
+            c, j = int(m.group(1)), int(m.group(2))
+            for t in range(nt):
+                i = slot_thread(_thr_local(t, Pst['rsorted'][:rb])) ^ c
+                assert 0 <= i < len(lds)
+                lds[i] = v[t][j]
+            continue
+        m = re.match(r"\s+w\[(\d+)\] = lds\[tn \^ (\d+)\];", ln)
+        if m:
+            j, c = int(m.group(1)), int(m.group(2))
+            for t in range(nt):
+                i = slot_thread(_thr_local(t, Sst['rsorted'][:rb])) ^ c
+                w[t][j] = lds[i]
+            continue
+        if "__syncthreads" in ln:
+            continue
+    for t in range(nt):
+        for j in range(1 << rb):
+            assert w[t][j] == _thr_local(t, Sst['rsorted'][:rb]) ^ loc(Sst['rbits'], j), (t, j)
+
+
+@pytest.mark.parametrize("n,tb,rb", [(16, 6, 5), (14, 5, 4), (15, 6, 3)])
+def test_half_lds_exchanges_move_every_element(n, tb, rb):
+    """Every generated two-half LDS exchange (half_lds passes) delivers each
+    tile element to the register the next layout expects, through slots
+    inside the half-size LDS buffer, and both halves use the shared bit."""
+    from qaravan_amd import specialize as SP
+    circ = brickwork_circuit(n, 6)
+    cfg = P.TileConfig(tb=tb, low_bits=4, max_cost=1e9, io_lane_bits=4, rb=rb, specialize=True, half_lds=True)
+    prog = compile_circ(circ, cfg)
+    h, passes, stages, ops, _, _ = prog_interp.decode(prog.blob)
+    checked = 0
+    for p in passes:
+        if p['kind'] != P.PASS_TILE or not (int(p['flags']) & P.PASSF_HALF_LDS):
+            continue
+        s0, ns = int(p['stage_begin']), int(p['n_stages'])
+        for s in range(1, ns):
+            Pst, Sst = stages[s0 + s - 1], stages[s0 + s]
+            lines = SP._half_exchange("float", rb, Pst['rbits'], Sst['rbits'], s == 1, s - 1, s)
+            _run_half_exchange(lines, int(p['tb']), rb, Pst, Sst, int(p['L']))
+            checked += 1
+    assert checked > 0
+
+
+def test_half_lds_flag_and_shared_bits():
+    circ = brickwork_circuit(16, 8)
+    cfg = P.TileConfig(tb=6, low_bits=4, max_cost=160, io_lane_bits=4, rb=5, specialize=True, half_lds=True)
+    prog = compile_circ(circ, cfg)
+    nflag = 0
+    for p in prog.passes:
+        if p.kind != P.PASS_TILE:
+            continue
+        shared = all(set(a.rbits) & set(b.rbits) for a, b in zip(p.stages, p.stages[1:]))
+        assert bool(p.flags & P.PASSF_HALF_LDS) == shared
+        nflag += shared
+        # first and last layouts keep the 4 lane bits off the registers
+        assert not set(p.stages[0].rbits) & set(range(4))
+        assert not set(p.stages[-1].rbits) & set(range(4))
+    assert nflag > 0
+    with pytest.raises(ValueError):
+        compile_circ(circ, P.TileConfig(tb=6, low_bits=4, rb=4, half_lds=True))
