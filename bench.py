@@ -584,7 +584,7 @@ def main():
                    "tile": {"tb": fwd["cfg"].tb, "low_bits": fwd["cfg"].low_bits,
                             "max_cost": fwd["cfg"].max_cost, "fuse2q": fwd["cfg"].fuse2q,
                             "io_lane_bits": fwd["cfg"].io_lane_bits, "register_bits": fwd["cfg"].rb,
-                            "half_lds": fwd["cfg"].half_lds},
+                            "lds_split": fwd["cfg"].lds_split},
                    "fused_passes": fwd["n_tile"]},
         "gates_per_s": round(fwd["gates_per_s"], 1),
         "c64_vs_c128_l2_err": err,

@@ -85,9 +85,11 @@ enum {
      * tile-local bit 0 = address bit 0. */
     SVH_PASSF_PREFETCH = 1,
     /* TILE, per-pass compiled forward kernels: every LDS re-distribution moves
-     * the tile in two halves, split on a register bit both layouts share, so
-     * the pass needs LDS for 2^(L-1) amplitudes instead of 2^L. */
-    SVH_PASSF_HALF_LDS = 2
+     * the tile in two halves (HALF) or four quarters (QUARTER), split on
+     * register bits both layouts share, so the pass needs LDS for 2^(L-1) or
+     * 2^(L-2) amplitudes instead of 2^L. */
+    SVH_PASSF_HALF_LDS = 2,
+    SVH_PASSF_QUARTER_LDS = 4
 };
 
 /* A stage's layout: thread t holds registers j = 0..2^rb-1 at tile-local index

@@ -36,6 +36,7 @@ ABI_VERSION = 1
 PASS_TILE, PASS_DENSE, PASS_NORMALIZE = 1, 2, 3
 PASSF_PREFETCH = 1
 PASSF_HALF_LDS = 2
+PASSF_QUARTER_LDS = 4
 OP_MAT1, OP_MAT2, OP_CX, OP_CZ, OP_ROT1, OP_ROT2 = 1, 2, 3, 4, 5, 6
 OPF_NEG, OPF_GRAD = 1, 2
 RB = 4
@@ -320,10 +321,11 @@ class TileConfig:
               fewer LDS re-distributions per pass); 3 in the per-pass compiled
               forward and adjoint kernels (8 amplitudes per thread: twice the
               waves for small states).
-    half_lds: per-pass compiled forward kernels move the tile through LDS in
-              two halves, split on a register bit that the two layouts of a
-              re-distribution share: half the LDS per workgroup, so more
-              workgroups per CU (the stage planner keeps such a bit).
+    lds_split: per-pass compiled forward kernels move the tile through LDS
+              in 2^lds_split parts (0, 1 or 2), split on register bits that
+              the two layouts of a re-distribution share: 1/2^lds_split of the
+              LDS per workgroup, so more workgroups per CU (the stage planner
+              keeps that many shared bits).
     """
     tb: int = 8
     low_bits: int = 6
@@ -333,7 +335,7 @@ class TileConfig:
     fuse2q: bool = True
     io_lane_bits: int = 6
     rb: int = RB
-    half_lds: bool = False
+    lds_split: int = 0
 
 
 def _is_barrier(op):
@@ -424,10 +426,10 @@ def partition(ops, n, cfg):
     return passes
 
 
-def _stage_ops(p, io_lane_bits=6, keep_shared=False):
+def _stage_ops(p, io_lane_bits=6, keep_shared=0):
     """Split a tile pass's ops into register stages (tile-local bits).
-    keep_shared: every stage keeps at least one register bit of the stage
-    before it where the ops leave a free register slot (half-LDS exchanges)."""
+    keep_shared: every stage keeps at least this many register bits of the
+    stage before it (split LDS exchanges); the ops of a stage leave room."""
     RB = p.rb
     L = p.tb + RB
     loc = {g: i for i, g in enumerate(p.bits)}
@@ -457,16 +459,20 @@ def _stage_ops(p, io_lane_bits=6, keep_shared=False):
                 blocked.add(op, s)
                 continue
             new = [x for x in lb if x not in R]
-            # keep_shared: the last free slot is for a bit of the previous layout
-            room = RB - (1 if keep_shared and cur is not None and not set(R + new) & set(cur.rbits) else 0)
+            # keep_shared: the last free slots are for bits of the previous layout
+            room = RB - (max(0, keep_shared - len(set(R + new) & set(cur.rbits))) if cur is not None else 0)
             if len(R) + len(new) <= room:
                 R += new
             else:
                 blocked.add(op, s)
             if len(R) == RB:
                 break
-        if keep_shared and cur is not None and len(R) < RB and not set(R) & set(cur.rbits):
-            R.append(max(cur.rbits))            # a bit both layouts hold
+        if cur is not None:                     # bits both layouts hold
+            for x in sorted(cur.rbits, reverse=True):
+                if len(set(R) & set(cur.rbits)) >= keep_shared or len(R) == RB:
+                    break
+                if x not in R:
+                    R.append(x)
         for x in reversed(range(L)):            # fill with high tile bits
             if len(R) == RB:
                 break
@@ -678,18 +684,23 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
                          "no prefetch, no adjoint)")
     if cfg.rb == 3 and (cfg.prefetch or not cfg.specialize):
         raise ValueError("rb=3 runs only in the per-pass compiled kernels (specialize=True, no prefetch)")
-    if cfg.half_lds and (cfg.prefetch or not cfg.specialize):
-        raise ValueError("half_lds runs only in the per-pass compiled kernels (specialize=True, no prefetch)")
+    if cfg.lds_split and (cfg.prefetch or not cfg.specialize):
+        raise ValueError("lds_split runs only in the per-pass compiled kernels (specialize=True, no prefetch)")
+    if cfg.lds_split not in (0, 1, 2) or cfg.lds_split >= cfg.rb:
+        raise ValueError("lds_split must be 0, 1 or 2 and below the register bits")
     tb = min(cfg.tb, n - cfg.rb)
-    half = cfg.half_lds and not grad
+    split = 0 if grad else cfg.lds_split
     cfg = TileConfig(tb=tb, low_bits=max(1, cfg.low_bits), max_cost=cfg.max_cost,
                      prefetch=cfg.prefetch and not grad, specialize=cfg.specialize, fuse2q=cfg.fuse2q,
-                     io_lane_bits=cfg.io_lane_bits, rb=cfg.rb, half_lds=half)
+                     io_lane_bits=cfg.io_lane_bits, rb=cfg.rb, lds_split=split)
     passes = partition(ops, n, cfg)
     for p in passes:
         if p.kind == PASS_TILE:
-            _stage_ops(p, cfg.io_lane_bits, keep_shared=half)
-            if half and all(set(a.rbits) & set(b.rbits) for a, b in zip(p.stages, p.stages[1:])):
+            _stage_ops(p, cfg.io_lane_bits, keep_shared=split)
+            shared = min((len(set(a.rbits) & set(b.rbits)) for a, b in zip(p.stages, p.stages[1:])), default=RB)
+            if split >= 2 and shared >= 2:
+                p.flags |= PASSF_QUARTER_LDS
+            elif split >= 1 and shared >= 1:
                 p.flags |= PASSF_HALF_LDS
     prog = encode(n, passes, grad, n_params)
     prog.specialize = bool(cfg.specialize)

@@ -42,58 +42,76 @@ def _decode(blob):
 def lds_bytes(p, dtype_code, dual):
     es = 8 if dtype_code == 0 else 16
     nw = max(1, (2 ** int(p['tb'])) // 64)
-    amps = 2 ** (int(p['L']) - (1 if _half(p, dual) else 0))
+    amps = 2 ** (int(p['L']) - lds_split(p, dual))
     return (2 if dual else 1) * amps * es + (8 * nw * int(p['n_grad']) if dual else 0)
 
 
-def _half(p, dual):
-    """Forward pass whose LDS re-distributions go in two halves."""
-    return not dual and bool(int(p['flags']) & P.PASSF_HALF_LDS)
+def lds_split(p, dual):
+    """log2 of the parts a forward pass's LDS re-distributions go in."""
+    if dual:
+        return 0
+    f = int(p['flags'])
+    return 1 if f & P.PASSF_HALF_LDS else 2 if f & P.PASSF_QUARTER_LDS else 0
 
 
 def _lds_swz(x):
     return x ^ ((x >> 5) & 31)
 
 
-def _half_exchange(R, rb, P_rbits, S_rbits, first, sp, sc):
-    """One LDS re-distribution from stage sp's layout (register bits P_rbits)
-    to stage sc's (S_rbits) in two halves.  The halves split on h, the highest
-    tile-local bit that both layouts hold in registers: half 0 moves the
-    elements with bit h = 0, half 1 the others, each through the same 2^(L-1)
-    LDS slots at swizzle(index with bit h removed).  The removal and the
-    swizzle are linear over GF(2), so a slot is (thread part) ^ (constant per
-    register); the constants are computed here."""
-    P_rbits, S_rbits = [int(x) for x in P_rbits[:rb]], [int(x) for x in S_rbits[:rb]]
-    h = max(set(P_rbits) & set(S_rbits))
-    jp, js = P_rbits.index(h), S_rbits.index(h)
-    mask = (1 << h) - 1
+def split_bits(P_rbits, S_rbits, rb, k):
+    """The k highest tile-local bits both layouts hold in registers, highest first."""
+    common = set(int(x) for x in P_rbits[:rb]) & set(int(x) for x in S_rbits[:rb])
+    return sorted(common, reverse=True)[:k]
 
-    def drop(x):
-        return (x & mask) | ((x >> (h + 1)) << h)
+
+def drop_bits(x, hs):
+    """x with the bits hs (highest first) removed, the bits above shifted down."""
+    for h in hs:
+        x = (x & ((1 << h) - 1)) | ((x >> (h + 1)) << h)
+    return x
+
+
+def _split_exchange(R, rb, P_rbits, S_rbits, first, sp, sc, k):
+    """One LDS re-distribution from stage sp's layout (register bits P_rbits)
+    to stage sc's (S_rbits) in 2^k parts.  The parts split on hs, the k
+    highest tile-local bits that both layouts hold in registers: part q moves
+    the elements whose bits hs read q, each part through the same 2^(L-k) LDS
+    slots at swizzle(index with the bits hs removed).  Removal and swizzle are
+    linear over GF(2), so a slot is (thread part) ^ (constant per register);
+    the constants are computed here."""
+    P_rbits, S_rbits = [int(x) for x in P_rbits[:rb]], [int(x) for x in S_rbits[:rb]]
+    hs = split_bits(P_rbits, S_rbits, rb, k)
+    jp = [P_rbits.index(h) for h in hs]
+    js = [S_rbits.index(h) for h in hs]
 
     def roff(rbits, j, skip):
         x = 0
-        for k in range(rb):
-            if k != skip and (j >> k) & 1:
-                x |= 1 << rbits[k]
-        return _lds_swz(drop(x))
+        for kk in range(rb):
+            if kk not in skip and (j >> kk) & 1:
+                x |= 1 << rbits[kk]
+        return _lds_swz(drop_bits(x, hs))
+
+    def part(j, idx):
+        return sum(((j >> b) & 1) << i for i, b in enumerate(idx))
 
     def tpart(s):
-        t = f"thr_local<{rb}>(st[{s}], threadIdx.x)"
-        return f"lds_swz(({t} & {mask}) | (({t} >> {h + 1}) << {h}))"
+        x = f"thr_local<{rb}>(st[{s}], threadIdx.x)"
+        for h in hs:
+            x = f"(({x}) & {(1 << h) - 1}) | ((({x}) >> {h + 1}) << {h})"
+        return f"lds_swz({x})"
 
     out = ["    {",
            f"        const int tp = {tpart(sp)};",
            f"        const int tn = {tpart(sc)};",
            f"        cpx<{R}> w[{1 << rb}];"]
-    for half in (0, 1):
-        if not (first and half == 0):
+    for q in range(1 << len(hs)):
+        if not (first and q == 0):
             out.append("        __syncthreads();")
         out += [f"        lds[tp ^ {roff(P_rbits, j, jp)}] = v[{j}];"
-                for j in range(1 << rb) if (j >> jp) & 1 == half]
+                for j in range(1 << rb) if part(j, jp) == q]
         out.append("        __syncthreads();")
         out += [f"        w[{j}] = lds[tn ^ {roff(S_rbits, j, js)}];"
-                for j in range(1 << rb) if (j >> js) & 1 == half]
+                for j in range(1 << rb) if part(j, js) == q]
     out += [f"        for (int j = 0; j < {1 << rb}; ++j) v[j] = w[j];", "    }"]
     return out
 
@@ -130,8 +148,9 @@ def _body(R, p, stages, ops, dual):
         S = stages[s0 + s]
         if s > 0:
             first = "true" if s == 1 else "false"
-            if _half(p, dual):
-                out += _half_exchange(R, rb, stages[s0 + s - 1]['rbits'], S['rbits'], s == 1, s - 1, s)
+            if lds_split(p, dual):
+                out += _split_exchange(R, rb, stages[s0 + s - 1]['rbits'], S['rbits'], s == 1, s - 1, s,
+                                       lds_split(p, dual))
             else:
                 out.append(f"    sp_exchange2<{R}, {first}, {rb}>(lds, lds2, st[{s - 1}], st[{s}], v, w);" if dual
                            else f"    sp_exchange<{R}, {first}, {rb}>(lds, st[{s - 1}], st[{s}], v);")

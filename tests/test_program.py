@@ -325,28 +325,28 @@ def _thr_local(t, rsorted):
     return t
 
 
-def _run_half_exchange(lines, tb, rb, Pst, Sst, L):
-    """Executes one generated two-half exchange on the CPU for every thread:
+def _run_split_exchange(lines, tb, rb, Pst, Sst, L, hs):
+    """Executes one generated split exchange on the CPU for every thread:
     thread t starts with register j = tile element thr_local_P(t) ^ loc_P(j)
-    and must end with element thr_local_S(t) ^ loc_S(j)."""
+    and must end with element thr_local_S(t) ^ loc_S(j); every slot lies in
+    the 2^(L-k) LDS buffer.  (The thread part of a slot follows the same
+    formula as the generated code: swizzle of the thread index with the split
+    bits removed.)"""
     import re
-    h = None
-    for ln in lines:
-        m = re.search(r"& (\d+)\) \| \(\(thr_local<\d+>\(st\[\d+\], threadIdx.x\) >> (\d+)\)", ln)
-        if m:
-            h = int(m.group(2)) - 1
-    mask = (1 << h) - 1
+    from qaravan_amd import specialize as SP
 
     def slot_thread(tl):
-        return P.lds_swz((tl & mask) | ((tl >> (h + 1)) << h))
+        return P.lds_swz(SP.drop_bits(tl, hs))
 
     def loc(rbits, j):
         return sum(((j >> k) & 1) << int(rbits[k]) for k in range(rb))
 
+    for h in hs:
+        assert f">> {h + 1}) << {h})" in lines[1]
     nt = 1 << tb
     v = [[_thr_local(t, Pst['rsorted'][:rb]) ^ loc(Pst['rbits'], j) for j in range(1 << rb)] for t in range(nt)]
     w = [[None] * (1 << rb) for _ in range(nt)]
-    lds = [None] * (1 << (L - 1))
+    lds = [None] * (1 << (L - len(hs)))
     for ln in lines:
         m = re.match(r"\s+lds\[tp \^ (\d+)\] = v\[(\d+)\];", ln)
         if m:
@@ -363,50 +363,56 @@ def _run_half_exchange(lines, tb, rb, Pst, Sst, L):
                 i = slot_thread(_thr_local(t, Sst['rsorted'][:rb])) ^ c
                 w[t][j] = lds[i]
             continue
-        if "__syncthreads" in ln:
-            continue
     for t in range(nt):
         for j in range(1 << rb):
             assert w[t][j] == _thr_local(t, Sst['rsorted'][:rb]) ^ loc(Sst['rbits'], j), (t, j)
 
 
-@pytest.mark.parametrize("n,tb,rb", [(16, 6, 5), (14, 5, 4), (15, 6, 3)])
-def test_half_lds_exchanges_move_every_element(n, tb, rb):
-    """Every generated two-half LDS exchange (half_lds passes) delivers each
+@pytest.mark.parametrize("n,tb,rb,k", [(16, 6, 5, 1), (14, 5, 4, 1), (15, 6, 3, 1), (16, 6, 5, 2), (15, 5, 4, 2)])
+def test_split_lds_exchanges_move_every_element(n, tb, rb, k):
+    """Every generated split LDS exchange (lds_split passes) delivers each
     tile element to the register the next layout expects, through slots
-    inside the half-size LDS buffer, and both halves use the shared bit."""
+    inside the 1/2^k-size LDS buffer."""
     from qaravan_amd import specialize as SP
     circ = brickwork_circuit(n, 6)
-    cfg = P.TileConfig(tb=tb, low_bits=4, max_cost=1e9, io_lane_bits=4, rb=rb, specialize=True, half_lds=True)
+    cfg = P.TileConfig(tb=tb, low_bits=4, max_cost=1e9, io_lane_bits=4, rb=rb, specialize=True, lds_split=k)
     prog = compile_circ(circ, cfg)
     h, passes, stages, ops, _, _ = prog_interp.decode(prog.blob)
     checked = 0
     for p in passes:
-        if p['kind'] != P.PASS_TILE or not (int(p['flags']) & P.PASSF_HALF_LDS):
+        if p['kind'] != P.PASS_TILE or SP.lds_split(p, False) != k:
             continue
         s0, ns = int(p['stage_begin']), int(p['n_stages'])
         for s in range(1, ns):
             Pst, Sst = stages[s0 + s - 1], stages[s0 + s]
-            lines = SP._half_exchange("float", rb, Pst['rbits'], Sst['rbits'], s == 1, s - 1, s)
-            _run_half_exchange(lines, int(p['tb']), rb, Pst, Sst, int(p['L']))
+            hs = SP.split_bits(Pst['rbits'], Sst['rbits'], rb, k)
+            assert len(hs) == k
+            lines = SP._split_exchange("float", rb, Pst['rbits'], Sst['rbits'], s == 1, s - 1, s, k)
+            assert sum("__syncthreads" in ln for ln in lines) == 2 * (1 << k) - (s == 1)
+            _run_split_exchange(lines, int(p['tb']), rb, Pst, Sst, int(p['L']), hs)
             checked += 1
     assert checked > 0
 
 
-def test_half_lds_flag_and_shared_bits():
+@pytest.mark.parametrize("k", [1, 2])
+def test_lds_split_flag_and_shared_bits(k):
     circ = brickwork_circuit(16, 8)
-    cfg = P.TileConfig(tb=6, low_bits=4, max_cost=160, io_lane_bits=4, rb=5, specialize=True, half_lds=True)
+    cfg = P.TileConfig(tb=6, low_bits=4, max_cost=160, io_lane_bits=4, rb=5, specialize=True, lds_split=k)
     prog = compile_circ(circ, cfg)
     nflag = 0
     for p in prog.passes:
         if p.kind != P.PASS_TILE:
             continue
-        shared = all(set(a.rbits) & set(b.rbits) for a, b in zip(p.stages, p.stages[1:]))
-        assert bool(p.flags & P.PASSF_HALF_LDS) == shared
-        nflag += shared
+        shared = min((len(set(a.rbits) & set(b.rbits)) for a, b in zip(p.stages, p.stages[1:])), default=5)
+        want = min(k, shared)   # a pass that cannot keep k shared bits splits in fewer parts
+        got = 1 if p.flags & P.PASSF_HALF_LDS else 2 if p.flags & P.PASSF_QUARTER_LDS else 0
+        assert got == want
+        nflag += got == k
         # first and last layouts keep the 4 lane bits off the registers
         assert not set(p.stages[0].rbits) & set(range(4))
         assert not set(p.stages[-1].rbits) & set(range(4))
     assert nflag > 0
     with pytest.raises(ValueError):
-        compile_circ(circ, P.TileConfig(tb=6, low_bits=4, rb=4, half_lds=True))
+        compile_circ(circ, P.TileConfig(tb=6, low_bits=4, rb=4, lds_split=k))
+    with pytest.raises(ValueError):
+        compile_circ(circ, P.TileConfig(tb=6, low_bits=4, rb=3, specialize=True, lds_split=3))
