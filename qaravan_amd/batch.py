@@ -20,7 +20,7 @@ import torch
 
 from . import _lib
 from .adjoint import ParamProgram
-from .program import HDR_DT, PASS_DT, PASS_TILE, RB, TileConfig
+from .program import HDR_DT, PASS_DT, PASS_TILE, PASSF_HALF_LDS, PASSF_QUARTER_LDS, RB, TileConfig
 from .statevector_sim import _pauli_masks
 
 ITEM_DT = np.dtype([('psi', '<u8'), ('lam', '<u8'), ('pass', '<u8'), ('stages', '<u8'), ('ops', '<u8'),
@@ -55,14 +55,18 @@ class BatchedAdjoint:
     circuits (rotations carry ``param_index`` into their own θ_c)."""
 
     def __init__(self, circuits, terms_list, n_params_list, device="cuda", dtype=torch.complex64,
-                 tb_of_n=None, max_cost=1e9, keys=None, op_lists=None, ns=None, specialize=False):
+                 tb_of_n=None, max_cost=1e9, keys=None, op_lists=None, ns=None, specialize=False, lds_split=0):
         """``circuits``: qaravan-style circuits (rotations carry ``param_index``),
         or None with ``op_lists`` (elementary ops) and ``ns``.  Circuits with equal
         ``keys`` share compiled programs (θ lives on the device, so programs
         depend on the structure only).  ``specialize``: run every pass with a
         kernel compiled for its structure (one launch per structure and round)
-        instead of the op-interpreting batch kernel."""
+        instead of the op-interpreting batch kernel.  ``lds_split`` (with
+        ``specialize``): the compiled kernels move tiles through LDS in
+        2^lds_split parts, so more workgroups fit on a CU."""
         _lib.require_gpu()
+        if lds_split and not specialize:
+            raise ValueError("lds_split needs the compiled batch kernels (specialize=True)")
         self.device = torch.device(device)
         self.dtype = dtype
         self.code = _lib.dtype_code(dtype)
@@ -80,7 +84,8 @@ class BatchedAdjoint:
                 self.pps.append(cache[key])
                 continue
             n = self.ns[c]
-            cfg = TileConfig(tb=tb_of_n(n), low_bits=4, max_cost=max_cost, io_lane_bits=4)
+            cfg = TileConfig(tb=tb_of_n(n), low_bits=4, max_cost=max_cost, io_lane_bits=4,
+                             specialize=bool(lds_split), lds_split=lds_split)
             if circuits is not None:
                 pp = ParamProgram(circuits[c], terms_list[c], n_params_list[c], cfg=cfg, dtype=dtype)
             else:
@@ -197,7 +202,8 @@ class BatchedAdjoint:
                     ppass, pst, pops, pgrad = self._pass_ptrs(self.prog_addr[c][which], prog, r)
                     ng = sum(1 for st in p.stages for op, _ in st.ops if op.flags & 2) if which else 0
                     gkey = (tb, self.kname[id(prog)][r]) if self.kernels is not None else (tb, None)
-                    groups.setdefault(gkey, []).append(dict(
+                    split = 1 if p.flags & PASSF_HALF_LDS else 2 if p.flags & PASSF_QUARTER_LDS else 0
+                    groups.setdefault(gkey, []).append(dict(split=split,
                         psi=psi0 + es * int(self.amp_off[c]), lam=(lam0 + es * int(self.amp_off[c])) if which else 0,
                         pass_=ppass, stages=pst, ops=pops, trig=trig0 + es * int(self.par_off[c]),
                         gpart=gp0 + 8 * int(self.gp_off[c]), nwg=2 ** (n - L), n=n, ng=ng))
@@ -224,7 +230,8 @@ class BatchedAdjoint:
                     if kname is None:
                         self.launches.append(('tile', add_table(arr), len(items), wg, tb, which, mng))
                     else:
-                        L = tb + RB
+                        # one kernel text per group, so one split for all its items
+                        L = tb + RB - items[0]['split']
                         nw = max(1, 2 ** tb // 64)
                         lds = (2 if which else 1) * es * 2 ** L + (8 * nw * mng if which else 0)
                         self.launches.append(('spb', add_table(arr), len(items), wg, tb, lds,

@@ -625,7 +625,7 @@ static int launch_sp(hipFunction_t f, void *psi, void *lam, int n, int dtype, co
     const int L = hp.L;
     const size_t es = dtype == SVH_C64 ? 8 : 16;
     const size_t nw = ((size_t)1 << hp.tb) >= 64 ? ((size_t)1 << hp.tb) / 64 : 1;
-    const int split = lam ? 0 : (hp.flags & SVH_PASSF_HALF_LDS) ? 1 : (hp.flags & SVH_PASSF_QUARTER_LDS) ? 2 : 0;
+    const int split = (hp.flags & SVH_PASSF_HALF_LDS) ? 1 : (hp.flags & SVH_PASSF_QUARTER_LDS) ? 2 : 0;
     const size_t tile_amps = (size_t)1 << (L - split);
     const size_t lds = (lam ? 2 : 1) * es * tile_amps + (lam ? 8 * nw * (size_t)hp.n_grad : 0);
     if (lds > kMaxLds) return fail(-7, "specialised pass needs more than 160 KiB of LDS");

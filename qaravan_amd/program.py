@@ -321,7 +321,7 @@ class TileConfig:
               fewer LDS re-distributions per pass); 3 in the per-pass compiled
               forward and adjoint kernels (8 amplitudes per thread: twice the
               waves for small states).
-    lds_split: per-pass compiled forward kernels move the tile through LDS
+    lds_split: per-pass compiled kernels move the tile (and λ) through LDS
               in 2^lds_split parts (0, 1 or 2), split on register bits that
               the two layouts of a re-distribution share: 1/2^lds_split of the
               LDS per workgroup, so more workgroups per CU (the stage planner
@@ -689,7 +689,7 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
     if cfg.lds_split not in (0, 1, 2) or cfg.lds_split >= cfg.rb:
         raise ValueError("lds_split must be 0, 1 or 2 and below the register bits")
     tb = min(cfg.tb, n - cfg.rb)
-    split = 0 if grad else cfg.lds_split
+    split = cfg.lds_split
     cfg = TileConfig(tb=tb, low_bits=max(1, cfg.low_bits), max_cost=cfg.max_cost,
                      prefetch=cfg.prefetch and not grad, specialize=cfg.specialize, fuse2q=cfg.fuse2q,
                      io_lane_bits=cfg.io_lane_bits, rb=cfg.rb, lds_split=split)

@@ -46,10 +46,8 @@ def lds_bytes(p, dtype_code, dual):
     return (2 if dual else 1) * amps * es + (8 * nw * int(p['n_grad']) if dual else 0)
 
 
-def lds_split(p, dual):
-    """log2 of the parts a forward pass's LDS re-distributions go in."""
-    if dual:
-        return 0
+def lds_split(p, dual=False):
+    """log2 of the parts a pass's LDS re-distributions go in."""
     f = int(p['flags'])
     return 1 if f & P.PASSF_HALF_LDS else 2 if f & P.PASSF_QUARTER_LDS else 0
 
@@ -71,9 +69,10 @@ def drop_bits(x, hs):
     return x
 
 
-def _split_exchange(R, rb, P_rbits, S_rbits, first, sp, sc, k):
+def _split_exchange(R, rb, P_rbits, S_rbits, first, sp, sc, k, dual=False):
     """One LDS re-distribution from stage sp's layout (register bits P_rbits)
-    to stage sc's (S_rbits) in 2^k parts.  The parts split on hs, the k
+    to stage sc's (S_rbits) in 2^k parts (dual: ψ in v through lds and λ in
+    w through lds2, slot for slot).  The parts split on hs, the k
     highest tile-local bits that both layouts hold in registers: part q moves
     the elements whose bits hs read q, each part through the same 2^(L-k) LDS
     slots at swizzle(index with the bits hs removed).  Removal and swizzle are
@@ -100,19 +99,24 @@ def _split_exchange(R, rb, P_rbits, S_rbits, first, sp, sc, k):
             x = f"(({x}) & {(1 << h) - 1}) | ((({x}) >> {h + 1}) << {h})"
         return f"lds_swz({x})"
 
+    arrays = [("v", "lds", "x"), ("w", "lds2", "y")] if dual else [("v", "lds", "x")]
     out = ["    {",
            f"        const int tp = {tpart(sp)};",
-           f"        const int tn = {tpart(sc)};",
-           f"        cpx<{R}> w[{1 << rb}];"]
+           f"        const int tn = {tpart(sc)};"]
+    out += [f"        cpx<{R}> {t}[{1 << rb}];" for _, _, t in arrays]
     for q in range(1 << len(hs)):
         if not (first and q == 0):
             out.append("        __syncthreads();")
-        out += [f"        lds[tp ^ {roff(P_rbits, j, jp)}] = v[{j}];"
-                for j in range(1 << rb) if part(j, jp) == q]
+        for reg, buf, _ in arrays:
+            out += [f"        {buf}[tp ^ {roff(P_rbits, j, jp)}] = {reg}[{j}];"
+                    for j in range(1 << rb) if part(j, jp) == q]
         out.append("        __syncthreads();")
-        out += [f"        w[{j}] = lds[tn ^ {roff(S_rbits, j, js)}];"
-                for j in range(1 << rb) if part(j, js) == q]
-    out += [f"        for (int j = 0; j < {1 << rb}; ++j) v[j] = w[j];", "    }"]
+        for _, buf, t in arrays:
+            out += [f"        {t}[{j}] = {buf}[tn ^ {roff(S_rbits, j, js)}];"
+                    for j in range(1 << rb) if part(j, js) == q]
+    for reg, _, t in arrays:
+        out.append(f"        for (int j = 0; j < {1 << rb}; ++j) {reg}[j] = {t}[j];")
+    out.append("    }")
     return out
 
 
@@ -135,8 +139,8 @@ def _body(R, p, stages, ops, dual):
     out = ["    extern __shared__ unsigned char smem_raw[];",
            f"    cpx<{R}> *lds = reinterpret_cast<cpx<{R}> *>(smem_raw);"]
     if dual:
-        out += [f"    cpx<{R}> *lds2 = lds + {2 ** L};",
-                f"    double *gacc = reinterpret_cast<double *>(lds2 + {2 ** L});",
+        out += [f"    cpx<{R}> *lds2 = lds + {2 ** (L - lds_split(p, dual))};",
+                f"    double *gacc = reinterpret_cast<double *>(lds2 + {2 ** (L - lds_split(p, dual))});",
                 f"    cpx<{R}> v[{1 << rb}], w[{1 << rb}];",
                 f"    const uint64_t base = sp_base_wg<{R}, {tb}, {rb}>(pass, n, wg);",
                 f"    sp_load2<{R}, {tb}, {rb}>(psi, lam, base, st[0], v, w);"]
@@ -150,7 +154,7 @@ def _body(R, p, stages, ops, dual):
             first = "true" if s == 1 else "false"
             if lds_split(p, dual):
                 out += _split_exchange(R, rb, stages[s0 + s - 1]['rbits'], S['rbits'], s == 1, s - 1, s,
-                                       lds_split(p, dual))
+                                       lds_split(p, dual), dual)
             else:
                 out.append(f"    sp_exchange2<{R}, {first}, {rb}>(lds, lds2, st[{s - 1}], st[{s}], v, w);" if dual
                            else f"    sp_exchange<{R}, {first}, {rb}>(lds, st[{s - 1}], st[{s}], v);")

@@ -24,18 +24,20 @@ def hea_spec(n, layers):
     return spec
 
 
-@pytest.mark.parametrize("n,layers,tb,spz,rb", [(8, 2, 4, False, 4), (10, 3, 6, False, 4), (12, 2, 8, False, 4),
-                                              (10, 3, 6, True, 4), (12, 2, 5, True, 4), (11, 3, 7, True, 3),
-                                              (12, 4, 9, True, 3)])
+@pytest.mark.parametrize("n,layers,tb,spz,rb,split", [(8, 2, 4, False, 4, 0), (10, 3, 6, False, 4, 0),
+                                                    (12, 2, 8, False, 4, 0), (10, 3, 6, True, 4, 0),
+                                                    (12, 2, 5, True, 4, 0), (11, 3, 7, True, 3, 0),
+                                                    (12, 4, 9, True, 3, 0), (10, 3, 6, True, 4, 1),
+                                                    (12, 4, 9, True, 3, 1), (12, 2, 7, True, 4, 2)])
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
-def test_adjoint_vs_oracle(gpu, n, layers, tb, spz, rb, dtype):
+def test_adjoint_vs_oracle(gpu, n, layers, tb, spz, rb, split, dtype):
     from qaravan_amd.adjoint import AdjointModel
     theta = hea_theta(n, layers)
     circ = hea_circuit(n, layers, theta)
     terms = tfim_terms(n)
-    print(f"adjoint n={n} layers={layers} tb={tb} rb={rb} specialize={spz} {dtype}", flush=True)
+    print(f"adjoint n={n} layers={layers} tb={tb} rb={rb} specialize={spz} lds_split={split} {dtype}", flush=True)
     model = AdjointModel(circ, terms, len(theta), device=gpu, dtype=dtype,
-                         cfg=P.TileConfig(tb=tb, low_bits=6, max_cost=48, specialize=spz, rb=rb))
+                         cfg=P.TileConfig(tb=tb, low_bits=6, max_cost=48, specialize=spz, rb=rb, lds_split=split))
     th = torch.tensor(theta, device=gpu, requires_grad=True)
     e = model(th)
     e.backward()

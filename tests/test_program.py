@@ -356,7 +356,7 @@ def _run_split_exchange(lines, tb, rb, Pst, Sst, L, hs):
                 assert 0 <= i < len(lds)
                 lds[i] = v[t][j]
             continue
-        m = re.match(r"\s+w\[(\d+)\] = lds\[tn \^ (\d+)\];", ln)
+        m = re.match(r"\s+x\[(\d+)\] = lds\[tn \^ (\d+)\];", ln)
         if m:
             j, c = int(m.group(1)), int(m.group(2))
             for t in range(nt):
@@ -390,6 +390,13 @@ def test_split_lds_exchanges_move_every_element(n, tb, rb, k):
             lines = SP._split_exchange("float", rb, Pst['rbits'], Sst['rbits'], s == 1, s - 1, s, k)
             assert sum("__syncthreads" in ln for ln in lines) == 2 * (1 << k) - (s == 1)
             _run_split_exchange(lines, int(p['tb']), rb, Pst, Sst, int(p['L']), hs)
+            # the adjoint form moves λ (w, lds2, y) through the same slots at the same steps
+            dual = SP._split_exchange("float", rb, Pst['rbits'], Sst['rbits'], s == 1, s - 1, s, k, dual=True)
+            single = [ln for ln in dual if "lds2" not in ln and "y[" not in ln and " w[" not in ln]
+            assert single == lines
+            mirrored = [ln.replace("lds2", "lds").replace("= w[", "= v[").replace("y[", "x[")
+                        for ln in dual if "lds2" in ln]
+            assert mirrored == [ln for ln in lines if "lds[" in ln]
             checked += 1
     assert checked > 0
 
