@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-measure", action="store_true")
     ap.add_argument("--batch-interp", action="store_true", help="batch leg with the op-interpreting kernels")
+    ap.add_argument("--batch-lds-split", type=int, default=BATCH_LDS_SPLIT,
+                    help="batch leg: LDS re-distributions in 2^K parts (compiled kernels)")
     ap.add_argument("--profile-forward", action="store_true",
                     help="only the timed forward leg (for rocprofv3 runs)")
     ap.add_argument("--profile-blocks", action="store_true", help="only the MFMA block leg")
@@ -56,6 +58,10 @@ def parse():
     ap.add_argument("--n", type=int, default=28)
     ap.add_argument("--layers", type=int, default=40)
     return ap.parse_args()
+
+
+# LDS parts of the compiled batch kernels (log2), chosen by measurement
+BATCH_LDS_SPLIT = 0
 
 
 def log(*a):
@@ -372,7 +378,7 @@ def bench_measure(dev, n=28, reps=5, shots=1_000_000):
     return out
 
 
-def bench_batch(dev, nc=512, seed=20260004, check=6, specialize=True):
+def bench_batch(dev, nc=512, seed=20260004, check=6, specialize=True, lds_split=0):
     """512 HEA circuits (10..22 qubits, 4..32 layers) forward + adjoint as one
     HIP-graph launch; latency from device timestamps."""
     import numpy as np
@@ -386,7 +392,7 @@ def bench_batch(dev, nc=512, seed=20260004, check=6, specialize=True):
     b = BatchedAdjoint(None, [tfim_terms(int(n)) for n in ns], [len(t) for t in thetas], device=dev,
                        dtype=torch.complex64, keys=list(zip(ns.tolist(), ls.tolist())),
                        op_lists=[hea_ops(int(n), int(l)) for n, l in zip(ns, ls)], ns=ns.tolist(),
-                       specialize=specialize)
+                       specialize=specialize, lds_split=lds_split if specialize else 0)
     log(f"[bench] batch: planned {nc} circuits in {time.perf_counter() - t0:.1f} s, {len(b.launches)} launches")
     b.set_theta(thetas)
     b.capture()
@@ -551,7 +557,7 @@ def main():
         log(f"[bench] fused blocks: {blocks}")
     batch = None
     if not args.no_batch:
-        batch = bench_batch(dev, specialize=not args.batch_interp)
+        batch = bench_batch(dev, specialize=not args.batch_interp, lds_split=args.batch_lds_split)
         log(f"[bench] batch: {batch}")
     c0 = bench_c0(dev)
     log(f"[bench] c0: {c0}")
@@ -584,7 +590,7 @@ def main():
                    "tile": {"tb": fwd["cfg"].tb, "low_bits": fwd["cfg"].low_bits,
                             "max_cost": fwd["cfg"].max_cost, "fuse2q": fwd["cfg"].fuse2q,
                             "io_lane_bits": fwd["cfg"].io_lane_bits, "register_bits": fwd["cfg"].rb,
-                            "lds_split": fwd["cfg"].lds_split},
+                            "lds_split": fwd["cfg"].lds_split, "remap": fwd["cfg"].remap},
                    "fused_passes": fwd["n_tile"]},
         "gates_per_s": round(fwd["gates_per_s"], 1),
         "c64_vs_c128_l2_err": err,

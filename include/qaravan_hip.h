@@ -31,6 +31,8 @@
  *                         tensordot; here it is an FP32 MFMA GEMM (c64)
  *   svh_apply_block5x2    two such op_action calls (statevector_sim.py:50-52) on disjoint
  *                         qubits of one layer, fused into one sweep over the state
+ *   svh_permute_bits      no counterpart: the reference keeps qubit q at axis q; remapped
+ *                         programs move qubits between address bits and end with this
  *   svh_batch_*, svh_spb_* tensorQ/monte_carlo_sim.py:14-35 (many small StatevectorSim runs,
  *                         one after another); here a whole batch of parametrised circuits
  *                         with their adjoint gradients runs as one launch sequence
@@ -78,6 +80,12 @@ int svh_apply_block5(void *psi, int n, int dtype, const int *bits, const void *u
  * complex64 5-qubit dense passes on disjoint bits here. */
 int svh_apply_block5x2(void *psi, int n, int dtype, const int *bits_a, const void *ua_dev,
                        const int *bits_b, const void *ub_dev, void *stream);
+
+/* Out-of-place bit permutation: dst[i'] = src[i] where bit b of i becomes bit
+ * perm[b] of i' (perm is a host array of n distinct bits; 8 <= n <= 36).  A
+ * program compiled with qubit remapping ends with this step inside
+ * svh_run_program_sp, through scratch, so its result is in natural order. */
+int svh_permute_bits(const void *src, void *dst, int n, int dtype, const int *perm, void *stream);
 
 /* Bytes of device scratch svh_run_program / svh_run_adjoint need for `prog`. */
 int svh_program_scratch_bytes(const void *prog_host, int dtype, size_t *bytes);

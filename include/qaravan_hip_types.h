@@ -59,7 +59,11 @@ typedef struct {
     int32_t n_pool; /* doubles in the matrix pool */
     int64_t off_passes, off_stages, off_ops, off_grad, off_pool;
     int64_t n_params; /* length of θ read by ROT ops (0 if none)                  */
-    int64_t reserved[2];
+    /* 0, or the offset of int32 perm[n_qubits]: the program's passes move
+     * logical address bit q to physical bit perm[q]; the run ends with one
+     * permutation back to the natural order (needs scratch for one state). */
+    int64_t off_perm;
+    int64_t reserved;
 } svh_prog_header_t; /* 96 bytes */
 
 typedef struct {
@@ -89,7 +93,11 @@ enum {
      * register bits both layouts share, so the pass needs LDS for 2^(L-1) or
      * 2^(L-2) amplitudes instead of 2^L. */
     SVH_PASSF_HALF_LDS = 2,
-    SVH_PASSF_QUARTER_LDS = 4
+    SVH_PASSF_QUARTER_LDS = 4,
+    /* TILE, per-pass compiled forward kernels: the tile is stored through the
+     * stage record that follows the pass's n_stages records; its global bits
+     * permute the tile's bits (a qubit remap within the tile). */
+    SVH_PASSF_REMAP_STORE = 8
 };
 
 /* A stage's layout: thread t holds registers j = 0..2^rb-1 at tile-local index

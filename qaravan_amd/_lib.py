@@ -32,6 +32,7 @@ SIGNATURES = {
     "svh_apply_block5": (_c_int, [_c_void_p, _c_int, _c_int, ctypes.POINTER(_c_int), _c_void_p, _c_void_p]),
     "svh_apply_block5x2": (_c_int, [_c_void_p, _c_int, _c_int, ctypes.POINTER(_c_int), _c_void_p,
                                     ctypes.POINTER(_c_int), _c_void_p, _c_void_p]),
+    "svh_permute_bits": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, ctypes.POINTER(_c_int), _c_void_p]),
     "svh_program_scratch_bytes": (_c_int, [_c_void_p, _c_int, ctypes.POINTER(_c_size_t)]),
     "svh_run_program": (_c_int, [_c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
                                  _c_size_t, _c_void_p]),

@@ -49,7 +49,7 @@ def best_tile_config(n, dtype):
         cfg = TileConfig(tb=e["tb"], low_bits=e["low_bits"], max_cost=e["max_cost"],
                          prefetch=e.get("prefetch", False), specialize=e.get("specialize", False),
                          fuse2q=e.get("fuse2q", True), io_lane_bits=e.get("io_lane_bits", 6), rb=e.get("rb", 4),
-                         lds_split=e.get("lds_split", 0))
+                         lds_split=e.get("lds_split", 0), remap=e.get("remap", False))
     return replace(cfg, tb=min(cfg.tb, n - cfg.rb))
 
 
@@ -68,18 +68,17 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_
     psi = torch.empty(2 ** n, dtype=dtype, device=device)
     scratch = DeviceScratch(device)
     results = []
-    for tb, lb, mc, pf, spz, f2, io, rb, hl in itertools.product(space["tb"], space["low_bits"], space["max_cost"],
-                                                                 space.get("prefetch", [False]),
-                                                                 space.get("specialize", [False]),
-                                                                 space.get("fuse2q", [True]),
-                                                                 space.get("io_lane_bits", [6]), space.get("rb", [4]),
-                                                                 space.get("lds_split", [0])):
+    for tb, lb, mc, pf, spz, f2, io, rb, hl, rm in itertools.product(
+            space["tb"], space["low_bits"], space["max_cost"], space.get("prefetch", [False]),
+            space.get("specialize", [False]), space.get("fuse2q", [True]), space.get("io_lane_bits", [6]),
+            space.get("rb", [4]), space.get("lds_split", [0]), space.get("remap", [False])):
         if tb + rb > n or (dtype == torch.complex128 and tb > (8 if pf else 9)) or (pf and tb > 9):
             continue
-        if (pf and spz) or (rb == 5 and (pf or not spz)) or (hl and (pf or not spz or hl >= rb)):
+        if (pf and spz) or (rb == 5 and (pf or not spz)) or (hl and (pf or not spz or hl >= rb)) or \
+                (rm and (pf or not spz)):
             continue
         cfg = TileConfig(tb=tb, low_bits=lb, max_cost=mc, prefetch=pf, specialize=spz, fuse2q=f2, io_lane_bits=io,
-                         rb=rb, lds_split=hl)
+                         rb=rb, lds_split=hl, remap=rm)
         prog = compile_ops(ops, n, cfg)
         times = []
         for r in range(repeats + 1):
@@ -94,7 +93,7 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_
         t = min(times)
         results.append((t, cfg, prog.n_tile_passes()))
         log(f"[tune] n={n} {dtype} tb={tb} rb={rb} low={lb} cap={mc:g} prefetch={pf} specialize={spz} fuse2q={f2} "
-            f"io={io} lds_split={hl}: "
+            f"io={io} lds_split={hl} remap={rm}: "
             f"{t * 1e3:.3f} ms, {prog.n_tile_passes()} passes")
     results.sort(key=lambda x: x[0])
     t, cfg, npass = results[0]
@@ -103,6 +102,7 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_
         cache[_key(n, dtype)] = dict(tb=cfg.tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost,
                                      prefetch=cfg.prefetch, specialize=cfg.specialize, fuse2q=cfg.fuse2q,
                                      io_lane_bits=cfg.io_lane_bits, rb=cfg.rb, lds_split=cfg.lds_split,
+                                     remap=cfg.remap,
                                      ms=t * 1e3,
                                      passes=npass)
         with open(out_path or CACHE, "w") as f:
@@ -124,7 +124,7 @@ def best_adjoint_config(n, dtype):
     else:
         cfg = TileConfig(tb=e["tb"], low_bits=e["low_bits"], max_cost=e["max_cost"],
                          specialize=e.get("specialize", False), io_lane_bits=e.get("io_lane_bits", 6),
-                         rb=e.get("rb", 4))
+                         rb=e.get("rb", 4), lds_split=e.get("lds_split", 0))
     return replace(cfg, tb=min(cfg.tb, n - cfg.rb))
 
 
@@ -140,11 +140,12 @@ def tune_adjoint(n, layers, dtype, device, space=None, repeats=5, log=print, out
     circ = hea_circuit(n, layers, theta)
     th = torch.tensor(theta, device=device)
     results = []
-    for tb, mc, spz, io, rb in itertools.product(space["tb"], space["max_cost"], space["specialize"],
-                                                 space.get("io_lane_bits", [6]), space.get("rb", [4])):
-        if tb + rb > n or (rb != 4 and not spz):
+    for tb, mc, spz, io, rb, ls in itertools.product(space["tb"], space["max_cost"], space["specialize"],
+                                                     space.get("io_lane_bits", [6]), space.get("rb", [4]),
+                                                     space.get("lds_split", [0])):
+        if tb + rb > n or (rb != 4 and not spz) or (ls and (not spz or ls >= rb)):
             continue
-        cfg = TileConfig(tb=tb, low_bits=4, max_cost=mc, specialize=spz, io_lane_bits=io, rb=rb)
+        cfg = TileConfig(tb=tb, low_bits=4, max_cost=mc, specialize=spz, io_lane_bits=io, rb=rb, lds_split=ls)
         m = AdjointModel(circ, tfim_terms(n), len(theta), device=device, dtype=dtype, cfg=cfg)
         for _ in range(2):
             m.energy_and_grad(th)
@@ -155,7 +156,8 @@ def tune_adjoint(n, layers, dtype, device, space=None, repeats=5, log=print, out
         torch.cuda.synchronize(device)
         t = (time.perf_counter() - t0) / repeats
         results.append((t, cfg))
-        log(f"[tune-adjoint] n={n} tb={tb} rb={rb} cap={mc:g} specialize={spz} io={io}: {t * 1e3:.3f} ms")
+        log(f"[tune-adjoint] n={n} tb={tb} rb={rb} cap={mc:g} specialize={spz} io={io} lds_split={ls}: "
+            f"{t * 1e3:.3f} ms")
     results.sort(key=lambda x: x[0])
     t, cfg = results[0]
     cache = _load()
@@ -163,7 +165,7 @@ def tune_adjoint(n, layers, dtype, device, space=None, repeats=5, log=print, out
                                                                      max_cost=cfg.max_cost,
                                                                      specialize=cfg.specialize,
                                                                      io_lane_bits=cfg.io_lane_bits, rb=cfg.rb,
-                                                                     ms=t * 1e3)
+                                                                     lds_split=cfg.lds_split, ms=t * 1e3)
     with open(out_path or CACHE, "w") as f:
         json.dump(cache, f, indent=1, sort_keys=True)
     return cfg, results

@@ -18,6 +18,8 @@
 
 namespace svh {
 
+constexpr int kPermTileBits = 10;  // bit-permutation tiles: 1024 amplitudes
+
 // ------------------------------------------------ the pass kernel
 // One tile of one pass: workgroup-local work shared by k_tile (one circuit,
 // tile = blockIdx.x) and k_tile_batch (many circuits in one launch).
@@ -438,15 +440,24 @@ static int check_program(const void *prog_host, int n, const svh_prog_header_t *
                 return fail(-8, "tile pass " + std::to_string(p) + ": register bits must be 3, 4 or 5");
             if (P.L != P.tb + P.rb || P.L > h->n_qubits || P.L > SVH_MAX_TILE_BITS)
                 return fail(-8, "tile pass " + std::to_string(p) + ": bad L");
-            if (P.n_stages < 1 || P.stage_begin < 0 || P.stage_begin + P.n_stages > h->n_stages)
+            const int extra = (P.flags & SVH_PASSF_REMAP_STORE) ? 1 : 0;  // the store record
+            if (P.n_stages < 1 || P.stage_begin < 0 || P.stage_begin + P.n_stages + extra > h->n_stages)
                 return fail(-8, "tile pass " + std::to_string(p) + ": bad stage range");
+            if (extra) {
+                const svh_stage_t &A = stages[P.stage_begin + P.n_stages - 1];
+                const svh_stage_t &B = stages[P.stage_begin + P.n_stages];
+                for (int k = 0; k < P.rb; ++k)
+                    if (A.rbits[k] != B.rbits[k])
+                        return fail(-8, "tile pass " + std::to_string(p) + ": store record layout differs");
+                if (B.op_begin != B.op_end) return fail(-8, "tile pass " + std::to_string(p) + ": store record has ops");
+            }
             uint64_t m = 0;
             for (int i = 0; i < P.L; ++i) {
                 if (P.bits[i] < 0 || P.bits[i] >= h->n_qubits || ((m >> P.bits[i]) & 1))
                     return fail(-8, "tile pass " + std::to_string(p) + ": bad tile bits");
                 m |= 1ull << P.bits[i];
             }
-            for (int s = P.stage_begin; s < P.stage_begin + P.n_stages; ++s) {
+            for (int s = P.stage_begin; s < P.stage_begin + P.n_stages + extra; ++s) {
                 const svh_stage_t &S = stages[s];
                 int rm = 0;
                 for (int k = 0; k < P.rb; ++k) {
@@ -502,8 +513,117 @@ static int check_program(const void *prog_host, int n, const svh_prog_header_t *
             return fail(-8, "bad pass kind");
         }
     }
+    if (h->off_perm) {
+        const int32_t *perm = reinterpret_cast<const int32_t *>(b + h->off_perm);
+        uint64_t seen = 0;
+        for (int q = 0; q < h->n_qubits; ++q) {
+            if (perm[q] < 0 || perm[q] >= h->n_qubits || ((seen >> perm[q]) & 1))
+                return fail(-8, "final qubit permutation is not a permutation");
+            seen |= 1ull << perm[q];
+        }
+    }
     *hdr = h;
     return 0;
+}
+
+// ------------------------------------------------ bit permutation (out of place)
+// dst[i'] = src[i] where bit b of i becomes bit perm[b] of i'.  A workgroup
+// moves a tile of 2^TB amplitudes through LDS: the tile bits are the source
+// bits 0..3, the source bits that land on destination bits 0..3, and the
+// lowest other source bits, so both the loads and the stores run along 16
+// consecutive amplitudes.
+struct PermArgs {
+    int32_t tsrc[kPermTileBits];  // tile-local bit x -> source bit (ascending)
+    int32_t uidx[kPermTileBits];  // store-order bit i -> tile-local bit
+    int32_t udst[kPermTileBits];  // store-order bit i -> destination bit (ascending)
+    int32_t nsrc[64];             // non-tile source bits (ascending)
+    int32_t ndst[64];             // their destination bits
+    int32_t nnon;
+    int32_t tbits;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_permute_bits(const cpx<R> *__restrict__ src, cpx<R> *__restrict__ dst,
+                                                      PermArgs A) {
+    __shared__ cpx<R> tile[1 << kPermTileBits];
+    const int tid = threadIdx.x;
+    const int TB = A.tbits;
+    uint64_t sb = 0, db = 0, t = blockIdx.x;
+    for (int k = 0; k < A.nnon && t; ++k, t >>= 1) {
+        sb |= (t & 1ull) << A.nsrc[k];
+        db |= (t & 1ull) << A.ndst[k];
+    }
+    const int per = (1 << TB) / 256;
+    for (int r = 0; r < per; ++r) {
+        const int e = r * 256 + tid;
+        uint64_t so = 0;
+        for (int x = 0; x < TB; ++x) so |= (uint64_t)((e >> x) & 1) << A.tsrc[x];
+        tile[e ^ ((e >> 5) & 31)] = src[sb | so];
+    }
+    __syncthreads();
+    for (int r = 0; r < per; ++r) {
+        const int f = r * 256 + tid;
+        int x = 0;
+        uint64_t dof = 0;
+        for (int i = 0; i < TB; ++i) {
+            const int bit = (f >> i) & 1;
+            x |= bit << A.uidx[i];
+            dof |= (uint64_t)bit << A.udst[i];
+        }
+        dst[db | dof] = tile[x ^ ((x >> 5) & 31)];
+    }
+}
+
+static int permute_args(int n, const int *perm, PermArgs *A) {
+    const int TB = n < kPermTileBits ? n : kPermTileBits;
+    if (TB < 8) return fail(-2, "bit permutation needs n >= 8");
+    std::vector<int> inv(n, -1);
+    for (int b = 0; b < n; ++b) {
+        if (perm[b] < 0 || perm[b] >= n || inv[perm[b]] >= 0) return fail(-4, "perm is not a permutation");
+        inv[perm[b]] = b;
+    }
+    uint64_t tm = 0;
+    int cnt = 0;
+    auto add = [&](int b) {
+        if (!((tm >> b) & 1) && cnt < TB) {
+            tm |= 1ull << b;
+            ++cnt;
+        }
+    };
+    for (int b = 0; b < 4; ++b) add(b);
+    for (int d = 0; d < 4; ++d) add(inv[d]);
+    for (int b = 0; b < n && cnt < TB; ++b) add(b);
+    int x = 0;
+    for (int b = 0; b < n; ++b)
+        if ((tm >> b) & 1) A->tsrc[x++] = b;
+    // store order: tile bits by destination bit
+    int order[kPermTileBits];
+    for (int i = 0; i < TB; ++i) order[i] = i;
+    std::sort(order, order + TB, [&](int a, int c) { return perm[A->tsrc[a]] < perm[A->tsrc[c]]; });
+    for (int i = 0; i < TB; ++i) {
+        A->uidx[i] = order[i];
+        A->udst[i] = perm[A->tsrc[order[i]]];
+    }
+    A->nnon = 0;
+    for (int b = 0; b < n; ++b)
+        if (!((tm >> b) & 1)) {
+            A->nsrc[A->nnon] = b;
+            A->ndst[A->nnon] = perm[b];
+            A->nnon++;
+        }
+    A->tbits = TB;
+    return 0;
+}
+
+static int launch_permute(const void *src, void *dst, int n, int dtype, const int *perm, hipStream_t st) {
+    PermArgs A;
+    if (int e = permute_args(n, perm, &A)) return e;
+    const unsigned grid = (unsigned)(1ull << (n - A.tbits));
+    if (dtype == SVH_C64)
+        k_permute_bits<float><<<grid, 256, 0, st>>>((const cpx<float> *)src, (cpx<float> *)dst, A);
+    else
+        k_permute_bits<double><<<grid, 256, 0, st>>>((const cpx<double> *)src, (cpx<double> *)dst, A);
+    return checked_launch("k_permute_bits");
 }
 
 // Dense passes take their matrix from the (double) pool; c64 needs a float copy.
@@ -608,7 +728,9 @@ static size_t scratch_need(const svh_prog_header_t *h, int dtype, bool dual) {
         const size_t g = sizeof(double) * grad_doubles(h);
         if (g > need) need = g;
     }
-    return trig_bytes(h, dtype) + ((need + 255) & ~(size_t)255);
+    // a remapped program ends with a permutation through one state-sized buffer
+    const size_t perm = h->off_perm ? es * ((size_t)1 << h->n_qubits) : 0;
+    return trig_bytes(h, dtype) + ((need + 255) & ~(size_t)255) + perm;
 }
 
 // Kernels of a specialised program: svh_sp_<p> for the passes it covers.
@@ -719,6 +841,8 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
         if (P.kind == SVH_PASS_TILE && sp && sp->fn[p]) {
             e = launch_sp(sp->fn[p], psi, dual ? lam : nullptr, n, dtype, P, dpass + p, dstage, dop, hstage, trig,
                           gp, st);
+        } else if (P.kind == SVH_PASS_TILE && (P.flags & SVH_PASSF_REMAP_STORE)) {
+            return fail(-8, "remapped passes run only in their compiled kernels (svh_run_program_sp)");
         } else if (P.kind == SVH_PASS_TILE) {
             if (dtype == SVH_C64)
                 e = dual ? launch_tile<float, true>((cpx<float> *)psi, (cpx<float> *)lam, n, P, dpass + p, dstage,
@@ -765,6 +889,17 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
         if (distinct) return 0;
         k_grad_scatter<<<1, 64, 0, st>>>(slotsum, h->n_grad_slots, dgrad, grad_dev);
         return checked_launch("k_grad_scatter");
+    }
+    if (h->off_perm) {
+        // back to the natural order: physical bit perm[q] -> bit q, through
+        // the state-sized tail of scratch, then copied back in place
+        const int32_t *perm = reinterpret_cast<const int32_t *>(hb + h->off_perm);
+        std::vector<int> back(n);
+        for (int q = 0; q < n; ++q) back[perm[q]] = q;
+        const size_t es = dtype == SVH_C64 ? 8 : 16, bytes = es << n;
+        void *tmp = static_cast<char *>(scratch) + (need - bytes);
+        if (int e = launch_permute(psi, tmp, n, dtype, back.data(), st)) return e;
+        return hip_status(hipMemcpyAsync(psi, tmp, bytes, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync");
     }
     return 0;
 }
@@ -950,3 +1085,12 @@ int svh_run_adjoint(void *psi, void *lam, int n, int dtype, const void *prog_hos
 }
 
 }  // extern "C"
+
+
+extern "C" int svh_permute_bits(const void *src, void *dst, int n, int dtype, const int *perm, void *stream) {
+    if (!src || !dst || !perm) return fail(-1, "NULL pointer");
+    if (src == dst) return fail(-1, "the bit permutation is out of place: src and dst must differ");
+    if (n < 8 || n > kMaxQubits) return fail(-2, "bit permutation needs 8 <= n <= 36");
+    if (dtype != SVH_C64 && dtype != SVH_C128) return fail(-3, "bad dtype");
+    return launch_permute(src, dst, n, dtype, perm, (hipStream_t)stream);
+}

@@ -37,6 +37,7 @@ PASS_TILE, PASS_DENSE, PASS_NORMALIZE = 1, 2, 3
 PASSF_PREFETCH = 1
 PASSF_HALF_LDS = 2
 PASSF_QUARTER_LDS = 4
+PASSF_REMAP_STORE = 8
 OP_MAT1, OP_MAT2, OP_CX, OP_CZ, OP_ROT1, OP_ROT2 = 1, 2, 3, 4, 5, 6
 OPF_NEG, OPF_GRAD = 1, 2
 RB = 4
@@ -46,7 +47,7 @@ HDR_DT = np.dtype([('magic', '<u4'), ('version', '<i4'), ('n_qubits', '<i4'), ('
                    ('n_stages', '<i4'), ('n_ops', '<i4'), ('n_grad_slots', '<i4'), ('n_pool', '<i4'),
                    ('off_passes', '<i8'), ('off_stages', '<i8'), ('off_ops', '<i8'),
                    ('off_grad', '<i8'), ('off_pool', '<i8'), ('n_params', '<i8'),
-                   ('reserved', '<i8', (2,))])
+                   ('off_perm', '<i8'), ('reserved', '<i8')])
 PASS_DT = np.dtype([('kind', '<i4'), ('L', '<i4'), ('tb', '<i4'), ('rb', '<i4'),
                     ('stage_begin', '<i4'), ('n_stages', '<i4'), ('grad_begin', '<i4'),
                     ('n_grad', '<i4'), ('pool_off', '<i8'), ('flags', '<i4'), ('pad', '<i4'),
@@ -298,6 +299,11 @@ class Pass:
     mat: np.ndarray = None
     flags: int = 0
     rb: int = RB
+    # remapped store (tile-local bit x is written to physical bit store_bits[x]);
+    # the last layout keeps store_lanes off the registers and store_regs on them
+    store_bits: list = None
+    store_lanes: list = None
+    store_regs: list = None
 
 
 @dataclass
@@ -326,6 +332,10 @@ class TileConfig:
               the two layouts of a re-distribution share: 1/2^lds_split of the
               LDS per workgroup, so more workgroups per CU (the stage planner
               keeps that many shared bits).
+    remap: per-pass compiled forward kernels store each tile with its bits
+              permuted, so that the qubits the next passes need most come to
+              the low address bits every tile holds; fewer passes, plus one
+              permutation back to the natural order at the end.
     """
     tb: int = 8
     low_bits: int = 6
@@ -336,6 +346,7 @@ class TileConfig:
     io_lane_bits: int = 6
     rb: int = RB
     lds_split: int = 0
+    remap: bool = False
 
 
 def _is_barrier(op):
@@ -426,6 +437,96 @@ def partition(ops, n, cfg):
     return passes
 
 
+def partition_remap(ops, n, cfg):
+    """Greedy packing like ``partition``, with qubits moving between address
+    bits.  ``pos[q]`` is the physical address bit of logical bit q; ops are
+    placed in passes with physical bits.  After each pass the store permutes
+    the tile's bits so that up to ``low_bits`` qubits that the remaining ops
+    need soonest move to the low physical bits (which every tile holds); the
+    qubits they displace take the vacated positions.  So that the store stays
+    coalesced, the moved qubits come from the tile's low + rb lowest bits:
+    then the last layout can hold every lower non-moving tile bit in
+    registers and put the moved ones on lanes 0..low-1 in order.  Returns
+    (passes, pos) with pos the final layout."""
+    L = cfg.tb + cfg.rb
+    low = min(cfg.low_bits, L)
+    if L > n:
+        raise ValueError(f"tile of {L} bits exceeds n={n}")
+    pos = list(range(n))
+    rem = list(ops)
+    passes = []
+    while rem:
+        if _is_barrier(rem[0]):
+            raise ValueError("remap takes circuits of tile ops only (no dense or reset passes)")
+        T = set(range(low))
+        blocked = _Blocker()
+        taken = []
+        cost = 0.0
+        for i, op in enumerate(rem):
+            if _is_barrier(op):
+                break
+            b = set(op.bits)                           # logical: ordering
+            if blocked.blocks(op, b):
+                blocked.add(op, b)
+            else:
+                need = {pos[x] for x in b} - T         # physical: tile
+                c = OP_COST[op.kind]
+                if len(T) + len(need) <= L and (cost + c <= cfg.max_cost or not taken):
+                    T |= need
+                    taken.append(i)
+                    cost += c
+                else:
+                    blocked.add(op, b)
+            if len(blocked.other) == n or cost + 0.25 > cfg.max_cost:
+                break
+        x = 0
+        while len(T) < L:
+            if x not in T:
+                T.add(x)
+            x += 1
+        tset = set(taken)
+        pops = [EOp(o.kind, tuple(pos[x] for x in o.bits), o.mat, o.param, o.pauli, o.flags)
+                for o in (rem[i] for i in taken)]
+        rem = [op for i, op in enumerate(rem) if i not in tset]
+        Ts = sorted(T)
+        inv = {ph: q for q, ph in enumerate(pos)}
+        # qubits for the low bits: needed soonest, from the tile's low + rb lowest bits
+        cand = [inv[ph] for ph in Ts[:low + cfg.rb]]
+        new_low = []
+        for op in rem:
+            if _is_barrier(op):
+                break
+            for q in op.bits:
+                if q in cand and q not in new_low:
+                    new_low.append(q)
+            if len(new_low) >= low:
+                break
+        new_low = new_low[:low]
+        for q in [inv[ph] for ph in range(low)]:     # keep current low qubits otherwise
+            if len(new_low) < low and q not in new_low:
+                new_low.append(q)
+        lanes = sorted(Ts.index(pos[q]) for q in new_low)   # tile-local, ascending
+        dest = {x: ph for x, ph in enumerate(Ts)}
+        vacated = sorted(Ts[x] for x in lanes if Ts[x] >= low)
+        displaced = sorted(x for x in range(low) if x not in lanes)
+        for x, ph in zip(displaced, vacated):
+            dest[x] = ph
+        for i, x in enumerate(lanes):
+            dest[x] = i
+        p = Pass(PASS_TILE, Ts, pops, tb=cfg.tb, rb=cfg.rb)
+        if any(dest[x] != Ts[x] for x in range(L)):
+            p.store_bits = [dest[x] for x in range(L)]
+            p.store_lanes = lanes
+            p.store_regs = [x for x in range(lanes[-1]) if x not in lanes]
+            p.flags |= PASSF_REMAP_STORE
+            newpos = list(pos)
+            for x in range(L):
+                newpos[inv[Ts[x]]] = dest[x]
+            pos = newpos
+        passes.append(p)
+    return passes, pos
+
+
 def _stage_ops(p, io_lane_bits=6, keep_shared=0):
     """Split a tile pass's ops into register stages (tile-local bits).
     keep_shared: every stage keeps at least this many register bits of the
@@ -452,21 +553,27 @@ def _stage_ops(p, io_lane_bits=6, keep_shared=0):
                 cur.ops.extend(took)
                 pend = keep
                 continue
-        R, blocked = [], _Blocker()
-        for op, lb in pend:
-            s = set(lb)
-            if blocked.blocks(op, s):
-                blocked.add(op, s)
-                continue
-            new = [x for x in lb if x not in R]
-            # keep_shared: the last free slots are for bits of the previous layout
-            room = RB - (max(0, keep_shared - len(set(R + new) & set(cur.rbits))) if cur is not None else 0)
-            if len(R) + len(new) <= room:
-                R += new
-            else:
-                blocked.add(op, s)
-            if len(R) == RB:
-                break
+        def pick(avoid):
+            R, blocked = [], _Blocker()
+            for op, lb in pend:
+                s = set(lb)
+                if blocked.blocks(op, s) or s & avoid:
+                    blocked.add(op, s)
+                    continue
+                new = [x for x in lb if x not in R]
+                # keep_shared: the last free slots are for bits of the previous layout
+                room = RB - (max(0, keep_shared - len(set(R + new) & set(cur.rbits))) if cur is not None else 0)
+                if len(R) + len(new) <= room:
+                    R += new
+                else:
+                    blocked.add(op, s)
+                if len(R) == RB:
+                    break
+            return R
+        # the first layout is also the load layout: prefer ops off the lane bits
+        R = pick(lane_bits) if cur is None else []
+        if not R:
+            R = pick(set())
         if cur is not None:                     # bits both layouts hold
             for x in sorted(cur.rbits, reverse=True):
                 if len(set(R) & set(cur.rbits)) >= keep_shared or len(R) == RB:
@@ -485,7 +592,21 @@ def _stage_ops(p, io_lane_bits=6, keep_shared=0):
     # coalesced load/store: the first/last layout must keep lanes on low bits
     if set(stages[0].rbits) & lane_bits:
         stages.insert(0, Stage(_io_layout(stages[0].rbits, io_bits, lane_bits, keep_shared), []))
-    if set(stages[-1].rbits) & lane_bits:
+    if p.store_lanes is not None:
+        # remapped store: the lanes carry the tile bits bound for the low
+        # physical bits, in order, so every lower tile bit sits in a register
+        sl = list(p.store_lanes[:min(io_lane_bits, len(p.store_lanes))])
+        req = [x for x in range(sl[-1]) if x not in sl]
+        last = stages[-1]
+        if set(last.rbits) & set(sl) or not set(req) <= set(last.rbits):
+            R = list(req)
+            for x in sorted(last.rbits, reverse=True) + list(range(L - 1, -1, -1)):
+                if len(R) == RB:
+                    break
+                if x not in R and x not in sl:
+                    R.append(x)
+            stages.append(Stage(R, []))
+    elif set(stages[-1].rbits) & lane_bits:
         stages.append(Stage(_io_layout(stages[-1].rbits, io_bits, lane_bits, keep_shared), []))
     p.stages = stages
     return p
@@ -523,6 +644,7 @@ class Program:
         self._dev = {}
         self._sp = {}
         self.specialize = False
+        self.final_pos = None             # remapped programs: physical bit of logical bit q at the end
 
     def specialized(self, dtype_code, dual=False):
         """Loaded straight-line kernels of this program (compiled on first use)."""
@@ -557,7 +679,7 @@ class Program:
                     ops=sum(len(p.ops) for p in self.passes))
 
 
-def encode(n, passes, grad=False, n_params=0):
+def encode(n, passes, grad=False, n_params=0, final_pos=None):
     stage_rows, op_rows, grad_params, pool = [], [], [], []
     pass_rows = np.zeros(len(passes), dtype=PASS_DT)
     for pi, p in enumerate(passes):
@@ -619,6 +741,18 @@ def encode(n, passes, grad=False, n_params=0):
                     op_rows.append((kind, a, b, op.param, op.pauli, op.flags, gslot, 0, m, m.astype(np.float32)))
                 s['op_end'] = len(op_rows)
                 stage_rows.append(s)
+            if p.store_bits is not None:
+                # the store record: global bits of the last layout after the remap
+                s = np.zeros((), dtype=STAGE_DT)
+                last = p.stages[-1].rbits
+                tab = stage_tables(p.store_bits, p.tb, last)
+                s['rbits'][:p.rb] = last
+                s['rsorted'][:p.rb] = tab['rsorted']
+                s['tgbit'][:p.tb] = tab['tgbit']
+                s['lsw'][:1 << p.rb] = tab['lsw']
+                s['goff'][:1 << p.rb] = tab['goff']
+                s['op_begin'] = s['op_end'] = len(op_rows)
+                stage_rows.append(s)
             pr['n_grad'] = slot
         elif p.kind == PASS_DENSE:
             k = len(p.bits)
@@ -631,13 +765,15 @@ def encode(n, passes, grad=False, n_params=0):
     used = [int(r[3]) + 1 for r in op_rows if r[0] in (OP_ROT1, OP_ROT2)]
     grads = np.array(grad_params, dtype=np.int32)
     poola = np.array(pool, dtype=np.float64)
+    perm = (np.array(final_pos, dtype=np.int32) if final_pos is not None and list(final_pos) != list(range(n))
+            else np.zeros(0, np.int32))
 
     def align(x):
         return (x + 63) & ~63
     off = align(HDR_DT.itemsize)
     offs = {}
     for name, arr in (('passes', pass_rows), ('stages', stages), ('ops', opsa), ('grad', grads),
-                      ('pool', poola)):
+                      ('pool', poola), ('perm', perm)):
         offs[name] = off
         off = align(off + arr.nbytes)
     blob = np.zeros(off, dtype=np.uint8)
@@ -648,12 +784,16 @@ def encode(n, passes, grad=False, n_params=0):
     h['n_params'] = max([int(n_params)] + used) if used else 0
     for name in offs:
         h['off_' + name] = offs[name]
+    if not perm.size:
+        h['off_perm'] = 0
     blob[:HDR_DT.itemsize] = np.frombuffer(h.tobytes(), dtype=np.uint8)
     for name, arr in (('passes', pass_rows), ('stages', stages), ('ops', opsa), ('grad', grads),
-                      ('pool', poola)):
+                      ('pool', poola), ('perm', perm)):
         if arr.nbytes:
             blob[offs[name]:offs[name] + arr.nbytes] = np.frombuffer(arr.tobytes(), dtype=np.uint8)
-    return Program(n, passes, blob, grads, n_params)
+    prog = Program(n, passes, blob, grads, n_params)
+    prog.final_pos = list(final_pos) if perm.size else None
+    return prog
 
 
 def min_tile_n(grad=False):
@@ -690,10 +830,16 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
         raise ValueError("lds_split must be 0, 1 or 2 and below the register bits")
     tb = min(cfg.tb, n - cfg.rb)
     split = cfg.lds_split
+    remap = (cfg.remap and not grad and cfg.specialize and not cfg.prefetch
+             and not any(_is_barrier(op) for op in ops))
     cfg = TileConfig(tb=tb, low_bits=max(1, cfg.low_bits), max_cost=cfg.max_cost,
                      prefetch=cfg.prefetch and not grad, specialize=cfg.specialize, fuse2q=cfg.fuse2q,
-                     io_lane_bits=cfg.io_lane_bits, rb=cfg.rb, lds_split=split)
-    passes = partition(ops, n, cfg)
+                     io_lane_bits=cfg.io_lane_bits, rb=cfg.rb, lds_split=split, remap=remap)
+    final_pos = None
+    if remap:
+        passes, final_pos = partition_remap(ops, n, cfg)
+    else:
+        passes = partition(ops, n, cfg)
     for p in passes:
         if p.kind == PASS_TILE:
             _stage_ops(p, cfg.io_lane_bits, keep_shared=split)
@@ -702,7 +848,7 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
                 p.flags |= PASSF_QUARTER_LDS
             elif split >= 1 and shared >= 1:
                 p.flags |= PASSF_HALF_LDS
-    prog = encode(n, passes, grad, n_params)
+    prog = encode(n, passes, grad, n_params, final_pos)
     prog.specialize = bool(cfg.specialize)
     return prog
 

@@ -202,6 +202,12 @@ def _body(R, p, stages, ops, dual):
         out.append(f"    sp_store2<{R}, {tb}, {rb}>(psi, lam, base, st[{ns - 1}], v, w);")
         if ng:
             out.append(f"    sp_grad_flush_wg<{tb}>(gacc, {ng}, gpart, wg);")
+    elif int(p['flags']) & P.PASSF_REMAP_STORE:
+        # the store permutes the tile's bits: every wave must have loaded
+        # before any wave stores (with one stage no exchange barrier did it)
+        if ns == 1:
+            out.append("    __syncthreads();")
+        out.append(f"    sp_store<{R}, {tb}, {rb}>(psi, base, st[{ns}], v);")
     else:
         out.append(f"    sp_store<{R}, {tb}, {rb}>(psi, base, st[{ns - 1}], v);")
     return out

@@ -87,4 +87,33 @@ def run(blob, sv, theta=None, lam=None, grad=None):
                     sv = O.op_action(m, qubits, sv)
                     if lam is not None:
                         lam = O.op_action(m, qubits, lam)
+            if int(p['flags']) & P.PASSF_REMAP_STORE:
+                # the store record's tables give each tile bit its new address bit
+                last = stages[int(p['stage_begin']) + int(p['n_stages']) - 1]
+                rec = stages[int(p['stage_begin']) + int(p['n_stages'])]
+                nrb, tb = int(p['rb']), int(p['tb'])
+                rbits = [int(x) for x in last['rbits'][:nrb]]
+                assert rbits == [int(x) for x in rec['rbits'][:nrb]]
+                store = [None] * L
+                for k, x in enumerate(rbits):
+                    go = int(rec['goff'][1 << k])
+                    assert go > 0 and go & (go - 1) == 0
+                    store[x] = go.bit_length() - 1
+                thr = [x for x in range(L) if x not in rbits]
+                for i, x in enumerate(thr):
+                    store[x] = int(rec['tgbit'][i])
+                assert sorted(store) == sorted(tile_bits)
+                sv = permute_bits(sv, n, {tile_bits[x]: store[x] for x in range(L)})
+    if int(h['off_perm']):
+        off = int(h['off_perm'])
+        perm = np.frombuffer(blob[off:off + 4 * n].tobytes(), dtype='<i4')
+        sv = permute_bits(sv, n, {int(perm[q]): q for q in range(n)})
     return sv, lam
+
+
+def permute_bits(sv, n, move):
+    """State with address bit b moved to bit move[b] (bits not in ``move``
+    stay); axis i of the rank-n view is address bit n-1-i."""
+    src_of = {d: b for b, d in move.items()}
+    axes = [n - 1 - src_of.get(n - 1 - k, n - 1 - k) for k in range(n)]
+    return np.transpose(np.asarray(sv).reshape((2,) * n), axes).reshape(-1).copy()

@@ -423,3 +423,58 @@ def test_lds_split_flag_and_shared_bits(k):
         compile_circ(circ, P.TileConfig(tb=6, low_bits=4, rb=4, lds_split=k))
     with pytest.raises(ValueError):
         compile_circ(circ, P.TileConfig(tb=6, low_bits=4, rb=3, specialize=True, lds_split=3))
+
+
+@pytest.mark.parametrize("n,layers,tb,cap,seed", [(12, 4, 5, 64, 0), (14, 8, 6, 1e9, 1), (16, 10, 6, 96, 2),
+                                                  (16, 6, 7, 160, 3)])
+def test_remap_programs_match_oracle(n, layers, tb, cap, seed):
+    """Programs that move qubits between address bits at each store (and
+    permute back at the end) give the oracle's state, for the brickwork and
+    for a random circuit; stores stay coalesced and cover each tile once."""
+    rng = np.random.default_rng(seed)
+    cfg = P.TileConfig(tb=tb, low_bits=4, max_cost=cap, specialize=True, io_lane_bits=4, rb=5, lds_split=1,
+                       remap=True)
+    for circ in (brickwork_circuit(n, layers), _random_tile_circuit(n, 80, rng)):
+        prog = compile_circ(circ, cfg)
+        assert prog.final_pos is not None
+        h, passes, stages, ops, _, _ = prog_interp.decode(prog.blob)
+        nremap = 0
+        for p in passes:
+            if not int(p['flags']) & P.PASSF_REMAP_STORE:
+                continue
+            nremap += 1
+            L, ptb, rb = int(p['L']), int(p['tb']), int(p['rb'])
+            rec = stages[int(p['stage_begin']) + int(p['n_stages'])]
+            assert [int(x) for x in rec['tgbit'][:4]] == [0, 1, 2, 3]      # 128-byte store runs
+            tile = {sum(((i >> k) & 1) << int(p['bits'][k]) for k in range(L)) for i in range(1 << L)}
+            seen = set()
+            for t in range(1 << ptb):
+                tg = sum(((t >> i) & 1) << int(rec['tgbit'][i]) for i in range(ptb))
+                for j in range(1 << rb):
+                    seen.add(tg | int(rec['goff'][j]))
+            assert seen == tile
+        assert nremap > 0
+        sv0 = np.zeros(2 ** n, complex)
+        sv0[0] = 1
+        sv, _ = prog_interp.run(prog.blob, sv0)
+        assert np.linalg.norm(sv - O.run(circ.gate_list, n)) < 1e-12
+
+
+def _random_tile_circuit(n, ng, rng):
+    gates = []
+    for _ in range(ng):
+        if rng.random() < 0.5:
+            q = int(rng.integers(n))
+            gates.append(Gate("U", [q], random_unitary(2, rng)))
+        else:
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            gates.append(CNOT([a, b], n) if rng.random() < 0.5 else Gate("U2", [a, b], random_unitary(4, rng)))
+    return Circuit(gates, n=n)
+
+
+def test_remap_falls_back_without_tile_ops():
+    circ = Circuit([H(0), CNOT([1, 0], 10), Reset(3, "0"), CNOT([2, 1], 10)], n=10)
+    cfg = P.TileConfig(tb=5, low_bits=4, max_cost=64, specialize=True, io_lane_bits=4, rb=5, remap=True)
+    prog = compile_circ(circ, cfg)
+    assert prog.final_pos is None
+    assert not any(p.flags & P.PASSF_REMAP_STORE for p in prog.passes)
