@@ -111,13 +111,37 @@ def bench_forward(args, dev):
     wall = time.perf_counter() - t0
     ms_step = 1e3 * wall / args.steps
     prog_ms = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
-    tile_ms = prog_ms / n_tile
     state_b = 2 * (2 ** n) * 8
-    alg_bytes_step = n_tile * state_b + (2 ** n) * 8
+    perm_ms, perm_sweeps = 0.0, 0
+    if prog.final_pos is not None:
+        # a remapped program ends with a bit permutation through scratch and a
+        # copy back (two sweeps); time that part alone, same entry point and
+        # sizes, so the tile-pass average excludes it
+        import ctypes
+        back = [0] * n
+        for q, ph in enumerate(prog.final_pos):
+            back[ph] = q
+        tmp = torch.empty_like(psi)
+        pev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+        for k in range(args.steps):
+            pev[k][0].record(st)
+            _lib.check(lib.svh_permute_bits(_lib.ptr(psi), _lib.ptr(tmp), n, 0, (ctypes.c_int * n)(*back),
+                                            _lib.stream_ptr(dev)), "svh_permute_bits")
+            psi.copy_(tmp)
+            pev[k][1].record(st)
+        torch.cuda.synchronize(dev)
+        perm_ms = sum(e[0].elapsed_time(e[1]) for e in pev) / args.steps
+        perm_sweeps = 2
+        del tmp
+        # the timing loop above permuted psi again: recompute the result
+        step()
+        torch.cuda.synchronize(dev)
+    tile_ms = (prog_ms - perm_ms) / n_tile
+    alg_bytes_step = (n_tile + perm_sweeps) * state_b + (2 ** n) * 8
     return dict(n=n, layers=layers, gates=n_gates, cfg=cfg, prog=prog, ms_step=ms_step,
                 gbps=alg_bytes_step / (ms_step * 1e-3) / 1e9, gates_per_s=n_gates / (ms_step * 1e-3),
                 tile_ms=tile_ms, tile_gbps=state_b / (tile_ms * 1e-3) / 1e9, n_tile=n_tile,
-                state_bytes=state_b, psi=psi)
+                state_bytes=state_b, psi=psi, perm_ms=perm_ms, sweeps=n_tile + perm_sweeps)
 
 
 def check_forward(res, dev):
@@ -590,8 +614,11 @@ def main():
                    "tile": {"tb": fwd["cfg"].tb, "low_bits": fwd["cfg"].low_bits,
                             "max_cost": fwd["cfg"].max_cost, "fuse2q": fwd["cfg"].fuse2q,
                             "io_lane_bits": fwd["cfg"].io_lane_bits, "register_bits": fwd["cfg"].rb,
-                            "lds_split": fwd["cfg"].lds_split, "remap": fwd["cfg"].remap},
-                   "fused_passes": fwd["n_tile"]},
+                            "lds_split": fwd["cfg"].lds_split, "remap": fwd["cfg"].remap,
+                            "remap_free": fwd["cfg"].remap_free},
+                   "fused_passes": fwd["n_tile"],
+                   "state_sweeps": fwd["sweeps"],
+                   "final_permutation_ms": round(fwd["perm_ms"], 3)},
         "gates_per_s": round(fwd["gates_per_s"], 1),
         "c64_vs_c128_l2_err": err,
         "roofline": {"bound": "hbm", "achieved": round(fwd["tile_gbps"], 1), "peak": peak, "unit": "GB/s",

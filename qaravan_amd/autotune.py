@@ -49,7 +49,8 @@ def best_tile_config(n, dtype):
         cfg = TileConfig(tb=e["tb"], low_bits=e["low_bits"], max_cost=e["max_cost"],
                          prefetch=e.get("prefetch", False), specialize=e.get("specialize", False),
                          fuse2q=e.get("fuse2q", True), io_lane_bits=e.get("io_lane_bits", 6), rb=e.get("rb", 4),
-                         lds_split=e.get("lds_split", 0), remap=e.get("remap", False))
+                         lds_split=e.get("lds_split", 0), remap=e.get("remap", False),
+                         remap_free=e.get("remap_free", -1))
     return replace(cfg, tb=min(cfg.tb, n - cfg.rb))
 
 
@@ -68,17 +69,18 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_
     psi = torch.empty(2 ** n, dtype=dtype, device=device)
     scratch = DeviceScratch(device)
     results = []
-    for tb, lb, mc, pf, spz, f2, io, rb, hl, rm in itertools.product(
+    for tb, lb, mc, pf, spz, f2, io, rb, hl, rm, rf in itertools.product(
             space["tb"], space["low_bits"], space["max_cost"], space.get("prefetch", [False]),
             space.get("specialize", [False]), space.get("fuse2q", [True]), space.get("io_lane_bits", [6]),
-            space.get("rb", [4]), space.get("lds_split", [0]), space.get("remap", [False])):
+            space.get("rb", [4]), space.get("lds_split", [0]), space.get("remap", [False]),
+            space.get("remap_free", [-1])):
         if tb + rb > n or (dtype == torch.complex128 and tb > (8 if pf else 9)) or (pf and tb > 9):
             continue
         if (pf and spz) or (rb == 5 and (pf or not spz)) or (hl and (pf or not spz or hl >= rb)) or \
-                (rm and (pf or not spz)):
+                (rm and (pf or not spz)) or (not rm and rf != space.get("remap_free", [-1])[0]):
             continue
         cfg = TileConfig(tb=tb, low_bits=lb, max_cost=mc, prefetch=pf, specialize=spz, fuse2q=f2, io_lane_bits=io,
-                         rb=rb, lds_split=hl, remap=rm)
+                         rb=rb, lds_split=hl, remap=rm, remap_free=rf)
         prog = compile_ops(ops, n, cfg)
         times = []
         for r in range(repeats + 1):
@@ -93,7 +95,7 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_
         t = min(times)
         results.append((t, cfg, prog.n_tile_passes()))
         log(f"[tune] n={n} {dtype} tb={tb} rb={rb} low={lb} cap={mc:g} prefetch={pf} specialize={spz} fuse2q={f2} "
-            f"io={io} lds_split={hl} remap={rm}: "
+            f"io={io} lds_split={hl} remap={rm}/{rf}: "
             f"{t * 1e3:.3f} ms, {prog.n_tile_passes()} passes")
     results.sort(key=lambda x: x[0])
     t, cfg, npass = results[0]
@@ -102,7 +104,7 @@ def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_
         cache[_key(n, dtype)] = dict(tb=cfg.tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost,
                                      prefetch=cfg.prefetch, specialize=cfg.specialize, fuse2q=cfg.fuse2q,
                                      io_lane_bits=cfg.io_lane_bits, rb=cfg.rb, lds_split=cfg.lds_split,
-                                     remap=cfg.remap,
+                                     remap=cfg.remap, remap_free=cfg.remap_free,
                                      ms=t * 1e3,
                                      passes=npass)
         with open(out_path or CACHE, "w") as f:

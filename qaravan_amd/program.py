@@ -336,6 +336,10 @@ class TileConfig:
               permuted, so that the qubits the next passes need most come to
               the low address bits every tile holds; fewer passes, plus one
               permutation back to the natural order at the end.
+    remap_free: a remapped pass takes the lanes its last layout already has
+              when they hold at least this many of the qubits wanted next
+              (no extra layout change); otherwise it adds one layout change
+              to bring the wanted qubits to the lanes (-1: always).
     """
     tb: int = 8
     low_bits: int = 6
@@ -347,6 +351,7 @@ class TileConfig:
     rb: int = RB
     lds_split: int = 0
     remap: bool = False
+    remap_free: int = -1
 
 
 def _is_barrier(op):
@@ -490,22 +495,36 @@ def partition_remap(ops, n, cfg):
         rem = [op for i, op in enumerate(rem) if i not in tset]
         Ts = sorted(T)
         inv = {ph: q for q, ph in enumerate(pos)}
-        # qubits for the low bits: needed soonest, from the tile's low + rb lowest bits
-        cand = [inv[ph] for ph in Ts[:low + cfg.rb]]
-        new_low = []
+        # qubits wanted on the low bits: those the remaining ops need soonest
+        wanted = []
         for op in rem:
-            if _is_barrier(op):
+            if _is_barrier(op) or len(wanted) >= low + 2:
                 break
             for q in op.bits:
-                if q in cand and q not in new_low:
+                if q not in wanted:
+                    wanted.append(q)
+        p = Pass(PASS_TILE, Ts, pops, tb=cfg.tb, rb=cfg.rb)
+        _stage_ops(p, cfg.io_lane_bits, keep_shared=cfg.lds_split, free_store=True)
+        good = {Ts.index(pos[q]) for q in wanted if pos[q] in T}
+        lanes = _remap_last_stage(p, good, cfg.lds_split, low)
+        # lanes the last layout gives for free, or the soonest-needed qubits of
+        # the tile's low + rb lowest bits behind one more layout change
+        top = [Ts.index(pos[q]) for q in wanted[:low] if pos[q] in T and Ts.index(pos[q]) < low + cfg.rb]
+        if cfg.remap_free < 0 or sum(x in lanes for x in top) < min(len(top), cfg.remap_free):
+            cand = {inv[ph] for ph in Ts[:low + cfg.rb]}
+            new_low = []
+            for op in rem:
+                if _is_barrier(op) or len(new_low) >= low:
+                    break
+                new_low += [q for q in op.bits if q in cand and q not in new_low]
+            new_low = new_low[:low]
+            for q in [inv[ph] for ph in range(low)]:
+                if len(new_low) < low and q not in new_low:
                     new_low.append(q)
-            if len(new_low) >= low:
-                break
-        new_low = new_low[:low]
-        for q in [inv[ph] for ph in range(low)]:     # keep current low qubits otherwise
-            if len(new_low) < low and q not in new_low:
-                new_low.append(q)
-        lanes = sorted(Ts.index(pos[q]) for q in new_low)   # tile-local, ascending
+            lanes = sorted(Ts.index(pos[q]) for q in new_low)
+            p = Pass(PASS_TILE, Ts, pops, tb=cfg.tb, rb=cfg.rb)
+            p.store_lanes = lanes
+            _stage_ops(p, cfg.io_lane_bits, keep_shared=cfg.lds_split)
         dest = {x: ph for x, ph in enumerate(Ts)}
         vacated = sorted(Ts[x] for x in lanes if Ts[x] >= low)
         displaced = sorted(x for x in range(low) if x not in lanes)
@@ -513,7 +532,6 @@ def partition_remap(ops, n, cfg):
             dest[x] = ph
         for i, x in enumerate(lanes):
             dest[x] = i
-        p = Pass(PASS_TILE, Ts, pops, tb=cfg.tb, rb=cfg.rb)
         if any(dest[x] != Ts[x] for x in range(L)):
             p.store_bits = [dest[x] for x in range(L)]
             p.store_lanes = lanes
@@ -527,10 +545,12 @@ def partition_remap(ops, n, cfg):
     return passes, pos
 
 
-def _stage_ops(p, io_lane_bits=6, keep_shared=0):
+def _stage_ops(p, io_lane_bits=6, keep_shared=0, free_store=False):
     """Split a tile pass's ops into register stages (tile-local bits).
     keep_shared: every stage keeps at least this many register bits of the
-    stage before it (split LDS exchanges); the ops of a stage leave room."""
+    stage before it (split LDS exchanges); the ops of a stage leave room.
+    free_store: the store may remap bits (the lanes of the last layout go to
+    the low address bits), so the last layout needs no extra stage."""
     RB = p.rb
     L = p.tb + RB
     loc = {g: i for i, g in enumerate(p.bits)}
@@ -606,10 +626,58 @@ def _stage_ops(p, io_lane_bits=6, keep_shared=0):
                 if x not in R and x not in sl:
                     R.append(x)
             stages.append(Stage(R, []))
-    elif set(stages[-1].rbits) & lane_bits:
+    elif set(stages[-1].rbits) & lane_bits and not free_store:
         stages.append(Stage(_io_layout(stages[-1].rbits, io_bits, lane_bits, keep_shared), []))
     p.stages = stages
     return p
+
+
+def _remap_last_stage(p, good, keep_shared, low):
+    """Register bits of a remapped pass's last layout, and its lanes (the
+    ``low`` lowest non-register tile bits, which the store sends to address
+    bits 0..low-1 in order).  The bits the last ops use stay in registers; the
+    free register slots first keep ``keep_shared`` bits of the layout before
+    (split LDS exchanges), then take low tile bits whose qubits are not in
+    ``good`` (wanted soon), so that wanted qubits become lanes; the rest are
+    high tile bits."""
+    last = p.stages[-1]
+    RB = p.rb
+    L = p.tb + RB
+    used = {x for _, lb in last.ops for x in lb}
+    prev = set(p.stages[-2].rbits) if len(p.stages) > 1 else set()
+    R = set(used)
+    # shared bits for the split exchange: prefer unwanted low ones
+    need = max(0, keep_shared - len(R & prev)) if prev else 0
+    for x in sorted(prev - R, key=lambda x: (x in good, -x if x >= low + RB else x)):
+        if need == 0 or len(R) == RB:
+            break
+        R.add(x)
+        need -= 1
+    # push unwanted low bits into registers so wanted ones become lanes
+    lanes = []
+    for x in range(L):
+        if len(lanes) == low:
+            break
+        if x in R:
+            continue
+        if x not in good and len(R) < RB and any(g > x and g not in R for g in good):
+            R.add(x)
+        else:
+            lanes.append(x)
+    for x in range(L - 1, -1, -1):                  # fill above the lanes
+        if len(R) == RB:
+            break
+        if x not in R and x not in lanes and x > lanes[-1]:
+            R.add(x)
+    for x in range(L - 1, -1, -1):                  # (cannot happen unless L is tiny)
+        if len(R) == RB:
+            break
+        if x not in R and x not in lanes:
+            R.add(x)
+    ordered = [x for x in last.rbits if x in R] + sorted(R - set(last.rbits))
+    p.stages[-1] = Stage(ordered[:RB], last.ops)
+    nonreg = [x for x in range(L) if x not in R]
+    return nonreg[:low]
 
 
 def _io_layout(nb, io_bits, lane_bits, keep_shared):
@@ -834,7 +902,8 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
              and not any(_is_barrier(op) for op in ops))
     cfg = TileConfig(tb=tb, low_bits=max(1, cfg.low_bits), max_cost=cfg.max_cost,
                      prefetch=cfg.prefetch and not grad, specialize=cfg.specialize, fuse2q=cfg.fuse2q,
-                     io_lane_bits=cfg.io_lane_bits, rb=cfg.rb, lds_split=split, remap=remap)
+                     io_lane_bits=cfg.io_lane_bits, rb=cfg.rb, lds_split=split, remap=remap,
+                     remap_free=cfg.remap_free)
     final_pos = None
     if remap:
         passes, final_pos = partition_remap(ops, n, cfg)
@@ -842,7 +911,8 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
         passes = partition(ops, n, cfg)
     for p in passes:
         if p.kind == PASS_TILE:
-            _stage_ops(p, cfg.io_lane_bits, keep_shared=split)
+            if not remap:
+                _stage_ops(p, cfg.io_lane_bits, keep_shared=split)
             shared = min((len(set(a.rbits) & set(b.rbits)) for a, b in zip(p.stages, p.stages[1:])), default=RB)
             if split >= 2 and shared >= 2:
                 p.flags |= PASSF_QUARTER_LDS
