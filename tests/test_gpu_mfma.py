@@ -78,7 +78,7 @@ def test_block5x2_rejects_bad_arguments(gpu):
                                   _lib.ptr(ud), None) == -2
 
 
-@pytest.mark.parametrize("n,layers", [(12, 3), (15, 6), (20, 5)])
+@pytest.mark.parametrize("n,layers", [(12, 3), (15, 6), (20, 5), (17, 12)])
 def test_haar_block_circuit_vs_oracle(gpu, n, layers):
     from qaravan_amd import StatevectorSim
     circ = haar_block_circuit(n, layers)
