@@ -102,7 +102,10 @@ int svh_run_program(void *psi, int n, int dtype, const void *prog_host,
  * every op spelled out (qaravan_amd/specialize.py).  svh_sp_load resolves the
  * kernels of n_passes passes across the given code-object files;
  * svh_run_program_sp runs the program launching them where present and the
- * generic kernels elsewhere.  The handle owns the loaded modules. */
+ * generic kernels elsewhere.  The handle owns the loaded modules.  Kernel
+ * signature: svh_sp_<p>(psi, n, pass, stages, ops, trig); in a program with a
+ * final permutation (header off_perm), the last pass's kernel takes an output
+ * pointer after psi and stores there (the scratch tail the permutation reads). */
 int svh_sp_load(const char *const *paths, int npaths, int n_passes, void **handle);
 int svh_sp_free(void *handle);
 int svh_run_program_sp(void *handle, void *psi, int n, int dtype, const void *prog_host,

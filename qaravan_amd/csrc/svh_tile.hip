@@ -743,7 +743,7 @@ struct SpHandle {
 // the adjoint, svh_spd_<p>(psi, lam, n, pass, st, ops, trig, gpart).
 static int launch_sp(hipFunction_t f, void *psi, void *lam, int n, int dtype, const svh_pass_t &hp,
                      const svh_pass_t *dp, const svh_stage_t *dst, const svh_op_t *dop, const svh_stage_t *hst,
-                     const void *trig, double *gpart, hipStream_t st) {
+                     const void *trig, double *gpart, hipStream_t st, void *out = nullptr) {
     const int L = hp.L;
     const size_t es = dtype == SVH_C64 ? 8 : 16;
     const size_t nw = ((size_t)1 << hp.tb) >= 64 ? ((size_t)1 << hp.tb) / 64 : 1;
@@ -759,6 +759,11 @@ static int launch_sp(hipFunction_t f, void *psi, void *lam, int n, int dtype, co
         void *args[] = {&psi, &lam, &nn, (void *)&dp, (void *)&stp, (void *)&opp, (void *)&trig, (void *)&gpart};
         return hip_status(hipModuleLaunchKernel(f, grid, 1, 1, 1u << hp.tb, 1, 1, (unsigned)lds, st, args, nullptr),
                           "hipModuleLaunchKernel(svh_spd)");
+    }
+    if (out) {  // the out-of-place last pass of a remapped program
+        void *args[] = {&psi, &out, &nn, (void *)&dp, (void *)&stp, (void *)&opp, (void *)&trig};
+        return hip_status(hipModuleLaunchKernel(f, grid, 1, 1, 1u << hp.tb, 1, 1, (unsigned)lds, st, args, nullptr),
+                          "hipModuleLaunchKernel(svh_sp)");
     }
     void *args[] = {&psi, &nn, (void *)&dp, (void *)&stp, (void *)&opp, (void *)&trig};
     return hip_status(hipModuleLaunchKernel(f, grid, 1, 1, 1u << hp.tb, 1, 1, (unsigned)lds, st, args, nullptr),
@@ -777,6 +782,10 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
     const size_t need = scratch_need(h, dtype, dual);
     if (!scratch || scratch_bytes < need)
         return fail(-5, "scratch too small: need " + std::to_string(need) + " bytes");
+    // a remapped program's final permutation goes through the scratch tail
+    const size_t perm_bytes = h->off_perm ? (dtype == SVH_C64 ? 8 : 16) * ((size_t)1 << n) : 0;
+    void *perm_tmp = h->off_perm ? static_cast<char *>(scratch) + (need - perm_bytes) : nullptr;
+    bool last_in_tmp = false;
     if (h->n_params > 0 && !theta_dev) return fail(-1, "program has rotation ops but theta is NULL");
     const char *hb = static_cast<const char *>(prog_host);
     const char *db = static_cast<const char *>(prog_dev);
@@ -839,8 +848,12 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
             goff += ((size_t)1 << (n - P.L)) * P.n_grad;
         }
         if (P.kind == SVH_PASS_TILE && sp && sp->fn[p]) {
+            // a remapped program's last compiled pass stores into the scratch
+            // tail, which the final permutation reads
+            void *out = (!dual && h->off_perm && p == h->n_passes - 1) ? perm_tmp : nullptr;
             e = launch_sp(sp->fn[p], psi, dual ? lam : nullptr, n, dtype, P, dpass + p, dstage, dop, hstage, trig,
-                          gp, st);
+                          gp, st, out);
+            if (out) last_in_tmp = true;
         } else if (P.kind == SVH_PASS_TILE && (P.flags & SVH_PASSF_REMAP_STORE)) {
             return fail(-8, "remapped passes run only in their compiled kernels (svh_run_program_sp)");
         } else if (P.kind == SVH_PASS_TILE) {
@@ -891,15 +904,15 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
         return checked_launch("k_grad_scatter");
     }
     if (h->off_perm) {
-        // back to the natural order: physical bit perm[q] -> bit q, through
-        // the state-sized tail of scratch, then copied back in place
+        // back to the natural order: physical bit perm[q] -> bit q.  The last
+        // pass normally left the state in the scratch tail (one permutation
+        // sweep into psi); otherwise permute into the tail and copy back.
         const int32_t *perm = reinterpret_cast<const int32_t *>(hb + h->off_perm);
         std::vector<int> back(n);
         for (int q = 0; q < n; ++q) back[perm[q]] = q;
-        const size_t es = dtype == SVH_C64 ? 8 : 16, bytes = es << n;
-        void *tmp = static_cast<char *>(scratch) + (need - bytes);
-        if (int e = launch_permute(psi, tmp, n, dtype, back.data(), st)) return e;
-        return hip_status(hipMemcpyAsync(psi, tmp, bytes, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync");
+        if (last_in_tmp) return launch_permute(perm_tmp, psi, n, dtype, back.data(), st);
+        if (int e = launch_permute(psi, perm_tmp, n, dtype, back.data(), st)) return e;
+        return hip_status(hipMemcpyAsync(psi, perm_tmp, perm_bytes, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync");
     }
     return 0;
 }

@@ -127,7 +127,7 @@ def eligible(p, dtype_code, dual=False):
             and lds_bytes(p, dtype_code, dual) <= MAX_LDS and (dual or int(p['n_grad']) == 0))
 
 
-def _body(R, p, stages, ops, dual):
+def _body(R, p, stages, ops, dual, store_to="psi"):
     """Statements of one pass after the prologue has defined psi (and lam),
     n, pass, st (this pass's stages), ops (this pass's ops), trig, gpart and
     wg (the tile index)."""
@@ -205,25 +205,35 @@ def _body(R, p, stages, ops, dual):
     elif int(p['flags']) & P.PASSF_REMAP_STORE:
         # the store permutes the tile's bits: every wave must have loaded
         # before any wave stores (with one stage no exchange barrier did it)
-        if ns == 1:
+        if ns == 1 and store_to == "psi":
             out.append("    __syncthreads();")
-        out.append(f"    sp_store<{R}, {tb}, {rb}>(psi, base, st[{ns}], v);")
+        out.append(f"    sp_store<{R}, {tb}, {rb}>({store_to}, base, st[{ns}], v);")
     else:
-        out.append(f"    sp_store<{R}, {tb}, {rb}>(psi, base, st[{ns - 1}], v);")
+        out.append(f"    sp_store<{R}, {tb}, {rb}>({store_to}, base, st[{ns - 1}], v);")
     return out
 
 
-def _pass_kernel(R, p, stages, ops, dual, name):
-    """A kernel for one pass of one program: tile index = blockIdx.x."""
+def _pass_kernel(R, p, stages, ops, dual, name, out_of_place=False):
+    """A kernel for one pass of one program: tile index = blockIdx.x.
+    out_of_place: the pass reads psi and stores into ``out`` (the last pass
+    of a program that ends with a bit permutation: the permutation then reads
+    ``out`` and writes psi, one sweep instead of a permutation and a copy)."""
     tb = int(p['tb'])
     lam = f"cpx<{R}> *__restrict__ lam, " if dual else ""
+    dst = f"cpx<{R}> *__restrict__ out, " if out_of_place else ""
     gp = ", double *__restrict__ gpart" if dual else ""
     head = [f'extern "C" __global__ void __launch_bounds__({1 << tb}) '
-            f"{name}(cpx<{R}> *__restrict__ psi, {lam}int n, const svh_pass_t *__restrict__ pass, "
+            f"{name}(cpx<{R}> *__restrict__ psi, {dst}{lam}int n, const svh_pass_t *__restrict__ pass, "
             f"const svh_stage_t *__restrict__ st, const svh_op_t *__restrict__ ops, "
             f"const cpx<{R}> *__restrict__ trig{gp}) {{",
             "    const uint64_t wg = blockIdx.x;"]
-    return head + _body(R, p, stages, ops, dual) + ["}", ""]
+    return head + _body(R, p, stages, ops, dual, "out" if out_of_place else "psi") + ["}", ""]
+
+
+def stores_out_of_place(h, pi):
+    """The compiled kernel of pass ``pi`` stores out of place: the last pass
+    of a program that ends with a bit permutation (header off_perm set)."""
+    return int(h['off_perm']) != 0 and pi == int(h['n_passes']) - 1
 
 
 def batch_body(blob, pi, dtype_code, dual):
@@ -275,7 +285,7 @@ def program_source(blob, dtype_code, only=None):
     for pi, p in enumerate(passes):
         if not eligible(p, dtype_code) or (only is not None and pi not in only):
             continue
-        out += _pass_kernel(R, p, stages, ops, False, f"svh_sp_{pi}")
+        out += _pass_kernel(R, p, stages, ops, False, f"svh_sp_{pi}", stores_out_of_place(h, pi))
     return "\n".join(out)
 
 
