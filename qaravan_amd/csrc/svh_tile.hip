@@ -18,7 +18,7 @@
 
 namespace svh {
 
-constexpr int kPermTileBits = 10;  // bit-permutation tiles: 1024 amplitudes
+constexpr int kPermTileBits = 11;  // bit-permutation tiles: 2048 amplitudes
 
 // ------------------------------------------------ the pass kernel
 // One tile of one pass: workgroup-local work shared by k_tile (one circuit,
