@@ -61,7 +61,7 @@ def parse():
 
 
 # LDS parts of the compiled batch kernels (log2), chosen by measurement
-BATCH_LDS_SPLIT = 0
+BATCH_LDS_SPLIT = 1
 
 
 def log(*a):
