@@ -217,7 +217,8 @@ k_block5x2(cpx<float> *__restrict__ psi, const cpx<float> *__restrict__ UA, cons
             p2 = __builtin_amdgcn_mfma_f32_32x32x2f32(Fi[t], x.y, p2, 0, 0, 0);
             p3 = __builtin_amdgcn_mfma_f32_32x32x2f32(Fs[t], x.x + x.y, p3, 0, 0, 0);
         }
-        __syncthreads();
+        // a wave reads and writes only its own 32 columns (the swizzle keeps
+        // them in its own slots), so no barrier separates its reads and writes
         // D layout: row i = (v & 3) + 4 hi + 8 (v >> 2) of register v
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
@@ -237,7 +238,6 @@ k_block5x2(cpx<float> *__restrict__ psi, const cpx<float> *__restrict__ UA, cons
             q2 = __builtin_amdgcn_mfma_f32_32x32x2f32(w.y, x.y, q2, 0, 0, 0);
             q3 = __builtin_amdgcn_mfma_f32_32x32x2f32(w.z, x.x + x.y, q3, 0, 0, 0);
         }
-        __syncthreads();
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int i = (v & 3) + 4 * hi + 8 * (v >> 2);

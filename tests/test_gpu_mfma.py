@@ -40,7 +40,9 @@ def test_block5_direct_vs_oracle(gpu, bits):
     (14, (0, 1, 2, 3, 4), (5, 6, 7, 8, 9)),          # one contiguous 4096-amplitude tile
     (20, (10, 11, 12, 13, 14), (15, 16, 17, 18, 19)),  # 32-byte runs, sibling tiles grouped per XCD
     (20, (15, 16, 17, 18, 19), (2, 3, 4, 5, 6)),      # lowest other bits 0, 1; no sibling grouping
-    (20, (3, 4, 5, 6, 7), (15, 16, 17, 18, 19)),      # bit 2 outside the tile: pairs of sibling tiles
+    (20, (3, 4, 5, 6, 7), (15, 16, 17, 18, 19)),      # three spare bits (0, 1, 2): 64-byte runs
+    (16, (3, 4, 5, 6, 7), (11, 12, 13, 14, 15)),      # three spare bits, whole-line runs
+    (13, (8, 9, 10, 11, 12), (3, 4, 5, 6, 7)),        # three spare bits, a single 8192-amplitude tile
     (17, (3, 9, 1, 13, 6), (16, 0, 11, 2, 8)),        # scrambled bits
     (12, (11, 10, 9, 8, 7), (6, 5, 4, 3, 2)),         # single tile
 ])
