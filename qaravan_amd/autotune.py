@@ -126,7 +126,7 @@ def best_adjoint_config(n, dtype):
     else:
         cfg = TileConfig(tb=e["tb"], low_bits=e["low_bits"], max_cost=e["max_cost"],
                          specialize=e.get("specialize", False), io_lane_bits=e.get("io_lane_bits", 6),
-                         rb=e.get("rb", 4), lds_split=e.get("lds_split", 0))
+                         rb=e.get("rb", 4), lds_split=e.get("lds_split", 0), remap=e.get("remap", False))
     return replace(cfg, tb=min(cfg.tb, n - cfg.rb))
 
 
@@ -142,12 +142,13 @@ def tune_adjoint(n, layers, dtype, device, space=None, repeats=5, log=print, out
     circ = hea_circuit(n, layers, theta)
     th = torch.tensor(theta, device=device)
     results = []
-    for tb, mc, spz, io, rb, ls in itertools.product(space["tb"], space["max_cost"], space["specialize"],
-                                                     space.get("io_lane_bits", [6]), space.get("rb", [4]),
-                                                     space.get("lds_split", [0])):
-        if tb + rb > n or (rb != 4 and not spz) or (ls and (not spz or ls >= rb)):
+    for tb, mc, spz, io, rb, ls, rm in itertools.product(space["tb"], space["max_cost"], space["specialize"],
+                                                         space.get("io_lane_bits", [6]), space.get("rb", [4]),
+                                                         space.get("lds_split", [0]), space.get("remap", [False])):
+        if tb + rb > n or (rb != 4 and not spz) or (ls and (not spz or ls >= rb)) or (rm and not spz):
             continue
-        cfg = TileConfig(tb=tb, low_bits=4, max_cost=mc, specialize=spz, io_lane_bits=io, rb=rb, lds_split=ls)
+        cfg = TileConfig(tb=tb, low_bits=4, max_cost=mc, specialize=spz, io_lane_bits=io, rb=rb, lds_split=ls,
+                         remap=rm)
         m = AdjointModel(circ, tfim_terms(n), len(theta), device=device, dtype=dtype, cfg=cfg)
         for _ in range(2):
             m.energy_and_grad(th)
@@ -158,7 +159,7 @@ def tune_adjoint(n, layers, dtype, device, space=None, repeats=5, log=print, out
         torch.cuda.synchronize(device)
         t = (time.perf_counter() - t0) / repeats
         results.append((t, cfg))
-        log(f"[tune-adjoint] n={n} tb={tb} rb={rb} cap={mc:g} specialize={spz} io={io} lds_split={ls}: "
+        log(f"[tune-adjoint] n={n} tb={tb} rb={rb} cap={mc:g} specialize={spz} io={io} lds_split={ls} remap={rm}: "
             f"{t * 1e3:.3f} ms")
     results.sort(key=lambda x: x[0])
     t, cfg = results[0]
@@ -167,7 +168,8 @@ def tune_adjoint(n, layers, dtype, device, space=None, repeats=5, log=print, out
                                                                      max_cost=cfg.max_cost,
                                                                      specialize=cfg.specialize,
                                                                      io_lane_bits=cfg.io_lane_bits, rb=cfg.rb,
-                                                                     lds_split=cfg.lds_split, ms=t * 1e3)
+                                                                     lds_split=cfg.lds_split, remap=cfg.remap,
+                                                                     ms=t * 1e3)
     with open(out_path or CACHE, "w") as f:
         json.dump(cache, f, indent=1, sort_keys=True)
     return cfg, results

@@ -96,8 +96,8 @@ k_block5(cpx<float> *__restrict__ psi, const cpx<float> *__restrict__ U, Block5A
             p2 = __builtin_amdgcn_mfma_f32_32x32x2f32(Fi[t], x.y, p2, 0, 0, 0);
             p3 = __builtin_amdgcn_mfma_f32_32x32x2f32(Fs[t], x.x + x.y, p3, 0, 0, 0);
         }
-        // every wave has read its slab before any wave overwrites LDS
-        __syncthreads();
+        // a wave reads and writes only its own 32 columns, so no barrier
+        // separates its reads and writes
         // D layout: row i = (v & 3) + 4 hi + 8 (v >> 2) of register v
 #pragma unroll
         for (int v = 0; v < 16; ++v) {

@@ -28,7 +28,8 @@ def hea_spec(n, layers):
                                                     (12, 2, 8, False, 4, 0), (10, 3, 6, True, 4, 0),
                                                     (12, 2, 5, True, 4, 0), (11, 3, 7, True, 3, 0),
                                                     (12, 4, 9, True, 3, 0), (10, 3, 6, True, 4, 1),
-                                                    (12, 4, 9, True, 3, 1), (12, 2, 7, True, 4, 2)])
+                                                    (12, 4, 9, True, 3, 1), (12, 2, 7, True, 4, 2),
+                                                    (14, 4, 8, True, 3, "remap"), (13, 3, 7, True, 4, "remap")])
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
 def test_adjoint_vs_oracle(gpu, n, layers, tb, spz, rb, split, dtype):
     from qaravan_amd.adjoint import AdjointModel
@@ -37,7 +38,11 @@ def test_adjoint_vs_oracle(gpu, n, layers, tb, spz, rb, split, dtype):
     terms = tfim_terms(n)
     print(f"adjoint n={n} layers={layers} tb={tb} rb={rb} specialize={spz} lds_split={split} {dtype}", flush=True)
     model = AdjointModel(circ, terms, len(theta), device=gpu, dtype=dtype,
-                         cfg=P.TileConfig(tb=tb, low_bits=6, max_cost=48, specialize=spz, rb=rb, lds_split=split))
+                         cfg=P.TileConfig(tb=tb, low_bits=6 if split != "remap" else 4, max_cost=48, specialize=spz,
+                                          rb=rb, lds_split=0 if split == "remap" else split,
+                                          remap=split == "remap", io_lane_bits=4 if split == "remap" else 6))
+    if split == "remap":
+        assert model.pp.fwd.final_pos is not None
     th = torch.tensor(theta, device=gpu, requires_grad=True)
     e = model(th)
     e.backward()
