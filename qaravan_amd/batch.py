@@ -33,10 +33,11 @@ TERM_DT = np.dtype([('x', '<u8'), ('z', '<u8'), ('cre', '<f8'), ('cim', '<f8')])
 assert ITEM_DT.itemsize == 72 and RED_DT.itemsize == 48 and HAM_DT.itemsize == 48
 
 
-def default_tb(n, dtype):
-    """Workgroup size per circuit: a 12-bit tile (256 threads) once the state
-    has room for it, otherwise the whole state in one tile."""
-    return max(4, min(8, n - RB))
+def default_tb(n, dtype, rb=RB):
+    """Workgroup size per circuit: a 12-bit tile (256 threads with 4 register
+    bits, 512 with 3) once the state has room for it, otherwise the whole
+    state in one tile."""
+    return max(4, min(12 - rb, n - rb))
 
 
 def _term_table(n, terms):
@@ -55,7 +56,8 @@ class BatchedAdjoint:
     circuits (rotations carry ``param_index`` into their own θ_c)."""
 
     def __init__(self, circuits, terms_list, n_params_list, device="cuda", dtype=torch.complex64,
-                 tb_of_n=None, max_cost=1e9, keys=None, op_lists=None, ns=None, specialize=False, lds_split=0):
+                 tb_of_n=None, max_cost=1e9, keys=None, op_lists=None, ns=None, specialize=False, lds_split=0,
+                 rb=RB):
         """``circuits``: qaravan-style circuits (rotations carry ``param_index``),
         or None with ``op_lists`` (elementary ops) and ``ns``.  Circuits with equal
         ``keys`` share compiled programs (θ lives on the device, so programs
@@ -63,14 +65,20 @@ class BatchedAdjoint:
         kernel compiled for its structure (one launch per structure and round)
         instead of the op-interpreting batch kernel.  ``lds_split`` (with
         ``specialize``): the compiled kernels move tiles through LDS in
-        2^lds_split parts, so more workgroups fit on a CU."""
+        2^lds_split parts, so more workgroups fit on a CU.  ``rb`` (with
+        ``specialize``): register bits per thread, 3 or 4."""
         _lib.require_gpu()
+        if rb != RB:
+            # 3 register bits passed the small-batch tests but the 512-circuit
+            # benchmark batch (up to 22 qubits, 32 layers) faulted the GPU once,
+            # cause not found: switched off until it is
+            raise NotImplementedError("batched programs run with 4 register bits only")
         if lds_split and not specialize:
             raise ValueError("lds_split needs the compiled batch kernels (specialize=True)")
         self.device = torch.device(device)
         self.dtype = dtype
         self.code = _lib.dtype_code(dtype)
-        tb_of_n = tb_of_n or (lambda n: default_tb(n, dtype))
+        tb_of_n = tb_of_n or (lambda n: default_tb(n, dtype, rb))
         nc = len(circuits) if circuits is not None else len(op_lists)
         self.nc = nc
         self.ns = [c.num_sites for c in circuits] if circuits is not None else [int(n) for n in ns]
@@ -85,7 +93,7 @@ class BatchedAdjoint:
                 continue
             n = self.ns[c]
             cfg = TileConfig(tb=tb_of_n(n), low_bits=4, max_cost=max_cost, io_lane_bits=4,
-                             specialize=bool(lds_split), lds_split=lds_split)
+                             specialize=bool(lds_split) or rb != RB, lds_split=lds_split, rb=rb)
             if circuits is not None:
                 pp = ParamProgram(circuits[c], terms_list[c], n_params_list[c], cfg=cfg, dtype=dtype)
             else:
@@ -132,7 +140,7 @@ class BatchedAdjoint:
             for p in pp.adj.passes:
                 if p.kind == PASS_TILE:
                     ng = sum(1 for st in p.stages for op, _ in st.ops if op.flags & 2)
-                    m = max(m, (2 ** (self.ns[c] - p.tb - RB)) * max(ng, 1))
+                    m = max(m, (2 ** (self.ns[c] - p.tb - p.rb)) * max(ng, 1))
             gp_sizes.append(m)
         self.gp_off = np.concatenate([[0], np.cumsum(gp_sizes)]).astype(np.int64)
         self.gpart = torch.zeros(int(self.gp_off[-1]), dtype=torch.float64, device=dev)
@@ -197,13 +205,13 @@ class BatchedAdjoint:
                         continue
                     p = prog.passes[r]
                     tb = p.tb
-                    L = tb + RB
+                    L = tb + p.rb
                     n = self.ns[c]
                     ppass, pst, pops, pgrad = self._pass_ptrs(self.prog_addr[c][which], prog, r)
                     ng = sum(1 for st in p.stages for op, _ in st.ops if op.flags & 2) if which else 0
                     gkey = (tb, self.kname[id(prog)][r]) if self.kernels is not None else (tb, None)
                     split = 1 if p.flags & PASSF_HALF_LDS else 2 if p.flags & PASSF_QUARTER_LDS else 0
-                    groups.setdefault(gkey, []).append(dict(split=split,
+                    groups.setdefault(gkey, []).append(dict(split=split, rb=p.rb,
                         psi=psi0 + es * int(self.amp_off[c]), lam=(lam0 + es * int(self.amp_off[c])) if which else 0,
                         pass_=ppass, stages=pst, ops=pops, trig=trig0 + es * int(self.par_off[c]),
                         gpart=gp0 + 8 * int(self.gp_off[c]), nwg=2 ** (n - L), n=n, ng=ng))
@@ -231,7 +239,7 @@ class BatchedAdjoint:
                         self.launches.append(('tile', add_table(arr), len(items), wg, tb, which, mng))
                     else:
                         # one kernel text per group, so one split for all its items
-                        L = tb + RB - items[0]['split']
+                        L = tb + items[0]['rb'] - items[0]['split']
                         nw = max(1, 2 ** tb // 64)
                         lds = (2 if which else 1) * es * 2 ** L + (8 * nw * mng if which else 0)
                         self.launches.append(('spb', add_table(arr), len(items), wg, tb, lds,
