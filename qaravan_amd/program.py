@@ -332,10 +332,11 @@ class TileConfig:
               the two layouts of a re-distribution share: 1/2^lds_split of the
               LDS per workgroup, so more workgroups per CU (the stage planner
               keeps that many shared bits).
-    remap: per-pass compiled forward kernels store each tile with its bits
+    remap: per-pass compiled kernels store each tile with its bits
               permuted, so that the qubits the next passes need most come to
               the low address bits every tile holds; fewer passes, plus one
-              permutation back to the natural order at the end.
+              permutation back to the natural order at the end (none for an
+              adjoint sweep, whose ψ and λ are scratch afterwards).
     remap_free: a remapped pass takes the lanes its last layout already has
               when they hold at least this many of the qubits wanted next
               (no extra layout change); otherwise it adds one layout change
@@ -898,7 +899,7 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
         raise ValueError("lds_split must be 0, 1 or 2 and below the register bits")
     tb = min(cfg.tb, n - cfg.rb)
     split = cfg.lds_split
-    remap = (cfg.remap and not grad and cfg.specialize and not cfg.prefetch
+    remap = (cfg.remap and cfg.specialize and not cfg.prefetch
              and not any(_is_barrier(op) for op in ops))
     cfg = TileConfig(tb=tb, low_bits=max(1, cfg.low_bits), max_cost=cfg.max_cost,
                      prefetch=cfg.prefetch and not grad, specialize=cfg.specialize, fuse2q=cfg.fuse2q,
@@ -907,6 +908,10 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
     final_pos = None
     if remap:
         passes, final_pos = partition_remap(ops, n, cfg)
+        if grad:
+            # an adjoint sweep only accumulates the gradient: the ψ and λ it
+            # leaves behind are scratch, so no permutation back at the end
+            final_pos = None
     else:
         passes = partition(ops, n, cfg)
     for p in passes:

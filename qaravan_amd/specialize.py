@@ -199,7 +199,10 @@ def _body(R, p, stages, ops, dual, store_to="psi"):
                 else:
                     raise ValueError(kind)
     if dual:
-        out.append(f"    sp_store2<{R}, {tb}, {rb}>(psi, lam, base, st[{ns - 1}], v, w);")
+        remap = int(p['flags']) & P.PASSF_REMAP_STORE
+        if remap and ns == 1:
+            out.append("    __syncthreads();")
+        out.append(f"    sp_store2<{R}, {tb}, {rb}>(psi, lam, base, st[{ns if remap else ns - 1}], v, w);")
         if ng:
             out.append(f"    sp_grad_flush_wg<{tb}>(gacc, {ng}, gpart, wg);")
     elif int(p['flags']) & P.PASSF_REMAP_STORE:

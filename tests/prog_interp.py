@@ -103,7 +103,10 @@ def run(blob, sv, theta=None, lam=None, grad=None):
                 for i, x in enumerate(thr):
                     store[x] = int(rec['tgbit'][i])
                 assert sorted(store) == sorted(tile_bits)
-                sv = permute_bits(sv, n, {tile_bits[x]: store[x] for x in range(L)})
+                move = {tile_bits[x]: store[x] for x in range(L)}
+                sv = permute_bits(sv, n, move)
+                if lam is not None:
+                    lam = permute_bits(lam, n, move)
     if int(h['off_perm']):
         off = int(h['off_perm'])
         perm = np.frombuffer(blob[off:off + 4 * n].tobytes(), dtype='<i4')

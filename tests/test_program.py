@@ -207,19 +207,27 @@ def _param_ops(circ, n):
     return P.lower_circuit(circ.build(), n, param_mode=True)
 
 
-@pytest.mark.parametrize("tb,layers,cost,rb", [(4, 3, 40, 4), (6, 3, 40, 4), (6, 6, 1e9, 4), (4, 5, 1e9, 4),
-                                               (5, 4, 1e9, 3), (7, 3, 40, 3)])
-def test_param_forward_and_adjoint_programs(tb, layers, cost, rb):
+@pytest.mark.parametrize("tb,layers,cost,rb,remap", [(4, 3, 40, 4, False), (6, 3, 40, 4, False),
+                                                     (6, 6, 1e9, 4, False), (4, 5, 1e9, 4, False),
+                                                     (5, 4, 1e9, 3, False), (7, 3, 40, 3, False),
+                                                     (5, 6, 1e9, 3, True), (4, 8, 40, 4, True)])
+def test_param_forward_and_adjoint_programs(tb, layers, cost, rb, remap):
     """Forward program with device-θ rotations, and the adjoint program's
     gradient, interpreted on the CPU, against the oracle's adjoint (RZ and CZ
-    are diagonal, so the scheduler may reorder them)."""
+    are diagonal, so the scheduler may reorder them).  With remap, both
+    programs move qubits between address bits; the forward permutes back at
+    the end, the adjoint sweep leaves ψ and λ permuted."""
     from qaravan_amd.adjoint import ParamProgram
     n = 10
     theta = hea_theta(n, layers)
     circ = hea_circuit(n, layers, theta)
     terms = tfim_terms(n)
-    pp = ParamProgram(circ, terms, len(theta), cfg=P.TileConfig(tb=tb, low_bits=6, max_cost=cost, rb=rb,
-                                                                specialize=rb != 4))
+    pp = ParamProgram(circ, terms, len(theta), cfg=P.TileConfig(tb=tb, low_bits=4 if remap else 6, max_cost=cost,
+                                                                rb=rb, specialize=rb != 4 or remap, remap=remap,
+                                                                io_lane_bits=4 if remap else 6))
+    if remap:
+        assert any(p.flags & P.PASSF_REMAP_STORE for p in pp.adj.passes)
+        assert pp.adj.final_pos is None
     assert all(p.rb == rb for p in pp.adj.passes if p.kind == P.PASS_TILE)
     psi, _ = prog_interp.run(pp.fwd.blob, O.init_state(n), theta=theta)
     ref = O.run(circ.gate_list, n)
