@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--batch-interp", action="store_true", help="batch leg with the op-interpreting kernels")
     ap.add_argument("--batch-lds-split", type=int, default=BATCH_LDS_SPLIT,
                     help="batch leg: LDS re-distributions in 2^K parts (compiled kernels)")
+    ap.add_argument("--batch-remap", type=int, default=int(BATCH_REMAP),
+                    help="batch leg: 1 = remapped adjoint programs (compiled kernels)")
     ap.add_argument("--profile-forward", action="store_true",
                     help="only the timed forward leg (for rocprofv3 runs)")
     ap.add_argument("--profile-blocks", action="store_true", help="only the MFMA block leg")
@@ -60,8 +62,10 @@ def parse():
     return ap.parse_args()
 
 
-# LDS parts of the compiled batch kernels (log2), chosen by measurement
+# LDS parts of the compiled batch kernels (log2) and remapped adjoint
+# programs in the batch, chosen by measurement
 BATCH_LDS_SPLIT = 1
+BATCH_REMAP = False
 
 
 def log(*a):
@@ -402,7 +406,7 @@ def bench_measure(dev, n=28, reps=5, shots=1_000_000):
     return out
 
 
-def bench_batch(dev, nc=512, seed=20260004, check=6, specialize=True, lds_split=0):
+def bench_batch(dev, nc=512, seed=20260004, check=6, specialize=True, lds_split=0, remap=False):
     """512 HEA circuits (10..22 qubits, 4..32 layers) forward + adjoint as one
     HIP-graph launch; latency from device timestamps."""
     import numpy as np
@@ -416,7 +420,8 @@ def bench_batch(dev, nc=512, seed=20260004, check=6, specialize=True, lds_split=
     b = BatchedAdjoint(None, [tfim_terms(int(n)) for n in ns], [len(t) for t in thetas], device=dev,
                        dtype=torch.complex64, keys=list(zip(ns.tolist(), ls.tolist())),
                        op_lists=[hea_ops(int(n), int(l)) for n, l in zip(ns, ls)], ns=ns.tolist(),
-                       specialize=specialize, lds_split=lds_split if specialize else 0)
+                       specialize=specialize, lds_split=lds_split if specialize else 0,
+                       remap=remap and specialize)
     log(f"[bench] batch: planned {nc} circuits in {time.perf_counter() - t0:.1f} s, {len(b.launches)} launches")
     b.set_theta(thetas)
     b.capture()
@@ -581,7 +586,8 @@ def main():
         log(f"[bench] fused blocks: {blocks}")
     batch = None
     if not args.no_batch:
-        batch = bench_batch(dev, specialize=not args.batch_interp, lds_split=args.batch_lds_split)
+        batch = bench_batch(dev, specialize=not args.batch_interp, lds_split=args.batch_lds_split,
+                            remap=bool(args.batch_remap))
         log(f"[bench] batch: {batch}")
     c0 = bench_c0(dev)
     log(f"[bench] c0: {c0}")

@@ -15,6 +15,8 @@ in one HIP graph and replayed as a single launch (``capture()``/``replay()``).
 Per-circuit completion times come from the device clock (s_memrealtime,
 100 MHz) written when a circuit's last gradient reduction finishes.
 """
+from dataclasses import replace
+
 import numpy as np
 import torch
 
@@ -57,7 +59,7 @@ class BatchedAdjoint:
 
     def __init__(self, circuits, terms_list, n_params_list, device="cuda", dtype=torch.complex64,
                  tb_of_n=None, max_cost=1e9, keys=None, op_lists=None, ns=None, specialize=False, lds_split=0,
-                 rb=RB):
+                 rb=RB, remap=False):
         """``circuits``: qaravan-style circuits (rotations carry ``param_index``),
         or None with ``op_lists`` (elementary ops) and ``ns``.  Circuits with equal
         ``keys`` share compiled programs (θ lives on the device, so programs
@@ -66,15 +68,17 @@ class BatchedAdjoint:
         instead of the op-interpreting batch kernel.  ``lds_split`` (with
         ``specialize``): the compiled kernels move tiles through LDS in
         2^lds_split parts, so more workgroups fit on a CU.  ``rb`` (with
-        ``specialize``): register bits per thread, 3 or 4."""
+        ``specialize``): register bits per thread, 3 or 4.  ``remap`` (with
+        ``specialize``): the adjoint programs move qubits between address bits
+        (fewer passes; the adjoint sweep needs no permutation back)."""
         _lib.require_gpu()
         if rb != RB:
             # 3 register bits passed the small-batch tests but the 512-circuit
             # benchmark batch (up to 22 qubits, 32 layers) faulted the GPU once,
             # cause not found: switched off until it is
             raise NotImplementedError("batched programs run with 4 register bits only")
-        if lds_split and not specialize:
-            raise ValueError("lds_split needs the compiled batch kernels (specialize=True)")
+        if (lds_split or remap) and not specialize:
+            raise ValueError("lds_split and remap need the compiled batch kernels (specialize=True)")
         self.device = torch.device(device)
         self.dtype = dtype
         self.code = _lib.dtype_code(dtype)
@@ -93,11 +97,15 @@ class BatchedAdjoint:
                 continue
             n = self.ns[c]
             cfg = TileConfig(tb=tb_of_n(n), low_bits=4, max_cost=max_cost, io_lane_bits=4,
-                             specialize=bool(lds_split) or rb != RB, lds_split=lds_split, rb=rb)
+                             specialize=bool(lds_split or remap) or rb != RB, lds_split=lds_split, rb=rb)
+            # forward programs stay in natural order (the batched Hamiltonian
+            # reads ψ in it); adjoint sweeps may remap, their ψ and λ are scratch
+            adj_cfg = replace(cfg, remap=remap)
             if circuits is not None:
-                pp = ParamProgram(circuits[c], terms_list[c], n_params_list[c], cfg=cfg, dtype=dtype)
+                pp = ParamProgram(circuits[c], terms_list[c], n_params_list[c], cfg=cfg, adj_cfg=adj_cfg,
+                                  dtype=dtype)
             else:
-                pp = ParamProgram(None, terms_list[c], n_params_list[c], cfg=cfg, dtype=dtype,
+                pp = ParamProgram(None, terms_list[c], n_params_list[c], cfg=cfg, adj_cfg=adj_cfg, dtype=dtype,
                                   ops=op_lists[c], n=n)
             if key is not None:
                 cache[key] = pp

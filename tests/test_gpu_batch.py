@@ -29,14 +29,15 @@ def make_batch(nc, seed, nmin=10, nmax=14, lmax=5):
     return circs, terms, thetas
 
 
-@pytest.mark.parametrize("spz", [False, True, "split", RB3])
+@pytest.mark.parametrize("spz", [False, True, "split", "remap", RB3])
 @pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
 def test_batch_matches_single_circuit_adjoint(gpu, dtype, spz):
     from qaravan_amd.adjoint import AdjointModel
     from qaravan_amd.batch import BatchedAdjoint
     circs, terms, thetas = make_batch(9, 5)
     b = BatchedAdjoint(circs, terms, [len(t) for t in thetas], device=gpu, dtype=dtype, specialize=bool(spz),
-                       lds_split=1 if spz == "split" else 0, rb=3 if spz == "rb3" else 4)
+                       lds_split=1 if spz in ("split", "remap") else 0, remap=spz == "remap",
+                       rb=3 if spz == "rb3" else 4)
     b.set_theta(thetas)
     e, _ = b.run()
     e = e.cpu().numpy()
@@ -52,12 +53,12 @@ def test_batch_matches_single_circuit_adjoint(gpu, dtype, spz):
         assert np.linalg.norm(grads[c] - gc) / np.linalg.norm(gc) < tol
 
 
-@pytest.mark.parametrize("spz", [False, True, "split", RB3])
+@pytest.mark.parametrize("spz", [False, True, "split", "remap", RB3])
 def test_batch_vs_oracle_c128(gpu, spz):
     from qaravan_amd.batch import BatchedAdjoint
     circs, terms, thetas = make_batch(4, 11, nmin=10, nmax=11, lmax=2)
     b = BatchedAdjoint(circs, terms, [len(t) for t in thetas], device=gpu, dtype=torch.complex128,
-                       specialize=bool(spz), lds_split=1 if spz == "split" else 0,
+                       specialize=bool(spz), lds_split=1 if spz in ("split", "remap") else 0, remap=spz == "remap",
                        rb=3 if spz == "rb3" else 4)
     b.set_theta(thetas)
     e, _ = b.run()
@@ -75,12 +76,12 @@ def test_batch_vs_oracle_c128(gpu, spz):
         assert np.linalg.norm(grads[c] - gr) / np.linalg.norm(gr) < 1e-10
 
 
-@pytest.mark.parametrize("spz", [False, True, "split", RB3])
+@pytest.mark.parametrize("spz", [False, True, "split", "remap", RB3])
 def test_batch_graph_replays_match_eager(gpu, spz):
     from qaravan_amd.batch import BatchedAdjoint
     circs, terms, thetas = make_batch(7, 23)
     b = BatchedAdjoint(circs, terms, [len(t) for t in thetas], device=gpu, dtype=torch.complex64,
-                       specialize=bool(spz), lds_split=1 if spz == "split" else 0,
+                       specialize=bool(spz), lds_split=1 if spz in ("split", "remap") else 0, remap=spz == "remap",
                        rb=3 if spz == "rb3" else 4)
     b.set_theta(thetas)
     e0, g0 = b.run()
