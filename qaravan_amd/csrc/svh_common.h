@@ -42,10 +42,10 @@ template <typename R>
 __device__ __forceinline__ cpx<R> cneg(cpx<R> a) {
     return {-a.x, -a.y};
 }
-// Im(conj(l) * z)
+// acc + Im(conj(l) * z), two fused multiply-adds in the state precision
 template <typename R>
-__device__ __forceinline__ double im_conj_mul(cpx<R> l, cpx<R> z) {
-    return (double)l.x * (double)z.y - (double)l.y * (double)z.x;
+__device__ __forceinline__ R im_conj_mul_acc(R acc, cpx<R> l, cpx<R> z) {
+    return fma(-l.y, z.x, fma(l.x, z.y, acc));
 }
 
 // Insert a zero bit at position B of p.

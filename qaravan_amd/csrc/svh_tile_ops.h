@@ -45,23 +45,27 @@ __device__ __forceinline__ void ap_rot1(cpx<R> *v, int pauli, R c, R s) {
     }
 }
 
-// Σ Im(conj(λ)·(Pψ)) over this thread's pairs.
+// Σ Im(conj(λ)·(Pψ)) over this thread's pairs.  The 2^RB products of one
+// thread are summed in the state precision (fused multiply-adds; for complex64
+// the rounding error stays below 2^(RB+1) FP32 ulps of Σ|λ||ψ| over the
+// thread's amplitudes, the size of the amplitudes' own rounding); every sum
+// across threads is FP64.
 template <typename R, int RB, int A>
 __device__ __forceinline__ double gr_rot1(const cpx<R> *v, const cpx<R> *w, int pauli) {
-    double acc = 0.0;
+    R acc = R(0);
 #pragma unroll
     for (int p = 0; p < (1 << (RB - 1)); ++p) {
         const int i0 = ins0<A>(p), i1 = i0 | (1 << A);
         const cpx<R> x0 = v[i0], x1 = v[i1], l0 = w[i0], l1 = w[i1];
         if (pauli == 1) {
-            acc += im_conj_mul(l0, x1) + im_conj_mul(l1, x0);
+            acc = im_conj_mul_acc(im_conj_mul_acc(acc, l0, x1), l1, x0);
         } else if (pauli == 2) {  // (Pψ)0 = -i x1, (Pψ)1 = i x0
-            acc += im_conj_mul(l0, cpx<R>{x1.y, -x1.x}) + im_conj_mul(l1, cpx<R>{-x0.y, x0.x});
-        } else {
-            acc += im_conj_mul(l0, x0) - im_conj_mul(l1, x1);
+            acc = im_conj_mul_acc(im_conj_mul_acc(acc, l0, cpx<R>{x1.y, -x1.x}), l1, cpx<R>{-x0.y, x0.x});
+        } else {  // (Pψ)1 = -x1
+            acc = im_conj_mul_acc(im_conj_mul_acc(acc, l0, x0), l1, cpx<R>{-x1.x, -x1.y});
         }
     }
-    return acc;
+    return (double)acc;
 }
 
 template <typename R, int HI, int LO>
@@ -139,9 +143,10 @@ __device__ __forceinline__ void ap_rot2(cpx<R> *v, int pauli, R c, R s) {
     }
 }
 
+// As gr_rot1, for the 2-qubit Pauli products (same precision).
 template <typename R, int RB, int HI, int LO>
 __device__ __forceinline__ double gr_rot2(const cpx<R> *v, const cpx<R> *w, int pauli) {
-    double acc = 0.0;
+    R acc = R(0);
 #pragma unroll
     for (int q = 0; q < (1 << (RB - 2)); ++q) {
         const int i00 = Quad<R, HI, LO>::base(q);
@@ -150,9 +155,9 @@ __device__ __forceinline__ double gr_rot2(const cpx<R> *v, const cpx<R> *w, int 
         cpx<R> y[4];
         pp_apply<R>(pauli, x, y);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc += im_conj_mul(w[id[r]], y[r]);
+        for (int r = 0; r < 4; ++r) acc = im_conj_mul_acc(acc, w[id[r]], y[r]);
     }
-    return acc;
+    return (double)acc;
 }
 
 // ------------------------------------------------ compile-time dispatch
@@ -178,13 +183,32 @@ __device__ __forceinline__ double gr_rot2(const cpx<R> *v, const cpx<R> *w, int 
         default: break;                                                             \
     }
 
-// Sum over the lanes of one wave; W = active lanes (64, or the whole
-// workgroup when it is smaller than a wave).
+// One DPP step: v + (v as seen through the lane pattern CTRL), rows outside
+// ROW_MASK adding 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_add(double v) {
+    return v + __builtin_amdgcn_update_dpp(0.0, v, CTRL, ROW_MASK, 0xf, false);
+}
+
+// Sum over the lanes of one wave, returned to every lane; W = active lanes (64,
+// or the whole workgroup when it is smaller than a wave: 16 or 32).  Row
+// operations of the DPP network, no LDS: lane pairs (quad_perm 1,0,3,2), quads
+// (quad_perm 2,3,0,1), 8 (row_half_mirror), 16 (row_mirror); then rows 1 and 3
+// add the last lane of the row before (row_bcast15) and rows 2 and 3 add lane
+// 31 (row_bcast31), which leaves the total in lane W-1.  Fixed order, so the
+// result does not vary from run to run.
 template <int W>
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, W);
-    return v;
+    static_assert(W == 16 || W == 32 || W == 64, "wave_sum: 16, 32 or 64 lanes");
+    v = dpp_add<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+    v = dpp_add<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+    v = dpp_add<0x141, 0xf>(v);  // row_half_mirror
+    v = dpp_add<0x140, 0xf>(v);  // row_mirror
+    if constexpr (W >= 32) v = dpp_add<0x142, 0xa>(v);  // row_bcast15
+    if constexpr (W == 64) v = dpp_add<0x143, 0xc>(v);  // row_bcast31
+    // readlane moves 32-bit integers: take the two halves of the double as they are
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), W - 1),
+                            __builtin_amdgcn_readlane(__double2loint(v), W - 1));
 }
 
 template <typename R>
