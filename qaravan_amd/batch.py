@@ -21,9 +21,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .adjoint import ParamProgram
+from .adjoint import ParamProgram, term_table
 from .program import HDR_DT, PASS_DT, PASS_TILE, PASSF_HALF_LDS, PASSF_QUARTER_LDS, RB, TileConfig
-from .statevector_sim import _pauli_masks
 
 ITEM_DT = np.dtype([('psi', '<u8'), ('lam', '<u8'), ('pass', '<u8'), ('stages', '<u8'), ('ops', '<u8'),
                     ('trig', '<u8'), ('gpart', '<u8'), ('wg_begin', '<i8'), ('n', '<i4'), ('pad', '<i4')])
@@ -31,7 +30,6 @@ RED_DT = np.dtype([('part', '<u8'), ('slot_param', '<u8'), ('out', '<u8'), ('tst
                    ('nblocks', '<i4'), ('ng', '<i4'), ('slot_begin', '<i8')])
 HAM_DT = np.dtype([('psi', '<u8'), ('lam', '<u8'), ('terms', '<u8'), ('epart', '<u8'),
                    ('block_begin', '<i8'), ('n', '<i4'), ('nterms', '<i4')])
-TERM_DT = np.dtype([('x', '<u8'), ('z', '<u8'), ('cre', '<f8'), ('cim', '<f8')])
 assert ITEM_DT.itemsize == 72 and RED_DT.itemsize == 48 and HAM_DT.itemsize == 48
 
 
@@ -40,17 +38,6 @@ def default_tb(n, dtype, rb=RB):
     bits, 512 with 3) once the state has room for it, otherwise the whole
     state in one tile."""
     return max(4, min(12 - rb, n - rb))
-
-
-def _term_table(n, terms):
-    t = np.zeros(len(terms), dtype=TERM_DT)
-    xm, zm = _pauli_masks(n, [p for _, p in terms])
-    ph_re = [1.0, 0.0, -1.0, 0.0]
-    ph_im = [0.0, 1.0, 0.0, -1.0]
-    for i, (c, _) in enumerate(terms):
-        ny = bin(int(xm[i]) & int(zm[i])).count("1") & 3
-        t[i] = (xm[i], zm[i], float(c) * ph_re[ny], float(c) * ph_im[ny])
-    return t
 
 
 class BatchedAdjoint:
@@ -154,7 +141,7 @@ class BatchedAdjoint:
         self.gpart = torch.zeros(int(self.gp_off[-1]), dtype=torch.float64, device=dev)
         self.ep_off = np.concatenate([[0], np.cumsum([2 ** n // 1024 for n in self.ns])]).astype(np.int64)
         self.epart = torch.zeros(int(self.ep_off[-1]), dtype=torch.float64, device=dev)
-        tt = [_term_table(self.ns[c], terms_list[c]) for c in range(nc)]
+        tt = [term_table(self.ns[c], terms_list[c]) for c in range(nc)]
         self.t_off = np.concatenate([[0], np.cumsum([len(t) for t in tt])]).astype(np.int64)
         self.terms = torch.from_numpy(np.concatenate(tt).view(np.uint8).copy()).to(dev)
         self.kernels = self._specialise() if specialize else None
