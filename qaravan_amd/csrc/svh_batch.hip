@@ -71,22 +71,27 @@ __global__ void __launch_bounds__(256) k_batch_ham(const svh_batch_ham_t *__rest
     cpx<R> *lam = reinterpret_cast<cpx<R> *>(it.lam);
     const HamTerm *terms = reinterpret_cast<const HamTerm *>(it.terms);
     const uint64_t i0 = (uint64_t)(b - it.block_begin) * 1024;
+    // terms outside, the thread's 4 amplitudes inside: 4 independent gathers
+    // per term in flight (each amplitude still sums its terms in table order)
+    double ar[4] = {0.0, 0.0, 0.0, 0.0}, ai[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < it.nterms; ++t) {
+        const HamTerm T = terms[t];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint64_t j = (i0 + (uint64_t)k * 256 + threadIdx.x) ^ T.x;
+            const cpx<R> v = psi[j];
+            const double s = psign(j & T.z);
+            ar[k] += s * (T.cre * v.x - T.cim * v.y);
+            ai[k] += s * (T.cre * v.y + T.cim * v.x);
+        }
+    }
     double e = 0.0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint64_t i = i0 + (uint64_t)k * 256 + threadIdx.x;
-        double ar = 0.0, ai = 0.0;
-        for (int t = 0; t < it.nterms; ++t) {
-            const HamTerm T = terms[t];
-            const uint64_t j = i ^ T.x;
-            const cpx<R> v = psi[j];
-            const double s = psign(j & T.z);
-            ar += s * (T.cre * v.x - T.cim * v.y);
-            ai += s * (T.cre * v.y + T.cim * v.x);
-        }
-        lam[i] = cpx<R>{(R)ar, (R)ai};
+        lam[i] = cpx<R>{(R)ar[k], (R)ai[k]};
         const cpx<R> p = psi[i];
-        e += (double)p.x * ar + (double)p.y * ai;  // Re(conj(ψ_i) λ_i)
+        e += (double)p.x * ar[k] + (double)p.y * ai[k];  // Re(conj(ψ_i) λ_i)
     }
     __shared__ double red[256];
     red[threadIdx.x] = e;
