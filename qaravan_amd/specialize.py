@@ -247,21 +247,31 @@ def batch_body(blob, pi, dtype_code, dual):
     return "\n".join(_body(R, passes[pi], stages, ops, dual))
 
 
+# Item tables hold addresses as integers.  Converting them through the global
+# address space (not straight to generic pointers) lets the compiler load the
+# uniform, read-only records (pass, stages, ops, angles) with scalar loads, as
+# it does for kernel-argument pointers, instead of per-lane flat loads.
+_GPTR = """template <typename T>
+__device__ __forceinline__ T *gptr(uint64_t a) {
+    return (T *)(__attribute__((address_space(1))) T *)a;
+}"""
+
+
 def batch_kernel(body, dtype_code, dual, tb, name):
     """Kernel ``name`` over a batch item table (svh_batch_item_t): workgroup b
     finds its item (last wg_begin <= b) and runs ``body`` on that circuit."""
     R = "float" if dtype_code == 0 else "double"
-    lam = (f"    cpx<{R}> *lam = reinterpret_cast<cpx<{R}> *>(it.lam);\n"
-           f"    double *gpart = reinterpret_cast<double *>(it.gpart);\n") if dual else ""
+    lam = (f"    cpx<{R}> *lam = gptr<cpx<{R}>>(it.lam);\n"
+           f"    double *gpart = gptr<double>(it.gpart);\n") if dual else ""
     return (f'extern "C" __global__ void __launch_bounds__({1 << tb}) '
             f"{name}(const svh_batch_item_t *__restrict__ items, int nitems) {{\n"
             f"    const svh_batch_item_t &it = items[sp_find_item(items, nitems)];\n"
-            f"    cpx<{R}> *psi = reinterpret_cast<cpx<{R}> *>(it.psi);\n" + lam +
+            f"    cpx<{R}> *psi = gptr<cpx<{R}>>(it.psi);\n" + lam +
             f"    const int n = it.n;\n"
-            f"    const svh_pass_t *pass = reinterpret_cast<const svh_pass_t *>(it.pass);\n"
-            f"    const svh_stage_t *st = reinterpret_cast<const svh_stage_t *>(it.stages) + pass->stage_begin;\n"
-            f"    const svh_op_t *ops = reinterpret_cast<const svh_op_t *>(it.ops) + st[0].op_begin;\n"
-            f"    const cpx<{R}> *trig = reinterpret_cast<const cpx<{R}> *>(it.trig);\n"
+            f"    const svh_pass_t *pass = gptr<const svh_pass_t>(it.pass);\n"
+            f"    const svh_stage_t *st = gptr<const svh_stage_t>(it.stages) + pass->stage_begin;\n"
+            f"    const svh_op_t *ops = gptr<const svh_op_t>(it.ops) + st[0].op_begin;\n"
+            f"    const cpx<{R}> *trig = gptr<const cpx<{R}>>(it.trig);\n"
             f"    const uint64_t wg = (uint64_t)((int64_t)blockIdx.x - it.wg_begin);\n"
             + body + "\n}\n")
 
@@ -386,7 +396,7 @@ def compile_batch(kernels, dtype_code, per_file=8, workers=8):
     for g in groups:
         if not g:
             continue
-        text = ['#include "svh_tile_ops.h"', "using namespace svh;", ""]
+        text = ['#include "svh_tile_ops.h"', "using namespace svh;", _GPTR, ""]
         for nm in g:
             body, dual, tb = kernels[nm]
             text.append(batch_kernel(body, dtype_code, dual, tb, nm))
