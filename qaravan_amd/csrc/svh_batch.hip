@@ -67,14 +67,34 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_batch_ham(const svh_batch_ham_t *__restrict__ items, int nitems) {
     const int64_t b = blockIdx.x;
     const svh_batch_ham_t &it = items[find_ham(items, nitems, b)];
-    const cpx<R> *psi = reinterpret_cast<const cpx<R> *>(it.psi);
-    cpx<R> *lam = reinterpret_cast<cpx<R> *>(it.lam);
-    const HamTerm *terms = reinterpret_cast<const HamTerm *>(it.terms);
+    // Table addresses converted through the global address space: the uniform
+    // term records then load through the scalar unit.
+    const cpx<R> *psi = (const cpx<R> *)(__attribute__((address_space(1))) const cpx<R> *)it.psi;
+    cpx<R> *lam = (cpx<R> *)(__attribute__((address_space(1))) cpx<R> *)it.lam;
+    const HamTerm *terms = (const HamTerm *)(__attribute__((address_space(1))) const HamTerm *)it.terms;
     const uint64_t i0 = (uint64_t)(b - it.block_begin) * 1024;
     // terms outside, the thread's 4 amplitudes inside: 4 independent gathers
     // per term in flight (each amplitude still sums its terms in table order)
     double ar[4] = {0.0, 0.0, 0.0, 0.0}, ai[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int t = 0; t < it.nterms; ++t) {
+    int t = 0;
+    for (; t + 2 <= it.nterms; t += 2) {  // 2 terms x 4 amplitudes: 8 gathers in flight
+        const HamTerm T[2] = {terms[t], terms[t + 1]};
+        cpx<R> v[2][4];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[u][k] = psi[(i0 + (uint64_t)k * 256 + threadIdx.x) ^ T[u].x];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint64_t j = (i0 + (uint64_t)k * 256 + threadIdx.x) ^ T[u].x;
+                const double s = psign(j & T[u].z);
+                ar[k] += s * (T[u].cre * v[u][k].x - T[u].cim * v[u][k].y);
+                ai[k] += s * (T[u].cre * v[u][k].y + T[u].cim * v[u][k].x);
+            }
+    }
+    for (; t < it.nterms; ++t) {
         const HamTerm T = terms[t];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
