@@ -553,22 +553,32 @@ __global__ void __launch_bounds__(256) k_permute_bits(const cpx<R> *__restrict__
         sb |= (t & 1ull) << A.nsrc[k];
         db |= (t & 1ull) << A.ndst[k];
     }
+    // Element e = r·256 + tid (TB >= 8).  Every offset below is linear over
+    // GF(2) in e and the bits of tid and r·256 are disjoint, so the thread's
+    // part is computed once and the workgroup-uniform part once per r.
     const int per = (1 << TB) / 256;
+    uint64_t so_t = 0, do_t = 0;
+    int x_t = 0;
+    for (int b = 0; b < 8; ++b) {
+        const int bit = (tid >> b) & 1;
+        so_t |= (uint64_t)bit << A.tsrc[b];
+        x_t |= bit << A.uidx[b];
+        do_t |= (uint64_t)bit << A.udst[b];
+    }
     for (int r = 0; r < per; ++r) {
+        uint64_t so = so_t;
+        for (int b = 8; b < TB; ++b) so |= (uint64_t)((r >> (b - 8)) & 1) << A.tsrc[b];
         const int e = r * 256 + tid;
-        uint64_t so = 0;
-        for (int x = 0; x < TB; ++x) so |= (uint64_t)((e >> x) & 1) << A.tsrc[x];
         tile[e ^ ((e >> 5) & 31)] = src[sb | so];
     }
     __syncthreads();
     for (int r = 0; r < per; ++r) {
-        const int f = r * 256 + tid;
-        int x = 0;
-        uint64_t dof = 0;
-        for (int i = 0; i < TB; ++i) {
-            const int bit = (f >> i) & 1;
-            x |= bit << A.uidx[i];
-            dof |= (uint64_t)bit << A.udst[i];
+        int x = x_t;
+        uint64_t dof = do_t;
+        for (int b = 8; b < TB; ++b) {
+            const int bit = (r >> (b - 8)) & 1;
+            x |= bit << A.uidx[b];
+            dof |= (uint64_t)bit << A.udst[b];
         }
         dst[db | dof] = tile[x ^ ((x >> 5) & 31)];
     }
