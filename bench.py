@@ -154,7 +154,7 @@ def check_forward(res, dev):
 
     from qaravan_amd import StatevectorSim
     from qaravan_amd.workloads import brickwork_circuit
-    sim = StatevectorSim(brickwork_circuit(res["n"], res["layers"]), device=dev, dtype=torch.complex128,
+    sim = StatevectorSim(brickwork_circuit(res["n"], res["layers"]), backend="hip", device=dev, dtype=torch.complex128,
                          tile=None)
     sim.run(progress_bar=False)
     ref = sim.get_statevector()

@@ -97,7 +97,13 @@ enum {
     /* TILE, per-pass compiled forward kernels: the tile is stored through the
      * stage record that follows the pass's n_stages records; its global bits
      * permute the tile's bits (a qubit remap within the tile). */
-    SVH_PASSF_REMAP_STORE = 8
+    SVH_PASSF_REMAP_STORE = 8,
+    /* TILE: run by the generic kernels (one kernel for every pass structure:
+     * stage records walked at run time, each stage dispatched to the compiled
+     * pattern library or interpreted); 3 or 4 register bits, tb >= 6.  In an
+     * adjoint sweep every wave writes its own row of gradient partials, so a
+     * pass has 2^(n-L) * 2^(tb-6) rows of n_grad doubles. */
+    SVH_PASSF_GENERIC = 16
 };
 
 /* A stage's layout: thread t holds registers j = 0..2^rb-1 at tile-local index
@@ -112,7 +118,10 @@ typedef struct {
     int32_t rsorted[8];  /* rbits[0..rb) in ascending order                         */
     int32_t tgbit[12];   /* global bit of thread-index bit i (i < tb)              */
     int32_t op_begin, op_end;
-    int32_t pad[2];
+    int32_t pattern;     /* generic kernels: id of this stage's op sequence in the
+                            compiled pattern library of the pass's (rb, direction),
+                            or -1: the stage is interpreted op by op            */
+    int32_t pad;
 } svh_stage_t; /* 512 bytes */
 
 typedef struct {

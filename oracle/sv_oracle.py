@@ -36,6 +36,11 @@ What each function restates (paths relative to the qaravan source tree):
                      partial_overlap(pre, post) over the gate's qubits, new
                      gate V·W† from the SVD env = W·S·V†, cost 1 − |Tr(U·env)|,
                      left sweep then right sweep.
+* ``u_matrix`` / ``kak_unitary`` / ``two_local_gates`` / ``infidelity``
+                     src/qaravan/core/param_gates.py:118-129,247-260,
+                     core/circuits.py:206-217 and the loss of
+                     examples/state_prep_with_ad.ipynb (cell 1); ``fd4_gradient``
+                     the fourth-order central difference used to check gradients.
 * ``measure_and_collapse_sv`` statevector_sim.py:195-215, including its order
                      convention: outcome character j is projected onto site
                      meas_sites[j] although the characters were drawn for the
@@ -58,7 +63,13 @@ Pinning (each is exercised in tests/test_oracle.py):
   measure_sv vs a direct per-outcome probability computation;
 * environments: Tr(M·env) = ⟨post|M|pre⟩ for random M (the identity that
   makes V·W† the maximiser), |Tr(U·env)| = Σ singular values for the update,
-  and a cost sequence that never increases (block-coordinate ascent).
+  and a cost sequence that never increases (block-coordinate ascent);
+* kak gates: U(π, 0, π) = X, U(π/2, 0, π) = H, U(0, 0, λ) = diag(1, e^{iλ});
+  expm(iπ/4·(XX+YY+ZZ)) = e^{iπ/4}·SWAP; unitarity of kak_unitary at random
+  angles; agreement with the package's closed-form centre (an independent
+  product formula); fd4_gradient exact on cubics and against analytic
+  derivatives of trigonometric test functions; infidelity in [0, 1] and 0 at
+  target = ψ(params).
 """
 import numpy as np
 
@@ -300,6 +311,64 @@ def fd_gradient(spec, theta, n, terms, eps=1e-3, idx=None):
         tp[i] += eps
         tm[i] -= eps
         g[j] = (param_energy(spec, tp, n, terms) - param_energy(spec, tm, n, terms)) / (2 * eps)
+    return g
+
+
+# ------------------------------------------------- two-local ansatz (kak gates)
+def u_matrix(t, p, l):
+    """qaravan's U(θ, φ, λ) (src/qaravan/core/param_gates.py:118-129)."""
+    c, s = np.cos(t / 2), np.sin(t / 2)
+    return np.array([[c, -np.exp(1j * l) * s], [np.exp(1j * p) * s, np.exp(1j * (p + l)) * c]],
+                    dtype=np.complex128)
+
+
+def kak_unitary(params):
+    """(U⊗U)·expm(i(p6·XX + p7·YY + p8·ZZ))·(U⊗U) (param_gates.py:247-260),
+    the centre by scipy.linalg.expm as in the reference."""
+    from scipy.linalg import expm
+    p = np.asarray(params, dtype=np.float64)
+    words = [np.kron(PAULI[c], PAULI[c]) for c in 'xyz']
+    left = np.kron(u_matrix(*p[0:3]), u_matrix(*p[3:6]))
+    right = np.kron(u_matrix(*p[9:12]), u_matrix(*p[12:15]))
+    return left @ expm(1j * sum(a * w for a, w in zip(p[6:9], words))) @ right
+
+
+class _G:
+    def __init__(self, name, indices, matrix):
+        self.name, self.indices, self.matrix = name, list(indices), matrix
+
+
+def two_local_gates(skeleton, params):
+    """The gate list of two_local_circ(skeleton, params) (circuits.py:206-217)."""
+    return [_G("rand_U", e, kak_unitary(params[15 * i:15 * (i + 1)])) for i, e in enumerate(skeleton)]
+
+
+def fidelity(params, skeleton, target, n, init=None):
+    """|⟨ψ(params)|target⟩|² for the two-local ansatz."""
+    psi = run(two_local_gates(skeleton, params), n, init)
+    return abs(np.vdot(psi, target)) ** 2
+
+
+def infidelity(params, skeleton, target, n, init=None):
+    """1 − |⟨ψ(params)|target⟩|², the loss of examples/state_prep_with_ad.ipynb
+    (cell 1) and, per input state, applications/compilation.py:73-90.  Its
+    gradient is −∇fidelity; differencing the fidelity itself keeps the
+    rounding of the constant 1 out of finite differences."""
+    return 1.0 - fidelity(params, skeleton, target, n, init)
+
+
+def fd4_gradient(f, x, idx, h=1e-3):
+    """Fourth-order central differences [f(x−2h) − 8f(x−h) + 8f(x+h) − f(x+2h)]/(12h):
+    truncation error h⁴·f⁽⁵⁾/30, about 1e-13 relative for the trigonometric
+    losses here at h = 1e-3."""
+    g = np.zeros(len(idx))
+    for j, i in enumerate(idx):
+        vals = []
+        for d in (-2, -1, 1, 2):
+            xx = np.array(x, dtype=np.float64)
+            xx[i] += d * h
+            vals.append(f(xx))
+        g[j] = (vals[0] - 8 * vals[1] + 8 * vals[2] - vals[3]) / (12 * h)
     return g
 
 

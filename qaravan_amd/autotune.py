@@ -54,6 +54,22 @@ def best_tile_config(n, dtype):
     return replace(cfg, tb=min(cfg.tb, n - cfg.rb))
 
 
+# below this size a run is launch- or latency-bound and the per-pass compiled
+# kernels (hipcc at first use of each new circuit structure) do not pay
+SIM_TUNED_MIN_QUBITS = 20
+
+
+def sim_tile_config(n, dtype):
+    """The tile configuration ``StatevectorSim`` uses when none is given: the
+    tuned one (nearest tuned size of this dtype) from SIM_TUNED_MIN_QUBITS
+    qubits up when the cache holds an entry for the dtype, else the
+    op-interpreting defaults, which need no compiler at run time."""
+    name = str(dtype).replace('torch.', '')
+    if n >= SIM_TUNED_MIN_QUBITS and any(k.startswith(name + ':') for k in _load()):
+        return best_tile_config(n, dtype)
+    return DEFAULTS.get(name, TileConfig())
+
+
 def tune(circ, dtype, device, space=None, repeats=3, write=True, log=print, out_path=None):
     """Time every configuration in ``space`` on ``circ`` and cache the fastest."""
     import torch

@@ -1,7 +1,7 @@
 """Circuit container and greedy ASAP layering.
 
-Mirrors src/qaravan/core/circuits.py: ``Circuit`` (9-24), ``construct_layers``
-(42-63), ``build`` (86-95), ``dag`` (112-114), slicing/len/add/mul (126-153),
+Mirrors src/qaravan/core/circuits.py: ``Circuit`` (9-24), ``decompose`` (31-40),
+``construct_layers`` (42-63), ``build`` (86-95), ``dag`` (112-114), slicing/len/add/mul (126-153),
 ``copy`` (163-170), ``compose_circuits`` (172-175) and the layer builders
 ``two_local_circ`` / ``rx_layer`` / ``ry_layer`` / ``rz_layer`` (206-232).
 Noise insertion (``add_noise``) is not part of the statevector path, which is
@@ -10,7 +10,7 @@ always noiseless.
 import numpy as np
 
 from .gates import Gate, random_unitary
-from .param_gates import RX, RY, RZ
+from .param_gates import RX, RY, RZ, kak_unitary
 
 
 class Circuit:
@@ -32,6 +32,17 @@ class Circuit:
         self.gate_list[gate_idx].matrix = new_array
         if self.layers is not None:
             self.construct_layers()
+
+    def decompose(self, basis='ZSX'):
+        """Replace every ``U`` gate that has angles by its five-gate ZSX
+        decomposition (circuits.py:31-40); other gates are kept."""
+        out = []
+        for g in self.gate_list:
+            if g.name == "U" and getattr(g, "angles", None) is not None:
+                out.extend(g.decompose(basis))
+            else:
+                out.append(g)
+        self.gate_list = out
 
     def construct_layers(self):
         """Greedy ASAP layering: a gate goes into the layer after the newest layer
@@ -56,6 +67,8 @@ class Circuit:
         if nm is not None:
             raise NotImplementedError("the statevector path is noiseless (nm must be None)")
         if not self.built:
+            if self.requires_decomp:
+                self.decompose()
             self.construct_layers()
             self.built = True
         return self
@@ -110,9 +123,24 @@ def compose_circuits(circ_list):
     return Circuit(gates, n=int(np.max([c.num_sites for c in circ_list])))
 
 
-def two_local_circ(skeleton, rng=None):
-    """One random 4×4 unitary per skeleton edge (circuits.py:206-217)."""
-    return Circuit([Gate("rand_U", list(e), random_unitary(4, rng)) for e in skeleton])
+def two_local_circ(skeleton, params=None, mag=None, rng=None):
+    """One two-qubit gate per skeleton edge (circuits.py:206-217): with
+    ``params``, edge i gets ``kak_unitary(params[15i:15i+15])`` (a torch
+    tensor of params gives differentiable gate matrices); with only ``mag``,
+    ``kak_unitary`` of 15 uniform angles in [0, mag); with neither, a random
+    unitary (``random_unitary(4)``).  ``rng`` (a numpy Generator) replaces
+    numpy's global RNG for the random draws."""
+    gates = []
+    for i, e in enumerate(skeleton):
+        if params is not None:
+            mat = kak_unitary(params[15 * i:15 * (i + 1)])
+        elif mag is not None:
+            r = np.random if rng is None else rng
+            mat = kak_unitary(r.random(15) * mag)
+        else:
+            mat = random_unitary(4, rng)
+        gates.append(Gate("rand_U", list(e), mat))
+    return Circuit(gates)
 
 
 def rx_layer(n, params=None):

@@ -77,8 +77,8 @@ def _polar_update(env):
 def sv_environment(circ, left_sv, gate_idx, device="cuda", dtype=torch.complex128):
     """(environment, gate matrix) of gate ``gate_idx`` (environments.py:7-19)."""
     g = circ[gate_idx]
-    sim1 = StatevectorSim(circ[:gate_idx], init_state=None, device=device, dtype=dtype)
-    sim2 = StatevectorSim(circ[gate_idx + 1:].dag(), init_state=left_sv, device=device, dtype=dtype)
+    sim1 = StatevectorSim(circ[:gate_idx], init_state=None, backend="hip", device=device, dtype=dtype)
+    sim2 = StatevectorSim(circ[gate_idx + 1:].dag(), init_state=left_sv, backend="hip", device=device, dtype=dtype)
     sim1.run(progress_bar=False)
     sim2.run(progress_bar=False)
     env = partial_overlap(sim1.get_statevector(), sim2.get_statevector(), skip=list(g.indices))
@@ -136,7 +136,7 @@ def environment_state_prep(target_sv, init_sv=None, circ=None, skeleton=None, co
     else:
         if circ is None:
             circ = two_local_circ(skeleton)
-        sim = StatevectorSim(circ, init_state=init_sv, device=dev, dtype=dtype)
+        sim = StatevectorSim(circ, init_state=init_sv, backend="hip", device=dev, dtype=dtype)
         sim.run(progress_bar=False)
         ansatz = sim.get_statevector()
         from .statevector_sim import inner

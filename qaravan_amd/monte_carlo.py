@@ -114,7 +114,7 @@ class MonteCarloSim:
 
     def expectation_sample(self, obs):
         """Re⟨ψ|O|ψ⟩ for one sampled trajectory (:30-35)."""
-        sim = StatevectorSim(self.circuit_sample(), init_state=self.init_state, device=self.device,
+        sim = StatevectorSim(self.circuit_sample(), init_state=self.init_state, backend="hip", device=self.device,
                              dtype=self.dtype, tile=self.tile)
         sim.run(progress_bar=False)
         return float(expectation_value(sim.get_statevector(), self.num_sites, obs, self._scratch).item())

@@ -147,3 +147,97 @@ class CPhase(ParamGate):
         if isinstance(self.angles, str):
             return Gate.dag(self)
         return CPhase(self.indices, -self.angles[0])
+
+
+class U(ParamGate):
+    """General single-qubit gate ``U(indices, θ, φ, λ)`` (param_gates.py:118-159):
+
+        [[cos(θ/2),          −e^{iλ} sin(θ/2)],
+         [e^{iφ} sin(θ/2),   e^{i(φ+λ)} cos(θ/2)]]
+
+    Angles may be floats or 0-d torch tensors (then the matrix is a
+    differentiable complex128 tensor).  Built from a matrix, the angles are
+    recovered by ``solve_angles`` after the global phase of the top-left
+    entry is divided out of the matrix, as the reference does."""
+
+    def __init__(self, indices, *args):
+        super().__init__("U", indices, *args)
+
+    def construct_matrix(self):
+        t, p, l = self.angles
+        if self.backend == "torch":
+            def as_t(a):
+                return a.to(torch.float64) if torch.is_tensor(a) else torch.tensor(float(a), dtype=torch.float64)
+            t, p, l = as_t(t), as_t(p), as_t(l)
+            c, s = torch.cos(t / 2), torch.sin(t / 2)
+            e_l, e_p = torch.exp(1j * l), torch.exp(1j * p)
+            return torch.stack([torch.stack([c.to(torch.complex128), -e_l * s]),
+                                torch.stack([e_p * s, e_p * e_l * c])]).to(torch.complex128)
+        t, p, l = (float(np.real(a)) for a in (t, p, l))
+        c, s = np.cos(t / 2), np.sin(t / 2)
+        return np.array([[c, -np.exp(1j * l) * s], [np.exp(1j * p) * s, np.exp(1j * (p + l)) * c]],
+                        dtype=np.complex128)
+
+    def solve_angles(self):
+        m = np.asarray(self.matrix, dtype=np.complex128)
+        m = m / np.exp(1j * np.angle(m[0, 0]))
+        self.matrix = m
+        t = 2 * np.arccos(np.clip(m[0, 0].real, -1.0, 1.0))
+        if abs(np.sin(t / 2)) > 1e-12:
+            l = np.angle(-m[0, 1] / np.sin(t / 2))
+            p = np.angle(m[1, 1] / m[0, 0]) - l
+        else:
+            l = p = np.angle(m[1, 1]) / 2
+        return t, p, l
+
+    def decompose(self, basis='ZSX'):
+        """Five gates in application order: RZ(λ), SX, RZ(θ+π), SX, RZ(φ+π)
+        (param_gates.py:145-159; qaravan's RZ(a) is exp(−i·a·Z))."""
+        if basis != 'ZSX':
+            raise NotImplementedError(f"{basis} basis decomposition has not been implemented")
+        from .gates import SX
+        t, p, l = self.solve_angles()
+        i = self.indices
+        return [RZ(i, l), SX(i), RZ(i, t + np.pi), SX(i), RZ(i, p + np.pi)]
+
+    def dag(self):
+        return Gate.dag(self)
+
+
+def _xyz_center(a, b, c):
+    """exp(i(a·XX + b·YY + c·ZZ)).  XX, YY and ZZ commute and square to I, so
+    the exponential is the product of cos·I + i·sin·P over the three words;
+    written out, it is diagonal in the Bell-type pairs {|00⟩,|11⟩} and
+    {|01⟩,|10⟩}:
+        ⟨00|·|00⟩ = ⟨11|·|11⟩ = e^{ic} cos(a−b),  ⟨00|·|11⟩ = ⟨11|·|00⟩ = i e^{ic} sin(a−b),
+        ⟨01|·|01⟩ = ⟨10|·|10⟩ = e^{−ic} cos(a+b), ⟨01|·|10⟩ = ⟨10|·|01⟩ = i e^{−ic} sin(a+b)."""
+    if torch.is_tensor(a) or torch.is_tensor(b) or torch.is_tensor(c):
+        def f(x):
+            return x.to(torch.float64) if torch.is_tensor(x) else torch.tensor(float(x), dtype=torch.float64)
+        a, b, c = f(a), f(b), f(c)
+        ep, em = torch.exp(1j * c), torch.exp(-1j * c)
+        d0, o0 = ep * torch.cos(a - b), 1j * ep * torch.sin(a - b)
+        d1, o1 = em * torch.cos(a + b), 1j * em * torch.sin(a + b)
+        z = torch.zeros((), dtype=torch.complex128)
+        return torch.stack([torch.stack([d0, z, z, o0]), torch.stack([z, d1, o1, z]),
+                            torch.stack([z, o1, d1, z]), torch.stack([o0, z, z, d0])]).to(torch.complex128)
+    a, b, c = (float(np.real(x)) for x in (a, b, c))
+    ep, em = np.exp(1j * c), np.exp(-1j * c)
+    d0, o0 = ep * np.cos(a - b), 1j * ep * np.sin(a - b)
+    d1, o1 = em * np.cos(a + b), 1j * em * np.sin(a + b)
+    return np.array([[d0, 0, 0, o0], [0, d1, o1, 0], [0, o1, d1, 0], [o0, 0, 0, d0]], dtype=np.complex128)
+
+
+def kak_unitary(params):
+    """Two-qubit gate (U⊗U)·exp(i(p6·XX + p7·YY + p8·ZZ))·(U⊗U) from 15
+    angles: params[0:3] and [3:6] the left single-qubit gates, [9:12] and
+    [12:15] the right ones (param_gates.py:247-260).  A torch tensor of angles
+    gives a differentiable complex128 matrix."""
+    if torch.is_tensor(params):
+        kron = torch.kron
+    else:
+        params = np.asarray(params, dtype=np.float64)
+        kron = np.kron
+    left = kron(U(0, *params[0:3]).matrix, U(0, *params[3:6]).matrix)
+    right = kron(U(0, *params[9:12]).matrix, U(0, *params[12:15]).matrix)
+    return left @ _xyz_center(params[6], params[7], params[8]) @ right

@@ -38,6 +38,7 @@ PASSF_PREFETCH = 1
 PASSF_HALF_LDS = 2
 PASSF_QUARTER_LDS = 4
 PASSF_REMAP_STORE = 8
+PASSF_GENERIC = 16
 OP_MAT1, OP_MAT2, OP_CX, OP_CZ, OP_ROT1, OP_ROT2 = 1, 2, 3, 4, 5, 6
 OPF_NEG, OPF_GRAD = 1, 2
 RB = 4
@@ -54,7 +55,7 @@ PASS_DT = np.dtype([('kind', '<i4'), ('L', '<i4'), ('tb', '<i4'), ('rb', '<i4'),
                     ('bits', '<i4', (MAX_TILE_BITS,))])
 STAGE_DT = np.dtype([('goff', '<i8', (32,)), ('lsw', '<i4', (32,)), ('rbits', '<i4', (8,)),
                      ('rsorted', '<i4', (8,)), ('tgbit', '<i4', (12,)), ('op_begin', '<i4'),
-                     ('op_end', '<i4'), ('pad', '<i4', (2,))])
+                     ('op_end', '<i4'), ('pattern', '<i4'), ('pad', '<i4')])
 OP_DT = np.dtype([('kind', '<i4'), ('a', '<i4'), ('b', '<i4'), ('param', '<i4'), ('pauli', '<i4'),
                   ('flags', '<i4'), ('grad_slot', '<i4'), ('pad', '<i4'), ('m', '<f8', (32,)),
                   ('mf', '<f4', (32,))])
@@ -123,10 +124,19 @@ class EOp:
 
 # ----------------------------------------------------------------- lowering
 def _as_np(m):
+    """A gate matrix as complex128 numpy.  A torch matrix that requires grad is
+    refused while grad mode is on: a compiled program holds plain numbers, so
+    the gradient would be dropped without a word.  Differentiable runs go
+    through qaravan_amd.state_autograd (StatevectorSim(backend="torch")),
+    which lowers detached values itself and supplies the backward pass."""
     try:
         import torch
         if torch.is_tensor(m):
-            m = m.detach().cpu().numpy()
+            if m.requires_grad and torch.is_grad_enabled():
+                raise NotImplementedError(
+                    "gate matrix requires grad: compiling it into a program would drop the gradient; run the "
+                    "circuit with StatevectorSim(backend='torch') (or under torch.no_grad() to use the values)")
+            m = m.detach().resolve_conj().cpu().numpy()
     except ImportError:  # pragma: no cover
         pass
     return np.asarray(m, dtype=np.complex128)
@@ -341,6 +351,10 @@ class TileConfig:
               when they hold at least this many of the qubits wanted next
               (no extra layout change); otherwise it adds one layout change
               to bring the wanted qubits to the lanes (-1: always).
+    generic:  tile passes run in the generic kernels (one compiled kernel
+              for every pass structure; stages dispatched to the pattern
+              library or interpreted): no compiler at run time.  3 or 4
+              register bits, tb >= 6, no LDS split; remap allowed.
     """
     tb: int = 8
     low_bits: int = 6
@@ -353,6 +367,7 @@ class TileConfig:
     lds_split: int = 0
     remap: bool = False
     remap_free: int = -1
+    generic: bool = False
 
 
 def _is_barrier(op):
@@ -719,8 +734,8 @@ class Program:
         """Loaded straight-line kernels of this program (compiled on first use)."""
         key = (dtype_code, dual)
         if key not in self._sp:
-            from .specialize import Specialized
-            self._sp[key] = Specialized(self, dtype_code, dual)
+            from .specialize import specialized_for
+            self._sp[key] = specialized_for(self, dtype_code, dual)
         return self._sp[key]
 
     @property
@@ -767,6 +782,7 @@ def encode(n, passes, grad=False, n_params=0, final_pos=None):
             slot = 0
             for st in p.stages:
                 s = np.zeros((), dtype=STAGE_DT)
+                s['pattern'] = -1
                 s['rbits'][:p.rb] = st.rbits
                 tab = stage_tables(p.bits, p.tb, st.rbits)
                 s['rsorted'][:p.rb] = tab['rsorted']
@@ -813,6 +829,7 @@ def encode(n, passes, grad=False, n_params=0, final_pos=None):
             if p.store_bits is not None:
                 # the store record: global bits of the last layout after the remap
                 s = np.zeros((), dtype=STAGE_DT)
+                s['pattern'] = -1
                 last = p.stages[-1].rbits
                 tab = stage_tables(p.store_bits, p.tb, last)
                 s['rbits'][:p.rb] = last
@@ -831,6 +848,12 @@ def encode(n, passes, grad=False, n_params=0, final_pos=None):
             pool.extend(np.ascontiguousarray(p.mat, dtype=np.complex128).reshape(-1).view(np.float64))
     stages = np.array(stage_rows, dtype=STAGE_DT) if stage_rows else np.zeros(0, STAGE_DT)
     opsa = np.array(op_rows, dtype=OP_DT) if op_rows else np.zeros(0, OP_DT)
+    for pi, p in enumerate(passes):
+        if p.kind == PASS_TILE and (p.flags & PASSF_GENERIC):
+            # stages the generic kernels run from the compiled pattern library
+            from .patterns import pattern_ids
+            s0, ns = int(pass_rows[pi]['stage_begin']), int(pass_rows[pi]['n_stages'])
+            stages['pattern'][s0:s0 + ns] = pattern_ids(stages[s0:s0 + ns], opsa, p.rb, grad)
     used = [int(r[3]) + 1 for r in op_rows if r[0] in (OP_ROT1, OP_ROT2)]
     grads = np.array(grad_params, dtype=np.int32)
     poola = np.array(pool, dtype=np.float64)
@@ -888,23 +911,28 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
         return encode(n, passes, grad, n_params)
     if cfg.rb not in (3, 4, 5):
         raise ValueError("rb (register bits per thread) must be 3, 4 or 5")
+    if cfg.generic and (cfg.specialize or cfg.prefetch or cfg.lds_split or cfg.rb not in (3, 4)):
+        raise ValueError("generic passes take 3 or 4 register bits, no LDS split, and exclude the compiled "
+                         "per-pass and prefetching kernels")
     if cfg.rb == 5 and (grad or cfg.prefetch or not cfg.specialize):
         raise ValueError("rb=5 runs only in the per-pass compiled forward kernels (specialize=True, "
                          "no prefetch, no adjoint)")
-    if cfg.rb == 3 and (cfg.prefetch or not cfg.specialize):
-        raise ValueError("rb=3 runs only in the per-pass compiled kernels (specialize=True, no prefetch)")
+    if cfg.rb == 3 and (cfg.prefetch or not (cfg.specialize or cfg.generic)):
+        raise ValueError("rb=3 runs only in the per-pass compiled or the generic kernels (no prefetch)")
     if cfg.lds_split and (cfg.prefetch or not cfg.specialize):
         raise ValueError("lds_split runs only in the per-pass compiled kernels (specialize=True, no prefetch)")
     if cfg.lds_split not in (0, 1, 2) or cfg.lds_split >= cfg.rb:
         raise ValueError("lds_split must be 0, 1 or 2 and below the register bits")
     tb = min(cfg.tb, n - cfg.rb)
+    if cfg.generic and tb < 6:
+        raise ValueError("generic passes need tb >= 6 (whole waves): n >= rb + 6")
     split = cfg.lds_split
-    remap = (cfg.remap and cfg.specialize and not cfg.prefetch
+    remap = (cfg.remap and (cfg.specialize or cfg.generic) and not cfg.prefetch
              and not any(_is_barrier(op) for op in ops))
     cfg = TileConfig(tb=tb, low_bits=max(1, cfg.low_bits), max_cost=cfg.max_cost,
                      prefetch=cfg.prefetch and not grad, specialize=cfg.specialize, fuse2q=cfg.fuse2q,
                      io_lane_bits=cfg.io_lane_bits, rb=cfg.rb, lds_split=split, remap=remap,
-                     remap_free=cfg.remap_free)
+                     remap_free=cfg.remap_free, generic=cfg.generic)
     final_pos = None
     if remap:
         passes, final_pos = partition_remap(ops, n, cfg)
@@ -916,6 +944,8 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
         passes = partition(ops, n, cfg)
     for p in passes:
         if p.kind == PASS_TILE:
+            if cfg.generic:
+                p.flags |= PASSF_GENERIC
             if not remap:
                 _stage_ops(p, cfg.io_lane_bits, keep_shared=split)
             shared = min((len(set(a.rbits) & set(b.rbits)) for a, b in zip(p.stages, p.stages[1:])), default=RB)

@@ -21,12 +21,19 @@ from . import program as P
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-CACHE_DIR = os.environ.get("QARAVAN_SP_CACHE", os.path.join(HERE, "_sp_cache"))
+# compiled objects are written to CACHE_DIR and looked up there and in the
+# package's own cache (objects built ahead of time, e.g. by build())
+PACKAGE_CACHE = os.path.join(HERE, "_sp_cache")
+CACHE_DIR = os.environ.get("QARAVAN_SP_CACHE", PACKAGE_CACHE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("QARAVAN_HIP_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "--genco",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I" + CSRC]
 MAX_LDS = 160 * 1024          # LDS per workgroup on gfx950
+
+# cold-start accounting: code objects compiled by this process and the wall
+# time of the compile calls that had to run hipcc
+STATS = {"objects_compiled": 0, "compile_wall_s": 0.0}
 
 
 def _decode(blob):
@@ -318,8 +325,10 @@ def _compile_one(src, tag):
     flags = " ".join(f for f in FLAGS if not f.startswith("-I"))
     key = hashlib.sha1((src + _headers_digest() + flags).encode()).hexdigest()[:20]
     co = os.path.join(CACHE_DIR, f"sp_{key}.hsaco")
-    if os.path.exists(co):
-        return co
+    for d in (CACHE_DIR, PACKAGE_CACHE):
+        hit = os.path.join(d, f"sp_{key}.hsaco")
+        if os.path.exists(hit):
+            return hit
     srcf = os.path.join(CACHE_DIR, f"sp_{key}.hip")
     with open(srcf, "w") as f:
         f.write(src)
@@ -330,6 +339,8 @@ def _compile_one(src, tag):
         err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
         raise RuntimeError("specialised program compile failed:\n" + err[-4000:])
     os.replace(tmp, co)
+    os.remove(srcf)
+    STATS["objects_compiled"] += 1
     # a cache miss costs seconds to tens of seconds: say so, one line each
     print(f"[specialize] compiled {os.path.basename(co)} in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     return co
@@ -338,7 +349,6 @@ def _compile_one(src, tag):
 def compile_program(blob, dtype_code, chunks=None, dual=False):
     """Code-object paths specialising ``blob`` (compiled on first use, in
     ``chunks`` parallel hipcc runs, cached by content hash)."""
-    from concurrent.futures import ThreadPoolExecutor
     _, passes, _, _ = _decode(blob)
     todo = [pi for pi, p in enumerate(passes) if eligible(p, dtype_code, dual)]
     if not todo:
@@ -348,8 +358,37 @@ def compile_program(blob, dtype_code, chunks=None, dual=False):
     groups = [set(todo[i::k]) for i in range(k)]
     gen = adjoint_source if dual else program_source
     srcs = [gen(blob, dtype_code, only=g) for g in groups if g]
-    with ThreadPoolExecutor(max_workers=len(srcs)) as ex:
-        return list(ex.map(lambda a: _compile_one(a[1], a[0]), enumerate(srcs)))
+    return _compile_all(srcs, len(srcs))
+
+
+def _compile_all(srcs, workers, tag=""):
+    from concurrent.futures import ThreadPoolExecutor
+    t0, before = time.time(), STATS["objects_compiled"]
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        out = list(ex.map(lambda a: _compile_one(a[1], tag + str(a[0])), enumerate(srcs)))
+    if STATS["objects_compiled"] > before:
+        STATS["compile_wall_s"] += time.time() - t0
+    return out
+
+
+_LOADED = {}            # structure key -> Specialized (most recent last)
+_LOADED_MAX = 64
+
+
+def specialized_for(prog, dtype_code, dual=False):
+    """Loaded kernels for ``prog``, shared by every program of the same pass
+    structure: the kernel text holds no matrix, angle or address, so programs
+    that differ only in those values (a circuit re-run with new gate
+    matrices, e.g. each step of an optimisation) reuse one loaded module."""
+    gen = adjoint_source if dual else program_source
+    key = (dtype_code, dual, hashlib.sha1(gen(prog.blob, dtype_code).encode()).hexdigest())
+    sp = _LOADED.pop(key, None)
+    if sp is None:
+        sp = Specialized(prog, dtype_code, dual)
+        while len(_LOADED) >= _LOADED_MAX:
+            _LOADED.pop(next(iter(_LOADED)))
+    _LOADED[key] = sp
+    return sp
 
 
 class Specialized:
@@ -370,7 +409,12 @@ class Specialized:
 
     def close(self):
         if self.handle:
+            import torch
+
             from . import _lib
+            # kernels of this module may still be queued: let them finish
+            # before the code object is unloaded
+            torch.cuda.synchronize()
             _lib.load().svh_sp_free(self.handle)
             self.handle = None
 
@@ -386,7 +430,6 @@ def compile_batch(kernels, dtype_code, per_file=8, workers=8):
     (body, dual, tb).  ``per_file`` kernels per code object (so the cache key
     of a file does not depend on the machine), ``workers`` hipcc runs at a
     time; cached by content hash like the per-pass kernels."""
-    from concurrent.futures import ThreadPoolExecutor
     names = sorted(kernels)
     if not names:
         return []
@@ -401,8 +444,7 @@ def compile_batch(kernels, dtype_code, per_file=8, workers=8):
             body, dual, tb = kernels[nm]
             text.append(batch_kernel(body, dtype_code, dual, tb, nm))
         srcs.append("\n".join(text))
-    with ThreadPoolExecutor(max_workers=max(1, min(workers, len(srcs)))) as ex:
-        return list(ex.map(lambda a: _compile_one(a[1], "b%d" % a[0]), enumerate(srcs)))
+    return _compile_all(srcs, min(workers, len(srcs)), "b")
 
 
 class BatchKernels:
@@ -427,6 +469,8 @@ class BatchKernels:
 
         from . import _lib
         if getattr(self, "handle", None) and self.handle.value:
+            import torch
+            torch.cuda.synchronize()
             _lib.load().svh_sp_free(self.handle)
             self.handle = ctypes.c_void_p()
 

@@ -2,15 +2,27 @@
 
 Surface follows src/qaravan/tensorQ/statevector_sim.py:13-109 and
 src/qaravan/core/base_sim.py:3-63: ``StatevectorSim(circ, init_state=None,
-backend=..., device=...)``, ``run(progress_bar, debug)`` returning the state as
-a rank-n tensor of shape (2,)*n, ``get_statevector()`` returning the flat 2^n
-vector, ``state``, ``ran``, ``apply_gate``, ``normalize_state``, and the
-observable helpers.  Differences, all deliberate:
+backend="numpy", device="cpu")``, ``run(progress_bar, debug)`` returning the
+state as a rank-n array of shape (2,)*n, ``get_statevector()`` returning the
+flat 2^n vector, ``state``, ``ran``, ``apply_gate``, ``normalize_state``, and
+the observable helpers.  Every backend computes on the GPU; the backend string
+only chooses what the caller gets back, as the reference's does
+(statevector_sim.py:19-29):
 
-* ``backend="hip"`` (the default and only value): the state is a torch tensor
-  on ``device`` (complex64 by default, ``dtype=torch.complex128`` optional) that
-  is updated IN PLACE by the library; the reference builds a new array per gate
-  (statevector_sim.py:139-140).
+* ``"numpy"`` (the default): complex128 on the GPU, results returned as host
+  numpy arrays (one device-to-host copy when the state is read).
+* ``"torch"``: complex64 (the reference casts to it) on the GPU, results
+  returned as torch tensors on ``device`` that autograd can differentiate
+  with respect to every gate matrix (and the initial state) that requires a
+  gradient: the backward pass is the adjoint sweep of qaravan_amd.state_autograd,
+  not a tape.
+* ``"hip"``: the state stays a device tensor that the library updates in
+  place (``device`` must then be a GPU; complex64 unless ``dtype`` says
+  otherwise).
+
+``dtype=`` overrides the precision of any backend.  Other differences, all
+deliberate:
+
 * ``run()`` compiles the whole layered circuit once into fused passes
   (qaravan_amd.program) instead of one contraction per gate.
 * Expectation values are FP64 reductions on the device; they do not re-run the
@@ -22,7 +34,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .program import TileConfig, compile_ops, lower_circuit, lower_gate
+from .program import compile_ops, lower_circuit, lower_gate
 from .utils import pretty_print_sv
 
 
@@ -126,20 +138,54 @@ def apply_matrix_inplace(flat, n, mat, indices):
 
 
 def op_action(op, indices, sv, local_dim=2):
-    """New state (M on ``indices``) ψ, same shape as ``sv`` (statevector_sim.py:
-    124-140): a device copy of ``sv`` updated by one dense-gate kernel."""
+    """New state (M on ``indices``) ψ, same shape and kind as ``sv``
+    (statevector_sim.py:124-140), computed by one dense-gate kernel on the
+    GPU: a numpy array in gives a numpy array out, a torch tensor gives a
+    tensor on its device, differentiable when ``op`` or ``sv`` requires grad."""
     if local_dim != 2:
         raise NotImplementedError("the HIP statevector path supports qubits (local_dim = 2) only")
-    t = sv if torch.is_tensor(sv) else torch.as_tensor(np.asarray(sv, dtype=np.complex128))
-    if t.device.type != "cuda":
-        _lib.require_gpu()
-        t = t.to("cuda")
-    flat = t.reshape(-1).clone()
-    n = flat.numel().bit_length() - 1
+    _lib.require_gpu()
     if isinstance(indices, (int, np.integer)):
         indices = [int(indices)]
-    apply_matrix_inplace(flat, n, op, list(indices))
-    return flat.view(t.shape)
+    indices = [int(q) for q in indices]
+    as_numpy = not torch.is_tensor(sv)
+    t = torch.as_tensor(np.asarray(sv, dtype=np.complex128)) if as_numpy else sv
+    dev = t.device if t.device.type == "cuda" else torch.device("cuda", torch.cuda.current_device())
+    dtype = t.dtype if t.dtype in (torch.complex64, torch.complex128) else torch.complex128
+    N = t.numel()
+    n = N.bit_length() - 1
+    if (1 << n) != N:
+        raise ValueError(f"state has {N} amplitudes, not a power of two")
+    from .gates import Gate
+    from .state_autograd import differentiable_run
+    g = Gate("op", indices, op)
+    out = differentiable_run(n, [g], dtype, dev, None, DeviceScratch(dev), psi0=t)
+    if as_numpy:
+        return out.detach().cpu().numpy().reshape(np.shape(sv))
+    return out.to(t.device).reshape(t.shape)
+
+
+def initial_state(n, init, dtype, device):
+    """Flat device state for ``init``: None = |0…0⟩, a bitstring (first
+    character = qubit 0, the most significant bit), or an array/tensor of
+    2^n amplitudes (statevector_sim.py:31-48)."""
+    lib = _lib.load()
+    code = _lib.dtype_code(dtype)
+    if init is None or isinstance(init, str):
+        idx = 0
+        if isinstance(init, str):
+            if len(init) != n:
+                raise ValueError("bitstring length must equal the number of qubits")
+            idx = int(init, 2)
+        flat = torch.empty(2 ** n, dtype=dtype, device=device)
+        _lib.check(lib.svh_init_basis(_lib.ptr(flat), n, code, idx, _lib.stream_ptr(device)), "svh_init_basis")
+        return flat
+    if isinstance(init, (np.ndarray, torch.Tensor)):
+        sv = torch.as_tensor(init)
+        if sv.numel() != 2 ** n:
+            raise ValueError(f"init_state has {sv.numel()} amplitudes, expected {2 ** n}")
+        return sv.detach().resolve_conj().reshape(-1).to(device=device, dtype=dtype).clone()
+    raise ValueError("init_state must be a NumPy array, Torch tensor, or a bitstring")
 
 
 def inner(a, b, n, scratch):
@@ -152,14 +198,16 @@ def inner(a, b, n, scratch):
     return torch.complex(out[0], out[1])
 
 
+_BACKENDS = ("numpy", "torch", "hip")
+_DEFAULT_DTYPE = {"numpy": torch.complex128, "torch": torch.complex64, "hip": torch.complex64}
+
+
 class StatevectorSim:
     """Dense statevector simulator on one MI355X (see module docstring)."""
 
-    def __init__(self, circ, init_state=None, backend="hip", device="cuda", dtype=torch.complex64,
-                 tile=None):
-        if backend != "hip":
-            raise ValueError(f"backend must be 'hip' (got {backend!r}); this package runs the statevector "
-                             "path only on the GPU")
+    def __init__(self, circ, init_state=None, backend="numpy", device="cpu", dtype=None, tile=None):
+        if backend not in _BACKENDS:
+            raise ValueError(f"backend must be one of {_BACKENDS} (got {backend!r})")
         _lib.require_gpu()
         self.circ = circ.copy()
         self.num_sites = circ.num_sites
@@ -170,55 +218,73 @@ class StatevectorSim:
         self.init_state = init_state
         self.backend = backend
         self.device = torch.device(device)
-        if self.device.type != "cuda":
-            raise ValueError("device must be a GPU device (e.g. 'cuda' or 'cuda:0')")
-        self.dtype = dtype
-        _lib.dtype_code(dtype)
-        self.tile = tile or TileConfig()
+        if backend == "hip" and self.device.type != "cuda":
+            raise ValueError("backend 'hip' keeps the state on a GPU device (e.g. 'cuda' or 'cuda:0')")
+        # where the kernels run: the given GPU, or the current one when the
+        # results are to be returned on the host
+        self.compute_device = (self.device if self.device.type == "cuda"
+                               else torch.device("cuda", torch.cuda.current_device()))
+        self.dtype = dtype or _DEFAULT_DTYPE[backend]
+        _lib.dtype_code(self.dtype)
+        if tile is None:
+            from .autotune import sim_tile_config
+            tile = sim_tile_config(self.num_sites, self.dtype)
+        self.tile = tile
         self._flat = None
+        self._host = None
         self.ran = False
         self._prog = None
-        self._scratch = DeviceScratch(self.device)
+        self._scratch = DeviceScratch(self.compute_device)
 
     # ------------------------------------------------------------ state
+    def _set_flat(self, t):
+        self._flat = t
+        self._host = None
+
     @property
     def state(self):
+        """The state as a rank-n array of shape (2,)*n: a numpy array
+        ("numpy"), a torch tensor on ``device`` ("torch"), or the device
+        tensor the library updates ("hip")."""
         if self._flat is None:
             return None
-        return self._flat.view([self.local_dim] * self.num_sites)
+        shape = [self.local_dim] * self.num_sites
+        if self.backend == "numpy":
+            if self._host is None:
+                self._host = self._flat.detach().cpu().numpy()
+            return self._host.reshape(shape)
+        if self.backend == "torch":
+            return self._flat.to(self.device).reshape(shape)
+        return self._flat.view(shape)
 
     @state.setter
     def state(self, value):
         if value is None:
-            self._flat = None
+            self._set_flat(None)
             return
-        t = torch.as_tensor(value).to(device=self.device, dtype=self.dtype).reshape(-1).contiguous()
-        self._flat = t
+        t = torch.as_tensor(value).reshape(-1)
+        if not (self.backend == "torch" and t.requires_grad):
+            t = t.detach().resolve_conj()
+        t = t.to(device=self.compute_device, dtype=self.dtype)
+        self._set_flat(t if t.requires_grad else t.contiguous().clone())
 
     def initialize_state(self):
         """|0…0⟩, a bitstring, or a given vector (statevector_sim.py:31-48)."""
-        n = self.num_sites
-        lib = _lib.load()
-        code = _lib.dtype_code(self.dtype)
-        if self.init_state is None or isinstance(self.init_state, str):
-            idx = 0
-            if isinstance(self.init_state, str):
-                if len(self.init_state) != n:
-                    raise ValueError("bitstring length must equal the number of qubits")
-                idx = int(self.init_state, 2)
-            if self._flat is None or self._flat.numel() != 2 ** n:
-                self._flat = torch.empty(2 ** n, dtype=self.dtype, device=self.device)
-            _lib.check(lib.svh_init_basis(_lib.ptr(self._flat), n, code, idx, _lib.stream_ptr(self.device)),
-                       "svh_init_basis")
-        elif isinstance(self.init_state, (np.ndarray, torch.Tensor)):
-            sv = torch.as_tensor(self.init_state)
-            if sv.numel() != 2 ** n:
-                raise ValueError(f"init_state has {sv.numel()} amplitudes, expected {2 ** n}")
-            self._flat = sv.reshape(-1).to(device=self.device, dtype=self.dtype).clone()
-        else:
-            raise ValueError("init_state must be a NumPy array, Torch tensor, or a bitstring")
+        init = self.init_state
+        if self.backend == "torch" and torch.is_tensor(init) and init.requires_grad:
+            self._set_flat(init.reshape(-1).to(device=self.compute_device, dtype=self.dtype))
+            return
+        self._set_flat(initial_state(self.num_sites, init, self.dtype, self.compute_device))
 
     # ------------------------------------------------------------ gates
+    def _check_no_grad(self, gates):
+        if self.backend == "hip" and torch.is_grad_enabled():
+            for g in gates:
+                if torch.is_tensor(g.matrix) and g.matrix.requires_grad:
+                    raise NotImplementedError(
+                        f"gate {g.name} has a matrix that requires grad: the 'hip' backend updates the state in "
+                        "place and does not differentiate; use backend='torch'")
+
     def program(self):
         if self._prog is None:
             circ = self.circ.build() if not self.circ.built else self.circ
@@ -226,12 +292,29 @@ class StatevectorSim:
             self._prog = compile_ops(ops, self.num_sites, self.tile)
         return self._prog
 
+    def _run_gates(self, gates):
+        """Apply ``gates`` (application order) to the current state."""
+        from .state_autograd import differentiable_run
+        if self.backend == "torch":
+            psi0 = self._flat
+            out = differentiable_run(self.num_sites, gates, self.dtype, self.compute_device, self.tile,
+                                     self._scratch, psi0=psi0)
+            self._set_flat(out)
+            return
+        self._check_no_grad(gates)
+        with torch.no_grad():
+            ops = []
+            for g in gates:
+                ops.extend(lower_gate(g, self.num_sites))
+            if ops:
+                run_program(self._flat, compile_ops(ops, self.num_sites, self.tile), self._scratch)
+        self._host = None
+
     def apply_gate(self, gate):
-        """Apply one gate in place (statevector_sim.py:50-52)."""
+        """Apply one gate (statevector_sim.py:50-52)."""
         if self._flat is None:
             self.initialize_state()
-        prog = compile_ops(lower_gate(gate, self.num_sites), self.num_sites, self.tile)
-        run_program(self._flat, prog, self._scratch)
+        self._run_gates([gate])
 
     def apply_reset(self, reset_op):
         self.apply_gate(reset_op)   # the lowered Reset carries its normalisation
@@ -249,84 +332,112 @@ class StatevectorSim:
                         print(self._pretty())
                 if debug == 1:
                     print(self._pretty())
+        elif self.backend == "torch":
+            gates = [g for layer in circ.layers for g in layer]
+            self._run_gates(gates)
         else:
-            prog = self.program()
-            if progress_bar:
-                from tqdm import tqdm
-                with tqdm(total=1, desc=f"{prog.n_passes} fused passes") as bar:
+            self._check_no_grad(circ.gate_list)
+            with torch.no_grad():
+                prog = self.program()
+                if progress_bar:
+                    from tqdm import tqdm
+                    with tqdm(total=1, desc=f"{prog.n_passes} fused passes") as bar:
+                        run_program(self._flat, prog, self._scratch)
+                        bar.update(1)
+                else:
                     run_program(self._flat, prog, self._scratch)
-                    bar.update(1)
-            else:
-                run_program(self._flat, prog, self._scratch)
+            self._host = None
         self.ran = True
         if circ.meas_sites is not None:
             self.measure(circ.meas_sites)      # drawn and discarded, as base_sim.py:49-50
         return self.state
 
     def get_statevector(self):
+        """The flat 2^n state (statevector_sim.py:102-105), of the backend's kind."""
         if not self.ran:
             self.run(progress_bar=False)
+        if self.backend == "numpy":
+            return self.state.reshape(-1)
+        if self.backend == "torch":
+            return self._flat.to(self.device).reshape(-1)
         return self._flat
 
+    def _device_state(self):
+        """The current state as a flat device tensor, outside any graph."""
+        if not self.ran and self._flat is None:
+            self.run(progress_bar=False)
+        return self._flat.detach()
+
     def normalize_state(self):
+        if self.backend == "torch" and self._flat.requires_grad:
+            self._set_flat(self._flat / torch.linalg.vector_norm(self._flat))
+            return
         lib = _lib.load()
         code = _lib.dtype_code(self.dtype)
-        out = torch.empty(1, dtype=torch.float64, device=self.device)
+        dev = self.compute_device
+        out = torch.empty(1, dtype=torch.float64, device=dev)
         buf = self._scratch.get(8 * 1024)
-        st = _lib.stream_ptr(self.device)
+        st = _lib.stream_ptr(dev)
         _lib.check(lib.svh_norm2(_lib.ptr(self._flat), self.num_sites, code, _lib.ptr(out), _lib.ptr(buf),
                                  buf.numel(), st), "svh_norm2")
         _lib.check(lib.svh_scale_by_inv_norm(_lib.ptr(self._flat), self.num_sites, code, _lib.ptr(out), st),
                    "svh_scale_by_inv_norm")
+        self._host = None
 
     # ------------------------------------------------------------ measurement
     def measure(self, meas_sites):
         """A measurement bitstring over the sorted sites (statevector_sim.py:54-57)."""
         from .measure import measure_sv
-        return measure_sv(self.get_statevector(), meas_sites)
+        return measure_sv(self._device_state(), meas_sites)
 
     def measure_and_collapse(self, meas_sites):
-        """(bitstring, post-measurement state of the other sites) (statevector_sim.py:59-61)."""
+        """(bitstring, post-measurement state of the other sites) (statevector_sim.py:59-61);
+        the state comes back in the backend's kind."""
         from .measure import measure_and_collapse_sv
-        return measure_and_collapse_sv(self.get_statevector(), meas_sites)
+        out, post = measure_and_collapse_sv(self._device_state(), meas_sites)
+        if self.backend == "numpy":
+            return out, post.cpu().numpy()
+        if self.backend == "torch":
+            return out, post.to(self.device)
+        return out, post
 
     # ------------------------------------------------------------ observables
     def pauli_expectation(self, pauli_string):
         """Re⟨ψ|P|ψ⟩ for a string over 'ixyz' (statevector_sim.py:75-79,217-234)."""
         if not self.ran:
             self.run(progress_bar=False)
-        return float(pauli_sum_expectation(self._flat, self.num_sites, [(1.0, pauli_string)],
+        return float(pauli_sum_expectation(self._device_state(), self.num_sites, [(1.0, pauli_string)],
                                            self._scratch).item())
 
     def expectation(self, terms):
         """Σ c·⟨P⟩ over (coeff, pauli string) terms, e.g. Hamiltonian.pauli_terms()."""
         if not self.ran:
             self.run(progress_bar=False)
-        return float(pauli_sum_expectation(self._flat, self.num_sites, terms, self._scratch).item())
+        return float(pauli_sum_expectation(self._device_state(), self.num_sites, terms, self._scratch).item())
+
+    def _local_ops_expectation(self, gates):
+        psi = self._device_state()
+        right = psi.clone()
+        ops = []
+        for g in gates:
+            ops.extend(lower_gate(g, self.num_sites))
+        run_program(right, compile_ops(ops, self.num_sites, self.tile, fuse=False), self._scratch)
+        return float(inner(psi, right, self.num_sites, self._scratch).real.item())
 
     def one_local_expectation(self, op, site):
         """Re⟨ψ|O_site|ψ⟩ for a 2x2 O (statevector_sim.py:81-94)."""
         if not self.ran:
             self.run(progress_bar=False)
-        right = self._flat.clone()
         from .gates import Gate
-        prog = compile_ops(lower_gate(Gate("O", [site], np.asarray(op)), self.num_sites), self.num_sites,
-                           self.tile, fuse=False)
-        run_program(right, prog, self._scratch)
-        return float(inner(self._flat, right, self.num_sites, self._scratch).real.item())
+        return self._local_ops_expectation([Gate("O", [site], np.asarray(op))])
 
     def local_expectation(self, local_ops):
         """Re⟨ψ|⊗_i O_i|ψ⟩ for one 2x2 per site (statevector_sim.py:63-73)."""
         if not self.ran:
             self.run(progress_bar=False)
-        right = self._flat.clone()
         from .gates import Gate
-        ops = []
-        for i in range(self.num_sites):
-            ops.extend(lower_gate(Gate("O", [i], np.asarray(local_ops[i])), self.num_sites))
-        prog = compile_ops(ops, self.num_sites, self.tile, fuse=False)
-        run_program(right, prog, self._scratch)
-        return float(inner(self._flat, right, self.num_sites, self._scratch).real.item())
+        return self._local_ops_expectation([Gate("O", [i], np.asarray(local_ops[i]))
+                                            for i in range(self.num_sites)])
 
     def _pretty(self):
         return pretty_print_sv(self._flat.detach().cpu().numpy(), self.local_dim)

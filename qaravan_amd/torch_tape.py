@@ -94,6 +94,22 @@ def _energy_elementwise(state, terms):
     return e
 
 
+def tape_run(gates, n, device, dtype=torch.complex64, init=None):
+    """The reference torch backend's run, restated: ψ from |0…0⟩ (or ``init``)
+    through every gate in order, each matrix cast to ``dtype`` on ``device``
+    (statevector_sim.py:19-24) and contracted with ``op_action`` (:124-140).
+    Flat result, differentiable by tape autograd through the matrices."""
+    if init is None:
+        state = torch.zeros([2] * n, dtype=dtype, device=device)
+        state.view(-1)[0] = 1.0
+    else:
+        state = torch.as_tensor(init).reshape([2] * n).to(dtype=dtype, device=device)
+    for g in gates:
+        m = g.matrix if torch.is_tensor(g.matrix) else torch.as_tensor(g.matrix)
+        state = op_action(m.to(dtype=dtype, device=device), list(g.indices), state)
+    return state.reshape(-1)
+
+
 def tape_energy(circ, theta, terms, device, kernel="tensordot"):
     """E(θ) through the restated torch backend; differentiable by autograd.
     Gates with ``param_index`` take θ[param_index] (a CPU float64 leaf, as in

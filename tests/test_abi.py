@@ -98,6 +98,8 @@ def test_product_fails_loudly_without_gpu():
     from qaravan_amd import StatevectorSim
     from qaravan_amd.workloads import brickwork_circuit as bw
     with pytest.raises(RuntimeError):
-        StatevectorSim(bw(4, 1))
+        StatevectorSim(bw(4, 1))                      # numpy results, GPU compute: no GPU here
+    with pytest.raises(RuntimeError):
+        StatevectorSim(bw(4, 1), backend="torch")
     with pytest.raises(ValueError):
-        StatevectorSim(bw(4, 1), backend="numpy")
+        StatevectorSim(bw(4, 1), backend="jax")

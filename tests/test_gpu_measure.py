@@ -55,7 +55,7 @@ def test_partial_overlap(gpu, dtype, n, skip):
 def test_rdm_properties_at_22_qubits(gpu):
     from qaravan_amd import StatevectorSim
     from qaravan_amd.workloads import brickwork_circuit
-    sim = StatevectorSim(brickwork_circuit(22, 6), device=gpu, dtype=C64)
+    sim = StatevectorSim(brickwork_circuit(22, 6), backend="hip", device=gpu, dtype=C64)
     sv = sim.get_statevector()
     for sites in ([0, 21], [10, 11, 12], [3, 9, 14, 20]):
         r = M.rdm_from_sv(sv, sites).cpu().numpy()
@@ -104,7 +104,7 @@ def test_simulator_measure_and_run_with_meas_sites(gpu):
     from qaravan_amd import CNOT, Circuit, H, StatevectorSim
     n = 4
     circ = Circuit([H(0), CNOT([1, 0], n), CNOT([2, 1], n), CNOT([3, 2], n)], n)
-    sim = StatevectorSim(circ, device=gpu, dtype=C128)
+    sim = StatevectorSim(circ, backend="hip", device=gpu, dtype=C128)
     sim.run(progress_bar=False)
     np.random.seed(3)
     outs = {sim.measure([0, 1, 2, 3]) for _ in range(40)}
@@ -115,7 +115,7 @@ def test_simulator_measure_and_run_with_meas_sites(gpu):
     exp[0 if out == '00' else 3] = 1.0
     assert np.allclose(col.cpu().numpy(), exp)
     circ2 = Circuit([H(0), CNOT([1, 0], n)], n, meas_sites=[0, 1])
-    sim2 = StatevectorSim(circ2, device=gpu, dtype=C64)
+    sim2 = StatevectorSim(circ2, backend="hip", device=gpu, dtype=C64)
     st = sim2.run(progress_bar=False)
     assert st.shape == (2,) * n and sim2.ran
 

@@ -85,10 +85,10 @@ def test_haar_block_circuit_vs_oracle(gpu, n, layers):
     from qaravan_amd import StatevectorSim
     circ = haar_block_circuit(n, layers)
     ref = O.run(circ.gate_list, n)
-    sim = StatevectorSim(circ, device=gpu, dtype=torch.complex64)
+    sim = StatevectorSim(circ, backend="hip", device=gpu, dtype=torch.complex64)
     sim.run(progress_bar=False)
     err = np.linalg.norm(sim.get_statevector().cpu().numpy() - ref)
-    sim128 = StatevectorSim(circ, device=gpu, dtype=torch.complex128)
+    sim128 = StatevectorSim(circ, backend="hip", device=gpu, dtype=torch.complex128)
     sim128.run(progress_bar=False)
     err128 = np.linalg.norm(sim128.get_statevector().cpu().numpy() - ref)
     print(f"haar n={n} layers={layers}: c64 err={err:.2e} c128 err={err128:.2e}", flush=True)

@@ -22,7 +22,7 @@ TOL = {C128: 1e-12, C64: 1e-5}
 
 def sim_state(circ, gpu, dtype=C64, cfg=None, init=None):
     from qaravan_amd import StatevectorSim
-    sim = StatevectorSim(circ, init_state=init, device=gpu, dtype=dtype, tile=cfg)
+    sim = StatevectorSim(circ, init_state=init, backend="hip", device=gpu, dtype=dtype, tile=cfg)
     sim.run(progress_bar=False)
     return sim, sim.get_statevector().cpu().numpy().astype(np.complex128)
 
@@ -154,7 +154,7 @@ def test_apply_gate_per_gate_matches_run(gpu):
     rng = np.random.default_rng(9)
     n = 10
     circ = random_circuit(n, 25, rng)
-    sim = StatevectorSim(circ, device=gpu, dtype=C128)
+    sim = StatevectorSim(circ, backend="hip", device=gpu, dtype=C128)
     sim.initialize_state()
     for g in circ.gate_list:
         sim.apply_gate(g)
@@ -168,7 +168,7 @@ def test_expectations_vs_oracle(gpu):
     rng = np.random.default_rng(21)
     n = 11
     circ = random_circuit(n, 60, rng)
-    sim = StatevectorSim(circ, device=gpu, dtype=C128)
+    sim = StatevectorSim(circ, backend="hip", device=gpu, dtype=C128)
     sim.run(progress_bar=False)
     ref = O.run(circ.gate_list, n)
     letters = "ixyz"

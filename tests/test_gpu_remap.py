@@ -54,7 +54,7 @@ def test_remapped_brickwork_vs_oracle(gpu, n, layers, tb, cap, split, dtype):
     cfg = P.TileConfig(tb=tb, low_bits=4, max_cost=cap, specialize=True, io_lane_bits=4, rb=5, lds_split=split,
                        remap=True)
     print(f"remap n={n} layers={layers} cfg={cfg} {dtype}", flush=True)
-    sim = StatevectorSim(circ, device=gpu, dtype=dtype, tile=cfg)
+    sim = StatevectorSim(circ, backend="hip", device=gpu, dtype=dtype, tile=cfg)
     sim.run(progress_bar=False)
     assert sim._prog.final_pos is not None
     sv = sim.get_statevector().cpu().numpy()
@@ -68,15 +68,15 @@ def test_remapped_brickwork_24q_c64_vs_c128_and_inverse(gpu):
     from qaravan_amd import Circuit, StatevectorSim
     circ = brickwork_circuit(24, 12)
     cfg = P.TileConfig(tb=8, low_bits=4, max_cost=160, specialize=True, io_lane_bits=4, rb=5, lds_split=1, remap=True)
-    ref = StatevectorSim(circ, device=gpu, dtype=torch.complex128, tile=cfg)
+    ref = StatevectorSim(circ, backend="hip", device=gpu, dtype=torch.complex128, tile=cfg)
     ref.run(progress_bar=False)
     a = ref.get_statevector().cpu().numpy()
-    sim = StatevectorSim(circ, device=gpu, dtype=torch.complex64, tile=cfg)
+    sim = StatevectorSim(circ, backend="hip", device=gpu, dtype=torch.complex64, tile=cfg)
     sim.run(progress_bar=False)
     b = sim.get_statevector().cpu().numpy()
     assert abs(np.linalg.norm(a) - 1) < 1e-12
     assert np.linalg.norm(a - b) < 1e-5
     inv = Circuit(circ.gate_list + [g.dag() for g in circ.gate_list[::-1]], n=24)
-    z = StatevectorSim(inv, device=gpu, dtype=torch.complex128, tile=cfg)
+    z = StatevectorSim(inv, backend="hip", device=gpu, dtype=torch.complex128, tile=cfg)
     z.run(progress_bar=False)
     assert abs(z.get_statevector()[0].item() - 1) < 1e-11

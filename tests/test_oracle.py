@@ -187,3 +187,76 @@ def test_hea_circuit_matches_spec():
     a = O.run(circ.gate_list, n)
     b = O.param_run(hea_spec(n, layers), theta, n)
     assert np.allclose(a, b, atol=1e-12)
+
+
+# ------------------------------------------------------------- kak ansatz
+def test_u_matrix_known_gates():
+    X = np.array([[0, 1], [1, 0]])
+    Hm = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+    assert np.allclose(O.u_matrix(np.pi, 0, np.pi), X, atol=1e-15)
+    assert np.allclose(O.u_matrix(np.pi / 2, 0, np.pi), Hm, atol=1e-15)
+    lam = 0.731
+    assert np.allclose(O.u_matrix(0, 0, lam), np.diag([1, np.exp(1j * lam)]), atol=1e-15)
+
+
+def test_kak_centre_is_swap_at_pi_over_4():
+    p = np.zeros(15)
+    p[6:9] = np.pi / 4
+    swap = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]])
+    assert np.allclose(O.kak_unitary(p), np.exp(1j * np.pi / 4) * swap, atol=1e-14)
+    assert np.allclose(O.kak_unitary(np.zeros(15)), np.eye(4), atol=1e-15)
+
+
+def test_kak_unitary_package_matches_oracle():
+    import torch
+    from qaravan_amd.param_gates import U, kak_unitary
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        p = rng.uniform(-np.pi, np.pi, 15)
+        ref = O.kak_unitary(p)
+        assert np.allclose(ref.conj().T @ ref, np.eye(4), atol=1e-13)
+        assert np.abs(kak_unitary(p) - ref).max() < 1e-14
+        got = kak_unitary(torch.tensor(p)).numpy()
+        assert np.abs(got - ref).max() < 1e-14
+        assert np.abs(U(0, *p[:3]).matrix - O.u_matrix(*p[:3])).max() < 1e-15
+
+
+def test_u_solve_angles_and_decompose():
+    from qaravan_amd.param_gates import U
+    rng = np.random.default_rng(6)
+    t, p, l = rng.uniform(0.1, 3.0, 3)
+    g = U(0, np.exp(0.4j) * O.u_matrix(t, p, l))          # built from a matrix with a global phase
+    assert np.allclose(g.angles, (t, p, l), atol=1e-12)
+    # the ZSX decomposition in application order, as the reference lists it reversed
+    m = np.eye(2, dtype=complex)
+    for d in U(0, t, p, l).decompose():
+        m = np.asarray(d.matrix) @ m
+    rz = lambda a: np.diag([np.exp(-1j * a), np.exp(1j * a)])
+    sx = 0.5 * np.array([[1 + 1j, 1 - 1j], [1 - 1j, 1 + 1j]]) * np.exp(-1j * np.pi / 4)
+    assert np.allclose(m, rz(p + np.pi) @ sx @ rz(t + np.pi) @ sx @ rz(l), atol=1e-12)
+
+
+def test_fd4_gradient_exact_on_cubic_and_trig():
+    f = lambda x: 3 * x[0] ** 3 - 2 * x[0] * x[1] + x[1] ** 2
+    x = np.array([0.7, -1.3])
+    g = O.fd4_gradient(f, x, [0, 1], h=1e-2)
+    assert np.allclose(g, [9 * x[0] ** 2 - 2 * x[1], -2 * x[0] + 2 * x[1]], rtol=1e-12)
+    f2 = lambda x: np.sin(2 * x[0]) * np.cos(x[1])
+    g2 = O.fd4_gradient(f2, x, [0, 1])
+    ref = [2 * np.cos(2 * x[0]) * np.cos(x[1]), -np.sin(2 * x[0]) * np.sin(x[1])]
+    assert np.allclose(g2, ref, rtol=1e-11, atol=1e-13)
+
+
+def test_infidelity_bounds_and_zero_at_own_state():
+    from qaravan_amd.skeletons import brickwall_skeleton
+    rng = np.random.default_rng(7)
+    n = 4
+    sk = brickwall_skeleton(n, 1, 2)       # n-qubit chain, two brickwork layers
+    p = rng.normal(size=15 * len(sk))
+    psi = O.run(O.two_local_gates(sk, p), n)
+    assert abs(np.linalg.norm(psi) - 1) < 1e-13
+    assert abs(O.infidelity(p, sk, psi, n)) < 1e-14
+    t = rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
+    t /= np.linalg.norm(t)
+    v = O.infidelity(p, sk, t, n)
+    assert 0 <= v <= 1
