@@ -44,7 +44,10 @@ def parse():
     ap.add_argument("--no-blocks", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-measure", action="store_true")
-    ap.add_argument("--batch-interp", action="store_true", help="batch leg with the op-interpreting kernels")
+    ap.add_argument("--batch-rounds", action="store_true",
+                    help="also run the round-synchronous batch (per-structure compiled kernels) for comparison")
+    ap.add_argument("--no-reference-workload", action="store_true")
+    ap.add_argument("--batch-interp", action="store_true", help="round batch with the op-interpreting kernels")
     ap.add_argument("--batch-lds-split", type=int, default=BATCH_LDS_SPLIT,
                     help="batch leg: LDS re-distributions in 2^K parts (compiled kernels)")
     ap.add_argument("--batch-remap", type=int, default=int(BATCH_REMAP),
@@ -73,40 +76,43 @@ def log(*a):
 
 
 def bench_forward(args, dev):
+    """The 28-qubit step through the drop-in surface: StatevectorSim(circ,
+    backend="hip").run() (initialisation + the tuned fused program)."""
     import torch
 
-    from qaravan_amd import _lib
-    from qaravan_amd.autotune import best_tile_config
-    from qaravan_amd.program import compile_ops, lower_circuit
-    from qaravan_amd.statevector_sim import DeviceScratch, run_program
+    from qaravan_amd import StatevectorSim, _lib
+    from qaravan_amd import specialize as SP
     from qaravan_amd.workloads import brickwork_circuit
 
     n, layers = args.n, args.layers
-    circ = brickwork_circuit(n, layers).build()
+    circ = brickwork_circuit(n, layers)
     n_gates = len(circ.gate_list)
-    cfg = best_tile_config(n, torch.complex64)
-    prog = compile_ops(lower_circuit(circ, n), n, cfg)
+    c0 = dict(SP.STATS)
+    t0 = time.perf_counter()
+    sim = StatevectorSim(circ, backend="hip", device=dev, dtype=torch.complex64)
+    prog = sim.program()
+    if prog.specialize:
+        prog.specialized(_lib.SVH_C64)          # compiled (or found in the cache) and loaded
+    setup_s = time.perf_counter() - t0
+    cold = dict(objects=SP.STATS["objects_compiled"] - c0["objects_compiled"],
+                compile_s=round(SP.STATS["compile_wall_s"] - c0["compile_wall_s"], 2), setup_s=round(setup_s, 2))
+    cfg = sim.tile
     n_tile = prog.n_tile_passes()
-    log(f"[bench] forward n={n} layers={layers} gates={n_gates} cfg={cfg} program={prog.describe()}")
+    log(f"[bench] forward n={n} layers={layers} gates={n_gates} cfg={cfg} program={prog.describe()} cold={cold}")
     lib = _lib.load()
-    psi = torch.empty(2 ** n, dtype=torch.complex64, device=dev)
-    scratch = DeviceScratch(dev)
     st = torch.cuda.current_stream(dev)
 
     def step(ev=None):
         if ev is not None:
             ev[0].record(st)
-        _lib.check(lib.svh_init_basis(_lib.ptr(psi), n, 0, 0, _lib.stream_ptr(dev)), "init")
+        sim.run(progress_bar=False)
         if ev is not None:
             ev[1].record(st)
-        run_program(psi, prog, scratch)
-        if ev is not None:
-            ev[2].record(st)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize(dev)
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -114,38 +120,44 @@ def bench_forward(args, dev):
     torch.cuda.synchronize(dev)
     wall = time.perf_counter() - t0
     ms_step = 1e3 * wall / args.steps
-    prog_ms = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
+    run_ms = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
+    psi = sim.get_statevector()
     state_b = 2 * (2 ** n) * 8
-    perm_ms, perm_sweeps = 0.0, 0
+    # the parts of run() that are not tile passes, timed alone on the same
+    # stream with the same entry points and sizes: the |0…0⟩ write and, for a
+    # remapped program, the final bit permutation (one sweep)
+    tmp = torch.empty_like(psi)
+    iev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    back = None
     if prog.final_pos is not None:
-        # a remapped program's last pass stores into scratch and the run ends
-        # with one bit permutation from there into psi (one sweep); time that
-        # step alone, same entry point and sizes, so the tile-pass average
-        # excludes it
         import ctypes
         back = [0] * n
         for q, ph in enumerate(prog.final_pos):
             back[ph] = q
-        tmp = torch.empty_like(psi)
-        pev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
-        for k in range(args.steps):
-            pev[k][0].record(st)
-            _lib.check(lib.svh_permute_bits(_lib.ptr(tmp), _lib.ptr(psi), n, 0, (ctypes.c_int * n)(*back),
-                                            _lib.stream_ptr(dev)), "svh_permute_bits")
-            pev[k][1].record(st)
-        torch.cuda.synchronize(dev)
-        perm_ms = sum(e[0].elapsed_time(e[1]) for e in pev) / args.steps
-        perm_sweeps = 1
-        del tmp
-        # the timing loop above overwrote psi: recompute the result
-        step()
-        torch.cuda.synchronize(dev)
-    tile_ms = (prog_ms - perm_ms) / n_tile
+        back = (ctypes.c_int * n)(*back)
+    for k in range(args.steps):
+        iev[k][0].record(st)
+        _lib.check(lib.svh_init_basis(_lib.ptr(tmp), n, 0, 0, _lib.stream_ptr(dev)), "init")
+        iev[k][1].record(st)
+        if back is not None:
+            _lib.check(lib.svh_permute_bits(_lib.ptr(tmp), _lib.ptr(psi), n, 0, back, _lib.stream_ptr(dev)),
+                       "svh_permute_bits")
+        iev[k][2].record(st)
+    torch.cuda.synchronize(dev)
+    init_ms = sum(e[0].elapsed_time(e[1]) for e in iev) / args.steps
+    perm_ms = sum(e[1].elapsed_time(e[2]) for e in iev) / args.steps if back is not None else 0.0
+    perm_sweeps = 1 if back is not None else 0
+    del tmp
+    sim.run(progress_bar=False)                 # the timing above overwrote psi
+    torch.cuda.synchronize(dev)
+    psi = sim.get_statevector()
+    tile_ms = (run_ms - init_ms - perm_ms) / n_tile
     alg_bytes_step = (n_tile + perm_sweeps) * state_b + (2 ** n) * 8
     return dict(n=n, layers=layers, gates=n_gates, cfg=cfg, prog=prog, ms_step=ms_step,
                 gbps=alg_bytes_step / (ms_step * 1e-3) / 1e9, gates_per_s=n_gates / (ms_step * 1e-3),
                 tile_ms=tile_ms, tile_gbps=state_b / (tile_ms * 1e-3) / 1e9, n_tile=n_tile,
-                state_bytes=state_b, psi=psi, perm_ms=perm_ms, sweeps=n_tile + perm_sweeps)
+                state_bytes=state_b, psi=psi, perm_ms=perm_ms, init_ms=init_ms, sweeps=n_tile + perm_sweeps,
+                cold=cold, sim=sim)
 
 
 def check_forward(res, dev):
@@ -225,12 +237,14 @@ def bench_adjoint(dev, iters=10, tape_host=False):
                tile=dict(tb=cfg.tb, low_bits=cfg.low_bits, max_cost=cfg.max_cost, specialize=cfg.specialize,
                          io_lane_bits=cfg.io_lane_bits),
                rel_grad_err_vs_c128=rel_err,
-               torch_autograd_grads_per_s=1.0 / res["tensordot"][0],
-               speedup=res["tensordot"][0] / t_adj,
-               torch_autograd_rel_grad_err_vs_c128=res["tensordot"][1],
-               torch_autograd_elementwise_grads_per_s=1.0 / res["elementwise"][0],
-               speedup_vs_elementwise_tape=res["elementwise"][0] / t_adj,
-               torch_autograd_elementwise_rel_grad_err_vs_c128=res["elementwise"][1],
+               torch_autograd_grads_per_s=1.0 / res["elementwise"][0],
+               speedup=res["elementwise"][0] / t_adj,
+               speedup_note="against the elementwise tape (slice arithmetic per gate, the faster tape); the "
+                            "tensordot tape, a literal restatement of the reference's op_action, is slower",
+               torch_autograd_rel_grad_err_vs_c128=res["elementwise"][1],
+               torch_autograd_tensordot_grads_per_s=1.0 / res["tensordot"][0],
+               speedup_vs_tensordot_tape=res["tensordot"][0] / t_adj,
+               torch_autograd_tensordot_rel_grad_err_vs_c128=res["tensordot"][1],
                energy=e_adj, energy_c128=float(e128.item()),
                n_params=len(theta), n_gates=len(circ.gate_list))
     if tape_host:
@@ -407,8 +421,10 @@ def bench_measure(dev, n=28, reps=5, shots=1_000_000):
 
 
 def bench_batch(dev, nc=512, seed=20260004, check=6, specialize=True, lds_split=0, remap=False):
-    """512 HEA circuits (10..22 qubits, 4..32 layers) forward + adjoint as one
-    HIP-graph launch; latency from device timestamps."""
+    """512 HEA circuits (10..22 qubits, 4..32 layers) forward + adjoint as
+    round-synchronous launches (one per pass structure and round) recorded in
+    one HIP graph; latency from device timestamps.  Kept for comparison with
+    the persistent batch (bench_persistent)."""
     import numpy as np
     import torch
 
@@ -435,8 +451,50 @@ def bench_batch(dev, nc=512, seed=20260004, check=6, specialize=True, lds_split=
     torch.cuda.synchronize(dev)
     dt = (time.perf_counter() - t0) / reps
     lat = b.latencies_us()
+    kern = f"{len(b.kernels.names)} pass-structure kernels" if specialize else "op-interpreting kernels"
+    return dict(workload=f"{nc} HEA circuits, round-synchronous launches in one HIP graph ({b.n_launches} "
+                         f"launches, {kern})",
+                circuits_per_s=nc / dt, ms_per_batch=dt * 1e3, p50_latency_ms=float(np.percentile(lat, 50)) / 1e3,
+                p99_latency_ms=float(np.percentile(lat, 99)) / 1e3)
+
+
+def bench_persistent(dev, nc=512, seed=20260004, check=6):
+    """The same 512 circuits as ONE persistent launch with a device task
+    queue (qaravan_amd.persistent), recorded in a HIP graph; per-circuit
+    latency from device timestamps; achieved bytes = each step's algorithmic
+    state traffic over the batch time; gradient error against complex128
+    single-circuit adjoint runs and, for a small sample, against the oracle."""
+    import numpy as np
+    import torch
+
+    from oracle import sv_oracle as O
+    from qaravan_amd import specialize as SP
+    from qaravan_amd.adjoint import AdjointModel
+    from qaravan_amd.persistent import BatchPlan, PersistentBatch
+    from qaravan_amd.workloads import batch_specs, hea_circuit, hea_ops, tfim_terms
+    ns, ls, thetas, rng = batch_specs(nc, seed)
+    c0 = dict(SP.STATS)
+    t0 = time.perf_counter()
+    plan = BatchPlan(ns.tolist(), [hea_ops(int(n), int(l)) for n, l in zip(ns, ls)],
+                     [tfim_terms(int(n)) for n in ns], [len(t) for t in thetas],
+                     keys=list(zip(ns.tolist(), ls.tolist())))
+    plan_s = time.perf_counter() - t0
+    b = PersistentBatch(None, None, None, None, device=dev, plan=plan)
+    log(f"[bench] persistent batch: planned in {plan_s:.1f} s: {plan.describe()}")
+    b.set_theta(thetas)
+    b.capture()
+    for _ in range(2):
+        b.run()
+    torch.cuda.synchronize(dev)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.run()
+    torch.cuda.synchronize(dev)
+    dt = (time.perf_counter() - t0) / reps
+    b.check()
+    lat = b.latencies_us()
     grads = b.gradients()
-    # accuracy: a sample of circuits against complex128 single-circuit adjoint runs
     worst = 0.0
     idx = list(np.argsort(ns)[-2:]) + list(rng.choice(nc, size=check - 2, replace=False))
     for c in idx:
@@ -446,41 +504,113 @@ def bench_batch(dev, nc=512, seed=20260004, check=6, specialize=True, lds_split=
         _, g = m.energy_and_grad(torch.tensor(thetas[c], device=dev))
         g = g.cpu().numpy()
         worst = max(worst, float(np.linalg.norm(grads[c] - g) / np.linalg.norm(g)))
-    amps = float(sum(2.0 ** int(n) for n in ns))
-    kern = f"{len(b.kernels.names)} pass-structure kernels" if specialize else "op-interpreting kernels"
+    # the oracle on the smallest circuit of the batch
+    cs = int(np.argmin(np.asarray(ns) * 100 + np.asarray(ls)))
+    n, L = int(ns[cs]), int(ls[cs])
+    spec = []
+    for l in range(L):
+        for k, w in enumerate("yz"):
+            spec += [(w, [q], (2 * l + k) * n + q, None) for q in range(n)]
+        spec += [(None, [i, i + 1], None, np.diag([1, 1, 1, -1]).astype(complex)) for i in range(n - 1)]
+    _, g_ref = O.adjoint_gradient(spec, thetas[cs], n, O.tfi_terms(n, -1.0, -1.0))
+    oracle_err = float(np.linalg.norm(grads[cs] - g_ref) / np.linalg.norm(g_ref))
+    alg = plan.algorithmic_bytes()
+    d = plan.describe()
     return dict(workload=f"{nc} HEA circuits, n~U{{10..22}}, layers~U{{4..32}}, TFIM loss, forward+adjoint, "
-                         f"one HIP graph ({b.n_launches} launches, {kern})",
-                circuits_per_s=nc / dt, ms_per_batch=dt * 1e3, p50_latency_ms=float(np.percentile(lat, 50)) / 1e3,
-                p99_latency_ms=float(np.percentile(lat, 99)) / 1e3, max_rel_grad_err_vs_c128_sample=worst,
-                checked_circuits=len(idx), total_amplitudes=amps)
+                         f"one persistent launch ({d['tasks']} tasks in {d['queues']} queues, generic kernels)",
+                circuits_per_s=nc / dt, ms_per_batch=dt * 1e3,
+                p50_latency_ms=float(np.percentile(lat, 50)) / 1e3, p99_latency_ms=float(np.percentile(lat, 99)) / 1e3,
+                max_rel_grad_err_vs_c128_sample=worst, checked_circuits=len(idx),
+                rel_grad_err_vs_oracle=oracle_err, oracle_circuit=f"{n}q x {L} layers",
+                algorithmic_bytes=alg, achieved_gbps=alg / dt / 1e9, hbm_frac=alg / dt / 8e12,
+                bytes_note="ψ read+write per forward tile pass, ψ and λ per adjoint pass, ψ read + λ written by "
+                           "the Hamiltonian step; most circuits' states live in L2/MALL, so this can exceed HBM",
+                plan_s=round(plan_s, 1), compile_s=round(SP.STATS["compile_wall_s"] - c0["compile_wall_s"], 2),
+                total_amplitudes=float(sum(2.0 ** int(n) for n in ns)))
 
 
-def cpu_baseline(n, layers, max_seconds=20.0):
+def bench_reference_workload(dev, ns=(16, 18, 20, 22, 24, 26)):
+    """The reference's own published GPU workload (examples/gpu_acceleration.ipynb,
+    cells 2-3): two_local_circ(brickwall_skeleton(n, 1)) — n-1 random 4x4
+    gates — run with StatevectorSim(circ, backend="torch", device="cuda"),
+    timed from construction to synchronisation as the notebook does.  The
+    reference's plot puts a Colab T4 at about 0.011 s (n=16) ... 4.5 s (n=26)."""
+    import numpy as np
+    import torch
+
+    from qaravan_amd import StatevectorSim, brickwall_skeleton, two_local_circ
+    from qaravan_amd import specialize as SP
+    t4 = {16: 0.011, 20: 0.12, 22: 0.32, 24: 1.15, 26: 4.5}
+    rng = np.random.default_rng(20260099)
+    out = {}
+    for n in ns:
+        circ = two_local_circ(brickwall_skeleton(n, 1), rng=rng)
+        c0 = SP.STATS["compile_wall_s"]
+        times = []
+        for r in range(4):
+            torch.cuda.synchronize(dev)
+            t0 = time.perf_counter()
+            sim = StatevectorSim(circ, backend="torch", device="cuda")
+            sim.run(progress_bar=False)
+            torch.cuda.synchronize(dev)
+            times.append(time.perf_counter() - t0)
+        out[str(n)] = dict(first_s=round(times[0], 4), warm_s=round(float(np.median(times[1:])), 5),
+                           compile_s=round(SP.STATS["compile_wall_s"] - c0, 2), t4_s=t4.get(n),
+                           speedup_vs_t4=(round(t4[n] / float(np.median(times[1:])), 1) if n in t4 else None))
+        del sim
+    torch.cuda.empty_cache()
+    return out
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:  # pragma: no cover
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n, layers, max_seconds=10.0):
     """Oracle (numpy complex128, one tensordot per gate) on the first gates of
-    the same layered circuit, until ~max_seconds of CPU work."""
+    the same layered circuit, until ~max_seconds of CPU work, with all BLAS
+    threads and with one."""
     import numpy as np
 
     from oracle import sv_oracle as O
     from qaravan_amd.workloads import brickwork_circuit
     try:
-        from threadpoolctl import threadpool_info
+        from threadpoolctl import threadpool_info, threadpool_limits
         threads = max([d.get("num_threads", 1) for d in threadpool_info()] + [1])
     except Exception:  # pragma: no cover
-        threads = os.cpu_count()
+        threadpool_limits, threads = None, os.cpu_count()
     circ = brickwork_circuit(n, layers)
     seq = [g for layer in O.construct_layers(circ.gate_list) for g in layer]
-    sv = O.init_state(n)
-    done = 0
-    t0 = time.perf_counter()
-    for g in seq:
-        sv = O.op_action(np.asarray(g.matrix), g.indices, sv)
-        done += 1
-        if time.perf_counter() - t0 > max_seconds:
-            break
-    dt = time.perf_counter() - t0
-    return dict(value=done / dt, unit="gates/s", cores=int(threads), kind="port",
+
+    def leg(limit):
+        sv = O.init_state(n)
+        done = 0
+        t0 = time.perf_counter()
+        for g in seq:
+            if threadpool_limits is not None:
+                with threadpool_limits(limits=limit):
+                    sv = O.op_action(np.asarray(g.matrix), g.indices, sv)
+            else:
+                sv = O.op_action(np.asarray(g.matrix), g.indices, sv)
+            done += 1
+            if time.perf_counter() - t0 > max_seconds:
+                break
+        return done, time.perf_counter() - t0
+
+    done, dt = leg(threads)
+    done1, dt1 = leg(1)
+    return dict(value=done / dt, unit="gates/s", cores=int(threads), kind="port", cpu_model=_cpu_model(),
+                one_thread=dict(value=done1 / dt1, unit="gates/s", cores=1,
+                                sample=f"first {done1} gates, {dt1:.1f} s"),
                 sample=f"first {done} gates (of {len(seq)}) of the {n}-qubit {layers}-layer brickwork "
-                       f"circuit, numpy complex128 oracle, {dt:.1f} s")
+                       f"circuit, numpy complex128 oracle, {dt:.1f} s with {threads} BLAS threads")
 
 
 def cpu_other_configs(max_seconds=15.0):
@@ -534,6 +664,19 @@ def cpu_other_configs(max_seconds=15.0):
     return out
 
 
+def valu_limiter():
+    """What limits the tile pass besides HBM: the FP32 VALU issue rate, from
+    the committed rocprofv3 counter run (SQ_INSTS_VALU over the 28-qubit
+    passes; VALU busy = instructions x 4 cycles / (SIMDs x cycles))."""
+    path = os.path.join(ROOT, "profiles", "tile_pass_valu_28q.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        d = json.load(f)
+    return {"limiter": "valu", "valu_busy": d.get("valu_busy_step"),
+            "valu_busy_heaviest_passes": d.get("valu_busy_heaviest"), "valu_source": "profiles/tile_pass_valu_28q.json"}
+
+
 def traffic_bytes(fwd):
     """HBM bytes per tile-pass launch from the committed rocprofv3 counter runs
     (FETCH_SIZE x2 calibration + WRITE_SIZE, KiB per dispatch), or None."""
@@ -576,6 +719,8 @@ def main():
         f"{fwd['gates_per_s']:.0f} gates/s, tile pass {fwd['tile_ms']:.3f} ms = {fwd['tile_gbps']:.0f} GB/s")
     err, norm = check_forward(fwd, dev)
     log(f"[bench] c64 vs c128 at {fwd['n']}q: {err:.3e} (norm {norm:.12f})")
+    del fwd["sim"], fwd["psi"]
+    torch.cuda.empty_cache()
     adj = None
     if not args.no_adjoint:
         adj = bench_adjoint(dev, tape_host=args.tape_host)
@@ -584,11 +729,18 @@ def main():
     if not args.no_blocks:
         blocks = bench_blocks(dev)
         log(f"[bench] fused blocks: {blocks}")
-    batch = None
+    batch = rounds = None
     if not args.no_batch:
-        batch = bench_batch(dev, specialize=not args.batch_interp, lds_split=args.batch_lds_split,
-                            remap=bool(args.batch_remap))
-        log(f"[bench] batch: {batch}")
+        batch = bench_persistent(dev)
+        log(f"[bench] persistent batch: {batch}")
+        if args.batch_rounds:
+            rounds = bench_batch(dev, specialize=not args.batch_interp, lds_split=args.batch_lds_split,
+                                 remap=bool(args.batch_remap))
+            log(f"[bench] round-synchronous batch: {rounds}")
+    ref_wl = None
+    if not args.no_reference_workload:
+        ref_wl = bench_reference_workload(dev)
+        log(f"[bench] reference workload: {ref_wl}")
     c0 = bench_c0(dev)
     log(f"[bench] c0: {c0}")
     meas = None
@@ -631,8 +783,10 @@ def main():
                      "frac": round(fwd["tile_gbps"] / peak, 4), "traffic": traffic_bytes(fwd),
                      "kernel": ("svh_sp_<p> (specialised tile pass)" if fwd["cfg"].specialize
                                 else "k_tile<float, tb, 4, false>"),
-                     "bytes_per_launch": fwd["state_bytes"], "avg_launch_ms": round(fwd["tile_ms"], 4)},
+                     "bytes_per_launch": fwd["state_bytes"], "avg_launch_ms": round(fwd["tile_ms"], 4),
+                     **valu_limiter()},
         "cpu_baseline": cpu,
+        "cold_start": {"forward": fwd["cold"]},
     }
     if adj is not None:
         out["adjoint"] = adj
@@ -640,6 +794,11 @@ def main():
         out["fused_blocks"] = blocks
     if batch is not None:
         out["batch"] = batch
+        out["cold_start"]["batch"] = dict(plan_s=batch["plan_s"], compile_s=batch["compile_s"])
+    if rounds is not None:
+        out["batch_rounds"] = rounds
+    if ref_wl is not None:
+        out["reference_workload"] = ref_wl
     if meas is not None:
         out["measure"] = meas
     out["c0"] = c0

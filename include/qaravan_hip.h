@@ -36,6 +36,8 @@
  *   svh_batch_*, svh_spb_* tensorQ/monte_carlo_sim.py:14-35 (many small StatevectorSim runs,
  *                         one after another); here a whole batch of parametrised circuits
  *                         with their adjoint gradients runs as one launch sequence
+ *   svh_batch_persistent  the same many runs (and applications/compilation.py:73-133 once
+ *                         per circuit), as one persistent launch with a device task queue
  *   svh_timestamp         no counterpart (device clock for per-circuit latencies)
  *   svh_marginal_probs    tensorQ/statevector_sim.py:185-187 measure_sv (diag of rdm_from_sv)
  *   svh_sample_cdf        tensorQ/statevector_sim.py:188-193 np.random.choice(outcomes, p=probs)
@@ -175,6 +177,29 @@ int svh_batch_trig(const double *theta_dev, int64_t np, void *trig_dev, int dtyp
 int svh_spb_load(const char *const *paths, int npaths, const char *const *names, int nnames, void **handle);
 int svh_spb_launch(void *handle, int index, const void *items_dev, int nitems, int64_t total_wg, int tb,
                    size_t lds_bytes, void *stream);
+
+/* Persistent batch: ONE launch (after three small fill kernels that zero the
+ * counters, queue heads and error flag) runs every task of a batch.  descs_dev:
+ * svh_pdesc_t array; tasks_dev: svh_ptask_t array holding nq queues back to
+ * back, queue q = tasks [qbeg[q], qbeg[q+1]) (qbeg_dev: nq + 1 int32, device);
+ * heads_dev: nq int32 scratch; counters_dev: ncounters int32 scratch named by
+ * the descriptors' dep/done/fin fields; err_dev: one int32, set to 1 when a
+ * dependency wait exceeded its bound (the results are then invalid).  Every
+ * descriptor's dependencies must come earlier in its own queue.  lds_bytes:
+ * the largest tile a task needs (2·2^L amplitudes for an adjoint tile);
+ * max_workgroups: cap on the resident grid (0 = fill the device).
+ * xcd_local != 0 asks for XCD-local synchronisation: a workgroup serves only
+ * the queue of the XCD it runs on and dependencies are made visible through
+ * that XCD's shared L2 (no device-wide write-back / invalidation per task);
+ * it is used only when the full grid covers exactly XCDs 0..nq-1 (checked
+ * once per device and grid shape by a small census kernel with a temporary
+ * allocation, outside graph capture), otherwise workgroups help other queues
+ * with device-scope release/acquire.  *used_mode (optional host int) = 1 for
+ * XCD-local, 0 otherwise.  Tile passes must be generic passes with 3 register
+ * bits.  Calls that overlap on one set of counters race. */
+int svh_batch_persistent(const void *descs_dev, const void *tasks_dev, const int32_t *qbeg_dev, int nq,
+                         int32_t *heads_dev, int32_t *counters_dev, int ncounters, int32_t *err_dev, int dtype,
+                         size_t lds_bytes, int max_workgroups, int xcd_local, int *used_mode, void *stream);
 
 /* out_dev[0] = wall clock (s_memrealtime, 100 MHz ticks), for batch latencies. */
 int svh_timestamp(uint64_t *out_dev, void *stream);

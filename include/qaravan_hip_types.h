@@ -169,6 +169,51 @@ typedef struct {
     int32_t n, nterms;
 } svh_batch_ham_t; /* 48 bytes */
 
+/* ---- persistent batch ---------------------------------------------------------
+ * One launch runs a whole batch (forward passes, λ = Hψ with the energy,
+ * adjoint passes, gradient sums) as a list of tasks.  Each task covers
+ * `count` units of one descriptor: tiles of a pass, 1024-amplitude blocks of
+ * the Hamiltonian step, or slots of a reduction.  A descriptor waits until
+ * counter `dep` reaches `dep_need` (the units of the step it reads) and adds
+ * its finished units to counter `done`, so every circuit advances as soon as
+ * its own previous step is complete.  Tasks sit in per-queue lists in an
+ * order where every dependency comes earlier in the same queue. */
+enum {
+    SVH_PT_FWD = 1, /* tile units of a forward pass                               */
+    SVH_PT_ADJ = 2, /* tile units of an adjoint pass (ψ and λ, gradient rows)      */
+    SVH_PT_HAM = 3, /* blocks of λ = Σ c_t P_t ψ plus per-block Re⟨ψ|λ⟩ partials   */
+    SVH_PT_RED = 4, /* slot sums of partial rows, fixed order                      */
+    SVH_PT_SCT = 5  /* one unit: out[slot_param[s]] += slotsum[s] for s ascending   */
+};
+
+typedef struct {
+    int32_t kind;       /* SVH_PT_*                                                  */
+    int32_t n;          /* qubits of the circuit                                       */
+    int32_t dep;        /* counter to wait for, or -1                                  */
+    int32_t dep_need;   /* value that counter must reach                               */
+    int32_t done;       /* counter that receives the finished units, or -1             */
+    int32_t units;      /* tiles / blocks / slots                                      */
+    int32_t ng;         /* ADJ, RED: gradient slots (partial-row length); HAM: terms  */
+    int32_t rows;       /* RED: partial rows                                           */
+    int32_t fin;        /* RED, SCT: counter of a circuit's finished sums, or -1       */
+    int32_t fin_need;   /* the value of `fin` once the circuit's gradient is complete  */
+    uint64_t psi, lam;  /* FWD, ADJ, HAM: state and adjoint vectors                    */
+    uint64_t pass;      /* FWD, ADJ: the pass record                                   */
+    uint64_t stages;    /* FWD, ADJ: the pass's first stage record                     */
+    uint64_t ops;       /* FWD, ADJ: the program's op array                            */
+    uint64_t trig;      /* FWD, ADJ: (cos, sin) table of the circuit                   */
+    uint64_t part;      /* ADJ, RED: partial rows; HAM: per-block energy partials      */
+    uint64_t out;       /* RED, SCT: float64 outputs; HAM: the term table              */
+    uint64_t slot_param;/* RED, SCT: int32 output index per slot                       */
+    uint64_t slotsum;   /* RED: 0 = store the sum at out[slot_param[s]]; else slotsum[s]
+                           (SCT then adds them in slot order)                           */
+    uint64_t tstamp;    /* RED, SCT: uint64 completion time of the circuit, or 0       */
+} svh_pdesc_t; /* 128 bytes */
+
+typedef struct {
+    int32_t desc, first, count, pad;
+} svh_ptask_t; /* 16 bytes */
+
 #ifdef __cplusplus
 }
 #endif

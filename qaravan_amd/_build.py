@@ -28,9 +28,16 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
+# per-file flags.  The generic tile kernels keep their register arrays in
+# VGPRs only if LLVM does not merge look-alike switch arms into one block that
+# addresses the array through a phi of pointers (SimplifyCFG sinking), which
+# would force the array into scratch memory.
+FILE_FLAGS = {"svh_gtile.hip": ["-mllvm", "-simplifycfg-sink-common=false"]}
+
+
 def _compile(src, objdir):
     obj = os.path.join(objdir, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:

@@ -33,6 +33,9 @@ def _adjoint_cfg(n, dtype, cfg):
     register bits are forward-only."""
     rb = min(cfg.rb, 4)
     tb_max = (9 if dtype == torch.complex64 else 8) + (4 - rb)
+    if cfg.generic:
+        # at most 512 threads, and ψ and λ tiles within 160 KiB of LDS
+        tb_max = min(9, (13 if dtype == torch.complex64 else 12) - rb)
     tb = min(cfg.tb, tb_max, n - rb)
     return replace(cfg, tb=tb, rb=rb, prefetch=False)
 
@@ -71,16 +74,30 @@ class ParamProgram:
         self.n_gates = sum(1 for _ in ops)
 
 
+def real_coefficients(terms):
+    """The real coefficients of a Pauli sum; a coefficient with an imaginary
+    part (beyond rounding, 1e-12·|c|) makes the sum non-Hermitian, whose
+    expectation ⟨ψ|H|ψ⟩ the loss here does not define, so it is refused."""
+    out = []
+    for c, p in terms:
+        c = complex(c)
+        if abs(c.imag) > 1e-12 * max(abs(c), 1e-300):
+            raise ValueError(f"term {p!r} has complex coefficient {c}: the Pauli sum must be Hermitian")
+        out.append(c.real)
+    return np.array(out, dtype=np.float64)
+
+
 def term_table(n, terms):
     """Device-layout Pauli-sum table: (xmask, zmask, Re, Im of c·i^{ny}) per term."""
     dt = np.dtype([('x', '<u8'), ('z', '<u8'), ('cre', '<f8'), ('cim', '<f8')])
     from .statevector_sim import _pauli_masks
     xm, zm = _pauli_masks(n, [p for _, p in terms])
+    cs = real_coefficients(terms)
     t = np.zeros(len(terms), dtype=dt)
     ph = [(1.0, 0.0), (0.0, 1.0), (-1.0, 0.0), (0.0, -1.0)]
-    for i, (c, _) in enumerate(terms):
+    for i, c in enumerate(cs):
         ny = bin(int(xm[i]) & int(zm[i])).count("1") & 3
-        t[i] = (xm[i], zm[i], float(np.real(c)) * ph[ny][0], float(np.real(c)) * ph[ny][1])
+        t[i] = (xm[i], zm[i], c * ph[ny][0], c * ph[ny][1])
     return t
 
 

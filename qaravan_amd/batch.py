@@ -40,6 +40,19 @@ def default_tb(n, dtype, rb=RB):
     return max(4, min(12 - rb, n - rb))
 
 
+def batch_config(n, dtype, rb=3, remap_adjoint=True):
+    """(forward, adjoint) tile configurations of one circuit in the persistent
+    batch: generic kernels, tiles of 2^12 amplitudes (2^11 for complex128, so
+    ψ and λ fit twice per CU's LDS) or the whole state when it is smaller, no
+    pass cap.  The forward program keeps the natural bit order (the
+    Hamiltonian step reads it); the adjoint sweep may remap, since its ψ and λ
+    are scratch afterwards."""
+    top = 12 if dtype == torch.complex64 else 11
+    tb = max(6, min(top - rb, n - rb))
+    cfg = TileConfig(tb=tb, low_bits=4, max_cost=1e9, io_lane_bits=4, rb=rb, generic=True)
+    return cfg, replace(cfg, remap=bool(remap_adjoint))
+
+
 class BatchedAdjoint:
     """Energies and gradients of E_c(θ_c) = ⟨ψ_c(θ_c)|H_c|ψ_c(θ_c)⟩ for a list of
     circuits (rotations carry ``param_index`` into their own θ_c)."""
@@ -94,6 +107,12 @@ class BatchedAdjoint:
             else:
                 pp = ParamProgram(None, terms_list[c], n_params_list[c], cfg=cfg, adj_cfg=adj_cfg, dtype=dtype,
                                   ops=op_lists[c], n=n)
+            slots = pp.adj.grad_params.tolist()
+            if len(set(slots)) != len(slots):
+                # the round reductions add slot sums with atomics: a parameter
+                # shared by several gates would get them in a varying order
+                raise NotImplementedError("gates sharing a parameter: use persistent.PersistentBatch, which adds "
+                                          "the slot sums of a parameter in slot order")
             if key is not None:
                 cache[key] = pp
             self.pps.append(pp)

@@ -422,8 +422,12 @@ static int launch_tile(cpx<R> *psi, cpx<R> *lam, int n, const svh_pass_t &hp, co
     return fail(-7, "tile pass tb not built");
 }
 
-// Validate a program's host copy against n; returns 0 or an error.
-static int check_program(const void *prog_host, int n, const svh_prog_header_t **hdr) {
+bool gtile_pattern_matches(int rb, bool dual, int pid, const svh_op_t *ops, int nops);
+
+// Validate a program's host copy against n; returns 0 or an error.  mode: the
+// direction the program is about to run in (0 forward, 1 adjoint sweep), which
+// fixes the pattern library its generic stages refer to; -1 skips that check.
+static int check_program(const void *prog_host, int n, const svh_prog_header_t **hdr, int mode) {
     if (!prog_host) return fail(-1, "program pointer is NULL");
     const auto *h = static_cast<const svh_prog_header_t *>(prog_host);
     if (h->magic != SVH_PROG_MAGIC) return fail(-8, "bad program magic");
@@ -440,6 +444,11 @@ static int check_program(const void *prog_host, int n, const svh_prog_header_t *
                 return fail(-8, "tile pass " + std::to_string(p) + ": register bits must be 3, 4 or 5");
             if (P.L != P.tb + P.rb || P.L > h->n_qubits || P.L > SVH_MAX_TILE_BITS)
                 return fail(-8, "tile pass " + std::to_string(p) + ": bad L");
+            const bool generic = (P.flags & SVH_PASSF_GENERIC) != 0;
+            if (generic && (P.rb < 3 || P.rb > 4 || P.tb < 6 || P.tb > 9 ||
+                            (P.flags & (SVH_PASSF_HALF_LDS | SVH_PASSF_QUARTER_LDS | SVH_PASSF_PREFETCH))))
+                return fail(-8, "tile pass " + std::to_string(p) +
+                                    ": generic passes need rb 3 or 4, 6 <= tb <= 9, no LDS split or prefetch");
             const int extra = (P.flags & SVH_PASSF_REMAP_STORE) ? 1 : 0;  // the store record
             if (P.n_stages < 1 || P.stage_begin < 0 || P.stage_begin + P.n_stages + extra > h->n_stages)
                 return fail(-8, "tile pass " + std::to_string(p) + ": bad stage range");
@@ -488,6 +497,10 @@ static int check_program(const void *prog_host, int n, const svh_prog_header_t *
                     if (S.lsw[j] < 0 || S.lsw[j] >= (1 << P.L))
                         return fail(-8, "stage " + std::to_string(s) + ": bad LDS offset table");
                 }
+                if (generic && S.pattern >= 0 && mode >= 0 &&
+                    (s >= P.stage_begin + P.n_stages ||
+                     !gtile_pattern_matches(P.rb, mode == 1, S.pattern, ops + S.op_begin, S.op_end - S.op_begin)))
+                    return fail(-8, "stage " + std::to_string(s) + ": pattern id does not match its op sequence");
                 for (int o = S.op_begin; o < S.op_end; ++o) {
                     const svh_op_t &O = ops[o];
                     const bool two = O.kind == SVH_OP_MAT2 || O.kind == SVH_OP_CX ||
@@ -710,15 +723,28 @@ static size_t trig_bytes(const svh_prog_header_t *h, int dtype) {
     return ((size_t)h->n_params * es + 255) & ~(size_t)255;
 }
 
+namespace svh {
+int gtile_launch(const void *psi, void *out, void *lam, int n, int dtype, const svh_pass_t &hp,
+                 const svh_pass_t *dp, const svh_stage_t *dst, const svh_op_t *dop, const void *trig,
+                 double *gpart, hipStream_t stream);
+}  // namespace svh
+
+// Rows of gradient partials an adjoint tile pass writes: one per tile, or one
+// per wave in the generic kernels.
+static uint64_t grad_rows(const svh_pass_t &P, int n) {
+    const uint64_t tiles = 1ull << (n - P.L);
+    return (P.flags & SVH_PASSF_GENERIC) ? tiles << (P.tb - 6) : tiles;
+}
+
 // Doubles of gradient partials of all tile passes of an adjoint program (each
-// pass has its own region: nb·ng rows), plus one sum per gradient slot.
+// pass has its own region: rows·ng), plus one sum per gradient slot.
 static size_t grad_doubles(const svh_prog_header_t *h) {
     const char *b = reinterpret_cast<const char *>(h);
     const auto *passes = reinterpret_cast<const svh_pass_t *>(b + h->off_passes);
     size_t d = (size_t)h->n_grad_slots;
     for (int p = 0; p < h->n_passes; ++p) {
         const svh_pass_t &P = passes[p];
-        if (P.kind == SVH_PASS_TILE && P.n_grad > 0) d += ((size_t)1 << (h->n_qubits - P.L)) * P.n_grad;
+        if (P.kind == SVH_PASS_TILE && P.n_grad > 0) d += grad_rows(P, h->n_qubits) * P.n_grad;
     }
     return d;
 }
@@ -788,7 +814,7 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
     if (dtype != SVH_C64 && dtype != SVH_C128) return fail(-3, "bad dtype");
     if (!prog_dev) return fail(-1, "device program pointer is NULL");
     const svh_prog_header_t *h = nullptr;
-    if (int e = check_program(prog_host, n, &h)) return e;
+    if (int e = check_program(prog_host, n, &h, dual ? 1 : 0)) return e;
     const size_t need = scratch_need(h, dtype, dual);
     if (!scratch || scratch_bytes < need)
         return fail(-5, "scratch too small: need " + std::to_string(need) + " bytes");
@@ -851,13 +877,20 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
             gp = sd + goff;
             if (chunk.npass == kGradChunk && (e = flush_chunk())) return e;
             chunk.off[chunk.npass] = (int64_t)goff;
-            chunk.nb[chunk.npass] = (int32_t)(1ull << (n - P.L));
+            chunk.nb[chunk.npass] = (int32_t)grad_rows(P, n);
             chunk.ng[chunk.npass] = P.n_grad;
             chunk.slot0[chunk.npass] = P.grad_begin;
             chunk.npass++;
-            goff += ((size_t)1 << (n - P.L)) * P.n_grad;
+            goff += grad_rows(P, n) * P.n_grad;
         }
-        if (P.kind == SVH_PASS_TILE && sp && sp->fn[p]) {
+        if (P.kind == SVH_PASS_TILE && (P.flags & SVH_PASSF_GENERIC)) {
+            // the generic kernels; a remapped program's last pass stores into
+            // the scratch tail, which the final permutation reads
+            void *out = (!dual && h->off_perm && p == h->n_passes - 1) ? perm_tmp : psi;
+            e = gtile_launch(psi, out, dual ? lam : nullptr, n, dtype, P, dpass + p, dstage + P.stage_begin, dop,
+                             trig, gp, st);
+            if (out != psi) last_in_tmp = true;
+        } else if (P.kind == SVH_PASS_TILE && sp && sp->fn[p]) {
             // a remapped program's last compiled pass stores into the scratch
             // tail, which the final permutation reads
             void *out = (!dual && h->off_perm && p == h->n_passes - 1) ? perm_tmp : nullptr;
@@ -987,7 +1020,7 @@ int svh_batch_tile(const void *items_dev, int nitems, int64_t total_wg, int dtyp
 int svh_program_scratch_bytes(const void *prog_host, int dtype, size_t *bytes) {
     if (!bytes) return fail(-1, "bytes pointer is NULL");
     const svh_prog_header_t *h = nullptr;
-    if (int e = check_program(prog_host, -1, &h)) return e;
+    if (int e = check_program(prog_host, -1, &h, -1)) return e;
     *bytes = scratch_need(h, dtype, true);
     return 0;
 }
@@ -1108,6 +1141,7 @@ int svh_run_adjoint(void *psi, void *lam, int n, int dtype, const void *prog_hos
 }
 
 }  // extern "C"
+
 
 
 extern "C" int svh_permute_bits(const void *src, void *dst, int n, int dtype, const int *perm, void *stream) {

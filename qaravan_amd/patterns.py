@@ -32,7 +32,8 @@ from . import program as P
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "csrc", "svh_patterns.json")
 HEADER = os.path.join(HERE, "csrc", "svh_patterns.h")
-LIB_RB = (3, 4)
+LIB_RB = (3,)           # register bits that get compiled patterns
+DISPATCH_RB = (3, 4)    # register bits the generic kernels are built for
 
 _table = None
 
@@ -115,6 +116,16 @@ def generate(keys_by_lib):
         out.append("    switch (pid) {")
         out += [f"        case {i}: pat{tag}_{i}<R, RB>({call}); return true;" for i in range(len(keys))]
         out += ["        default: return false;", "    }", "}", ""]
+        # host-side table of the same sequences, for the library's program checks
+        offs, flat = [0], []
+        for key in keys:
+            flat += [list(op) for op in key]
+            offs.append(len(flat))
+        out.append(f"constexpr int kPatCount{tag} = {len(keys)};")
+        out.append(f"constexpr int kPatOff{tag}[] = {{{', '.join(map(str, offs))}}};")
+        rows = ", ".join("{" + ", ".join(map(str, op)) + "}" for op in flat) or "{0, 0, 0, 0, 0}"
+        out.append(f"constexpr signed char kPatOps{tag}[][5] = {{{rows}}};")
+        out.append("")
     out += ["}  // namespace svh", ""]
     table = {f"{rb}:{int(dual)}": [list(map(list, k)) for k in keys] for (rb, dual), keys in sorted(keys_by_lib.items())}
     return "\n".join(out), json.dumps(table, separators=(",", ":"))
@@ -125,6 +136,8 @@ def _canonical_programs():
     hardware-efficient ansatz (RY, RZ on every qubit, then a CZ ladder) at
     10-22 qubits and 1-32 layers in the tile shapes the batched and
     single-circuit adjoint kernels use."""
+    from dataclasses import replace
+
     import torch
 
     from .adjoint import ParamProgram
@@ -135,12 +148,13 @@ def _canonical_programs():
     for rb in LIB_RB:
         for n in range(10, 23):
             for L in range(1, 33):
-                cfg, adj_cfg = batch_config(n, torch.complex64, rb)
-                pp = ParamProgram(None, tfim_terms(n), 2 * n * L, cfg=cfg, adj_cfg=adj_cfg, dtype=torch.complex64,
-                                  ops=hea_ops(n, L), n=n)
-                out += [(pp.fwd, False), (pp.adj, True)]
+                for remap in (True, False):
+                    cfg, adj_cfg = batch_config(n, torch.complex64, rb, remap)
+                    pp = ParamProgram(None, tfim_terms(n), 2 * n * L, cfg=cfg, adj_cfg=adj_cfg,
+                                      dtype=torch.complex64, ops=hea_ops(n, L), n=n)
+                    out += ([(pp.fwd, False)] if remap else []) + [(pp.adj, True)]
     for n in (16, 20, 22):
-        cfg = best_adjoint_config(n, torch.complex64)
+        cfg = replace(best_adjoint_config(n, torch.complex64), specialize=False, lds_split=0, generic=True)
         pp = ParamProgram(None, tfim_terms(n), 32 * n, cfg=cfg, dtype=torch.complex64, ops=hea_ops(n, 16), n=n)
         out += [(pp.fwd, False), (pp.adj, True)]
     return out
@@ -148,13 +162,13 @@ def _canonical_programs():
 
 def build_library():
     from .specialize import _decode
-    keys = {(rb, d): [] for rb in LIB_RB for d in (False, True)}
+    keys = {(rb, d): [] for rb in DISPATCH_RB for d in (False, True)}
     seen = {k: set() for k in keys}
     for prog, dual in _canonical_programs():
         _, passes, stages, ops = _decode(prog.blob)
         for p in passes:
             rb = int(p['rb'])
-            if int(p['kind']) != P.PASS_TILE or (rb, dual) not in keys:
+            if int(p['kind']) != P.PASS_TILE or rb not in LIB_RB:
                 continue
             for s in range(int(p['stage_begin']), int(p['stage_begin']) + int(p['n_stages'])):
                 st = stages[s]

@@ -1,0 +1,60 @@
+"""Ahead-of-time compilation of the per-pass kernels the library's standard
+workloads use, so that a fresh install runs them without waiting for hipcc.
+
+The per-pass compiled kernels (qaravan_amd.specialize) are keyed by the text
+of the pass structure, not by angles or matrices, so compiling a workload's
+program once covers every run of that structure.  ``build()`` calls
+``precompile()`` after building the library; everything else (the
+generic kernels, the persistent batch) is already inside libqaravan_hip.so.
+Workloads covered:
+
+* the 28-qubit, 40-layer brickwork circuit through ``StatevectorSim``'s
+  default (tuned) configuration, complex64;
+* the 20-qubit, 16-layer hardware-efficient ansatz gradient (forward and
+  adjoint programs of the tuned adjoint configuration), complex64;
+* the reference's published GPU workload, two_local_circ(brickwall_skeleton(n, 1))
+  at n = 20 ... 26, where ``StatevectorSim`` switches to the tuned kernels.
+"""
+import time
+
+import torch
+
+from . import specialize as SP
+
+
+def _programs():
+    from .adjoint import ParamProgram
+    from .autotune import best_adjoint_config, sim_tile_config
+    from .circuits import two_local_circ
+    from .program import compile_ops, lower_circuit
+    from .skeletons import brickwall_skeleton
+    from .workloads import brickwork_circuit, hea_ops, tfim_terms
+    c64 = torch.complex64
+    out = []
+    n = 28
+    prog = compile_ops(lower_circuit(brickwork_circuit(n, 40).build(), n), n, sim_tile_config(n, c64))
+    out.append(("brickwork 28q x 40, complex64", prog, False))
+    n = 20
+    pp = ParamProgram(None, tfim_terms(n), 2 * n * 16, cfg=best_adjoint_config(n, c64), dtype=c64,
+                      ops=hea_ops(n, 16), n=n)
+    out.append(("HEA 20q x 16 forward", pp.fwd, False))
+    out.append(("HEA 20q x 16 adjoint", pp.adj, True))
+    for n in (20, 22, 24, 26):
+        circ = two_local_circ(brickwall_skeleton(n, 1)).build()
+        prog = compile_ops(lower_circuit(circ, n), n, sim_tile_config(n, c64))
+        out.append((f"two-local brickwall {n}q", prog, False))
+    return out
+
+
+def precompile(log=print):
+    """Compile (or find in the cache) every listed program's kernels;
+    returns {workload: seconds}."""
+    done = {}
+    for name, prog, dual in _programs():
+        if not prog.specialize:
+            continue
+        t0 = time.time()
+        SP.compile_program(prog.blob, 0, dual=dual)
+        done[name] = round(time.time() - t0, 1)
+        log(f"[precompile] {name}: {done[name]} s")
+    return done

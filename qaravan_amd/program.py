@@ -376,7 +376,15 @@ def _is_barrier(op):
 
 def is_diagonal(op):
     """Diagonal in the computational basis: such ops commute with each other,
-    so a diagonal op may move past another diagonal op on shared bits."""
+    so a diagonal op may move past another diagonal op on shared bits.
+    (Cached on the op: the schedulers ask about every op many times.)"""
+    d = op.__dict__.get('_diag')
+    if d is None:
+        d = op.__dict__['_diag'] = _is_diagonal(op)
+    return d
+
+
+def _is_diagonal(op):
     if op.kind == 'cz':
         return True
     if op.kind in ('rot1', 'rot2'):
@@ -578,13 +586,17 @@ def _stage_ops(p, io_lane_bits=6, keep_shared=0, free_store=False):
     while pend:
         if cur is not None:
             took, blocked, keep = [], _Blocker(), []
-            for op, lb in pend:
+            creg = set(cur.rbits)
+            for i, (op, lb) in enumerate(pend):
                 s = set(lb)
-                if not blocked.blocks(op, s) and s <= set(cur.rbits):
+                if not blocked.blocks(op, s) and s <= creg:
                     took.append((op, lb))
                 else:
                     blocked.add(op, s)
                     keep.append((op, lb))
+                    if creg <= blocked.other:       # every register bit is held back:
+                        keep.extend(pend[i + 1:])   # nothing later can join this stage
+                        break
             if took:
                 cur.ops.extend(took)
                 pend = keep
@@ -595,6 +607,8 @@ def _stage_ops(p, io_lane_bits=6, keep_shared=0, free_store=False):
                 s = set(lb)
                 if blocked.blocks(op, s) or s & avoid:
                     blocked.add(op, s)
+                    if len(blocked.other) == L:     # every tile bit is held back
+                        break
                     continue
                 new = [x for x in lb if x not in R]
                 # keep_shared: the last free slots are for bits of the previous layout

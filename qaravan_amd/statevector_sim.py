@@ -100,7 +100,8 @@ def run_program(state, prog, scratch, theta=None):
 def pauli_sum_expectation(state, n, terms, scratch):
     """Σ_t c_t ⟨ψ|P_t|ψ⟩ as a 0-d float64 device tensor (FP64 accumulation)."""
     lib = _lib.load()
-    coeffs = np.array([float(np.real(c)) for c, _ in terms], dtype=np.float64)
+    from .adjoint import real_coefficients
+    coeffs = real_coefficients(terms)
     xm, zm = _pauli_masks(n, [p for _, p in terms])
     out = torch.empty(1, dtype=torch.float64, device=state.device)
     buf = scratch.get(8 * 1024 + 32 * len(terms) + 256)
@@ -114,7 +115,8 @@ def pauli_sum_expectation(state, n, terms, scratch):
 def pauli_sum_apply(state, out, n, terms, scratch):
     """out = Σ_t c_t P_t ψ on the device."""
     lib = _lib.load()
-    coeffs = np.array([float(np.real(c)) for c, _ in terms], dtype=np.float64)
+    from .adjoint import real_coefficients
+    coeffs = real_coefficients(terms)
     xm, zm = _pauli_masks(n, [p for _, p in terms])
     buf = scratch.get(32 * len(terms) + 256)
     _lib.check(lib.svh_pauli_apply(_lib.ptr(state), _lib.ptr(out), n, _lib.dtype_code(state.dtype),

@@ -56,3 +56,12 @@ def test_heisenberg_groups():
     assert e[0] / e[1] > 3.2
     two = Heisenberg(2, jx=0.5, jy=0.7, jz=1.1, h=0.0)
     assert np.allclose(trotter_unitary(two, 0.3, 1), expm(-1j * 0.3 * dense_h(two)), atol=1e-12)
+
+
+def test_complex_pauli_coefficients_refused():
+    from qaravan_amd.adjoint import real_coefficients, term_table
+    assert real_coefficients([(1.5, 'zz'), (-2 + 0j, 'xi')]).tolist() == [1.5, -2.0]
+    with pytest.raises(ValueError):
+        term_table(2, [(1.0, 'zz'), (0.5j, 'xx')])
+    with pytest.raises(ValueError):
+        real_coefficients([(1 + 1e-6j, 'z')])
