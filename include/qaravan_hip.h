@@ -99,6 +99,17 @@ int svh_run_program(void *psi, int n, int dtype, const void *prog_host,
                     const void *prog_dev, const double *theta_dev, void *scratch,
                     size_t scratch_bytes, void *stream);
 
+/* Allocation-free svh_run_program: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under graph capture, tight
+ * serving loops).  The scratch buffer is a fixed 16 MiB device array inside
+ * the library (each GPU it runs on holds its own copy until the process ends),
+ * so the caller passes only ψ and the program.  Programs whose scratch need
+ * (svh_program_scratch_bytes) exceeds 16 MiB are refused with -5; programs
+ * with compiled-only passes are refused as by svh_run_program.  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU race. */
+int svh_run_program_static(void *psi, int n, int dtype, const void *prog_host,
+                           const void *prog_dev, const double *theta_dev, void *stream);
+
 /* Specialised programs: code objects (hipcc --genco) holding kernels named
  * svh_sp_<p>, one per tile pass p they cover, generated for one program with
  * every op spelled out (qaravan_amd/specialize.py).  svh_sp_load resolves the

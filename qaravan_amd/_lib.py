@@ -36,6 +36,7 @@ SIGNATURES = {
     "svh_program_scratch_bytes": (_c_int, [_c_void_p, _c_int, ctypes.POINTER(_c_size_t)]),
     "svh_run_program": (_c_int, [_c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
                                  _c_size_t, _c_void_p]),
+    "svh_run_program_static": (_c_int, [_c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
     "svh_sp_load": (_c_int, [ctypes.POINTER(ctypes.c_char_p), _c_int, _c_int, ctypes.POINTER(_c_void_p)]),
     "svh_sp_free": (_c_int, [_c_void_p]),
     "svh_run_program_sp": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p,

@@ -14,6 +14,8 @@
 #include "svh_tile_ops.h"
 
 #include <algorithm>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 namespace svh {
@@ -1142,6 +1144,50 @@ int svh_run_adjoint(void *psi, void *lam, int n, int dtype, const void *prog_hos
 
 }  // extern "C"
 
+// ------------------------------------------------ allocation-free program runner
+// svh_run_program with the scratch buffer in a fixed device array instead of a
+// caller buffer, for callers that cannot allocate (graph capture, serving
+// loops).  Every device holds its own copy of the array for the life of the
+// process; the host only ever uses its device address, looked up once per
+// device (hipGetSymbolAddress) and kept here.
+namespace svh {
+constexpr size_t kStaticScratchBytes = 16u << 20;
+}  // namespace svh
+static __device__ __align__(256) unsigned char scratch[svh::kStaticScratchBytes];
+
+static int static_scratch(void **out) {
+    static std::mutex mu;
+    static std::unordered_map<int, void *> ptrs;
+    int dev = 0;
+    if (int e = hip_status(hipGetDevice(&dev), "hipGetDevice")) return e;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = ptrs.find(dev);
+    if (it != ptrs.end()) {
+        *out = it->second;
+        return 0;
+    }
+    void *p = nullptr;
+    if (int e = hip_status(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return e;
+    ptrs[dev] = p;
+    *out = p;
+    return 0;
+}
+
+extern "C" int svh_run_program_static(void *psi, int n, int dtype, const void *prog_host, const void *prog_dev,
+                                      const double *theta_dev, void *stream) {
+    if (!psi) return fail(-1, "state pointer is NULL");
+    if (dtype != SVH_C64 && dtype != SVH_C128) return fail(-3, "bad dtype");
+    const svh_prog_header_t *h = nullptr;
+    if (int e = check_program(prog_host, n, &h, 0)) return e;
+    const size_t need = scratch_need(h, dtype, false);
+    if (need > kStaticScratchBytes)
+        return fail(-5, "program needs " + std::to_string(need) + " bytes of scratch; the static scratch holds " +
+                            std::to_string(kStaticScratchBytes));
+    void *p = nullptr;
+    if (int e = static_scratch(&p)) return e;
+    return run_impl(psi, nullptr, n, dtype, prog_host, prog_dev, theta_dev, nullptr, p, kStaticScratchBytes, stream,
+                    false);
+}
 
 
 extern "C" int svh_permute_bits(const void *src, void *dst, int n, int dtype, const int *perm, void *stream) {
