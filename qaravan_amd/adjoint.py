@@ -161,6 +161,10 @@ class AdjointEnergy(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g_e):
+        if ctx.psi is None:
+            # the sweep walks ψ and λ back in place: they are gone after one backward
+            raise RuntimeError("AdjointEnergy: backward ran twice through the same graph (retain_graph=True); "
+                               "the adjoint sweep consumes the saved state, so evaluate the energy again instead")
         grad = adjoint_sweep(ctx.pp, ctx.psi, ctx.lam, ctx.th, ctx.scratch)
         ctx.psi = ctx.lam = None
         return (g_e * grad).to(ctx.theta_dtype), None, None

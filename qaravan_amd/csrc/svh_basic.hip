@@ -2,6 +2,11 @@
 // (norm, inner product, Pauli-sum expectation) and H·ψ for Pauli sums.
 #include "svh_common.h"
 
+#include <map>
+#include <mutex>
+#include <set>
+#include <tuple>
+
 namespace svh {
 
 static thread_local std::string g_last_error;
@@ -17,6 +22,43 @@ int hip_status(hipError_t e, const char *where) {
     if (e == hipSuccess) return 0;
     g_last_error = std::string(where) + ": " + hipGetErrorString(e);
     return (int)e;
+}
+
+static std::mutex g_launch_mu;
+
+int ensure_lds_limit(const void *kern, int bytes, const char *where) {
+    static std::set<std::tuple<int, const void *, int>> done;
+    int dev = 0;
+    if (int e = hip_status(hipGetDevice(&dev), "hipGetDevice")) return e;
+    std::lock_guard<std::mutex> lock(g_launch_mu);
+    const auto key = std::make_tuple(dev, kern, bytes);
+    if (done.count(key)) return 0;
+    if (int e = hip_status(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, bytes), where))
+        return e;
+    done.insert(key);
+    return 0;
+}
+
+int resident_shape(const void *kern, int threads, size_t lds, int *per_cu, int *cus) {
+    static std::map<std::tuple<int, const void *, int, size_t>, std::pair<int, int>> cache;
+    int dev = 0;
+    if (int e = hip_status(hipGetDevice(&dev), "hipGetDevice")) return e;
+    std::lock_guard<std::mutex> lock(g_launch_mu);
+    const auto key = std::make_tuple(dev, kern, threads, lds);
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+        int nb = 0, nc = 0;
+        if (int e = hip_status(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, threads, lds),
+                               "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
+            return e;
+        if (int e = hip_status(hipDeviceGetAttribute(&nc, hipDeviceAttributeMultiprocessorCount, dev),
+                               "hipDeviceGetAttribute"))
+            return e;
+        it = cache.emplace(key, std::make_pair(nb > 0 ? nb : 1, nc > 0 ? nc : 1)).first;
+    }
+    *per_cu = it->second.first;
+    *cus = it->second.second;
+    return 0;
 }
 
 // ---------------------------------------------------------------- init
@@ -314,9 +356,8 @@ int svh_apply_dense(void *psi, int n, int dtype, const int *bits, int k, const v
     } else {
         const size_t lds = sizeof(cpx<double>) * (K * K + 256);
         if (lds > 65536) {
-            hipError_t e = hipFuncSetAttribute((const void *)k_dense<double>,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            if (e != hipSuccess) return hip_status(e, "hipFuncSetAttribute(k_dense)");
+            if (int e = ensure_lds_limit((const void *)k_dense<double>, (int)lds, "hipFuncSetAttribute(k_dense)"))
+                return e;
         }
         k_dense<double><<<grid, 256, lds, st>>>((cpx<double> *)psi, n, k, mask, b03, b45,
                                                 (const cpx<double> *)mat_dev, N);

@@ -63,6 +63,14 @@ inline int checked_launch(const char *where) {
     return hip_status(hipGetLastError(), where);
 }
 
+// Raise a kernel's dynamic-LDS limit to `bytes` on the current device, once
+// per (device, kernel, bytes); thread-safe (a mutex-guarded set).
+int ensure_lds_limit(const void *kern, int bytes, const char *where);
+// Resident workgroups per CU of `kern` at (threads, lds) and the CU count of
+// the current device; computed once per (device, kernel, threads, lds) under
+// a mutex, so concurrent callers on several devices each get their own.
+int resident_shape(const void *kern, int threads, size_t lds, int *per_cu, int *cus);
+
 constexpr int kMaxQubits = 36;
 constexpr size_t kReduceBlocks = 1024;  // partials per FP64 reduction
 

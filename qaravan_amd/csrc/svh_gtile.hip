@@ -29,11 +29,7 @@ static int launch_gtile(const void *psi, void *out, void *lam, int n, const svh_
                         const svh_stage_t *dst, const svh_op_t *dop, const void *trig, double *gpart,
                         hipStream_t stream) {
     auto kern = k_gtile<R, RB, DUAL>;
-    static const int attr = [&] {
-        return (int)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        160 * 1024);
-    }();
-    if (attr != (int)hipSuccess) return hip_status((hipError_t)attr, "hipFuncSetAttribute(k_gtile)");
+    if (int e = ensure_lds_limit((const void *)kern, 160 * 1024, "hipFuncSetAttribute(k_gtile)")) return e;
     const size_t lds = (DUAL ? 2 : 1) * sizeof(cpx<R>) * ((size_t)1 << hp.L);
     if (lds > 160 * 1024) return fail(-7, "generic tile pass needs more than 160 KiB of LDS");
     const uint64_t grid = 1ull << (n - hp.L);

@@ -337,24 +337,11 @@ static int launch_tb_pf(cpx<R> *psi, int n, const svh_pass_t &hp, const svh_pass
     const size_t lds = 2 * sizeof(cpx<R>) * ((size_t)1 << L);
     if (lds > kMaxLds) return fail(-7, "prefetch tile pass needs " + std::to_string(lds) + " B of LDS");
     auto kern = k_tile_pf<R, TB>;
-    static int resident = 0;  // workgroups per CU, from the occupancy calculator
-    static int cus = 0;
-    if (!resident) {
-        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)kMaxLds);
-        if (e != hipSuccess) return hip_status(e, "hipFuncSetAttribute(k_tile_pf)");
-        int nb = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)kern, 1 << TB, lds);
-        if (e != hipSuccess) return hip_status(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-        int dev = 0;
-        if (int r = hip_status(hipGetDevice(&dev), "hipGetDevice")) return r;
-        if (int r = hip_status(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
-                               "hipDeviceGetAttribute"))
-            return r;
-        resident = nb > 0 ? nb : 1;
-    }
+    if (int e = ensure_lds_limit((const void *)kern, (int)kMaxLds, "hipFuncSetAttribute(k_tile_pf)")) return e;
+    int resident = 0, cus = 0;  // workgroups per CU (occupancy calculator) and CUs of this device
+    if (int e = resident_shape((const void *)kern, 1 << TB, lds, &resident, &cus)) return e;
     const uint64_t ntiles = 1ull << (n - L);
-    const uint64_t cap = (uint64_t)resident * (uint64_t)(cus > 0 ? cus : 256);
+    const uint64_t cap = (uint64_t)resident * (uint64_t)cus;
     const uint64_t grid = ntiles < cap ? ntiles : cap;
     kern<<<(unsigned)grid, 1 << TB, lds, stream>>>(psi, n, dp, st, op, trig, ntiles);
     return checked_launch("k_tile_pf");
@@ -371,13 +358,7 @@ static int launch_tb(cpx<R> *psi, cpx<R> *lam, int n, const svh_pass_t &hp, cons
                        (DUAL ? sizeof(double) * NW * (size_t)hp.n_grad : 0);
     if (lds > kMaxLds) return fail(-7, "tile pass needs " + std::to_string(lds) + " B of LDS (> 160 KiB)");
     auto kern = k_tile<R, TB, RB, DUAL>;
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)kMaxLds);
-        if (e != hipSuccess) return hip_status(e, "hipFuncSetAttribute(k_tile)");
-        attr_set = true;
-    }
+    if (int e = ensure_lds_limit((const void *)kern, (int)kMaxLds, "hipFuncSetAttribute(k_tile)")) return e;
     const uint64_t grid = 1ull << (n - L);
     if (grid > 0x7fffffffull) return fail(-7, "grid too large");
     kern<<<(unsigned)grid, 1 << TB, lds, stream>>>(psi, lam, n, dp, st, op, trig, gpart);
@@ -971,13 +952,7 @@ static int launch_batch_tb(const svh_batch_item_t *items, int nitems, int64_t to
                        (DUAL ? sizeof(double) * NW * (size_t)max_ngrad : 0);
     if (lds > kMaxLds) return fail(-7, "batched tile pass needs " + std::to_string(lds) + " B of LDS");
     auto kern = k_tile_batch<R, TB, 4, DUAL>;
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)kMaxLds);
-        if (e != hipSuccess) return hip_status(e, "hipFuncSetAttribute(k_tile_batch)");
-        attr_set = true;
-    }
+    if (int e = ensure_lds_limit((const void *)kern, (int)kMaxLds, "hipFuncSetAttribute(k_tile_batch)")) return e;
     if (total_wg <= 0 || total_wg > 0x7fffffff) return fail(-4, "bad workgroup count");
     kern<<<(unsigned)total_wg, 1 << TB, lds, stream>>>(items, nitems);
     return checked_launch("k_tile_batch");

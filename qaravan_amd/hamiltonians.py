@@ -43,8 +43,8 @@ class Hamiltonian:
             else:
                 arr = np.asarray(strength)
                 if len(arr) != num_x:
-                    raise ValueError(f"Field strength array length {len(arr)} must match "
-                                     f"number of sites {num_x}")
+                    raise ValueError(f"{op} field: {len(arr)} per-site strengths given for a chain of "
+                                     f"{num_x} sites")
                 for i, s in enumerate(arr):
                     if abs(s) > 1e-10:
                         self.terms.append((s, [i], [op]))

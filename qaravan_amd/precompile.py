@@ -9,7 +9,9 @@ generic kernels, the persistent batch) is already inside libqaravan_hip.so.
 Workloads covered:
 
 * the 28-qubit, 40-layer brickwork circuit through ``StatevectorSim``'s
-  default (tuned) configuration, complex64;
+  default (tuned) configuration, complex64, with the two variants the
+  full-size parity tests run (the circuit then its inverse, its first 6
+  layers);
 * the 20-qubit, 16-layer hardware-efficient ansatz gradient (forward and
   adjoint programs of the tuned adjoint configuration), complex64;
 * the reference's published GPU workload, two_local_circ(brickwall_skeleton(n, 1))
@@ -28,15 +30,24 @@ def _programs():
     from .circuits import two_local_circ
     from .program import compile_ops, lower_circuit
     from .skeletons import brickwall_skeleton
-    from .workloads import brickwork_circuit, hea_ops, tfim_terms
+    from .workloads import brickwork_circuit, hea_circuit, hea_theta, tfim_terms
     c64 = torch.complex64
     out = []
     n = 28
     prog = compile_ops(lower_circuit(brickwork_circuit(n, 40).build(), n), n, sim_tile_config(n, c64))
     out.append(("brickwork 28q x 40, complex64", prog, False))
+    # the full-size parity tests: the same circuit followed by its inverse, and its first 6 layers
+    from .circuits import Circuit
+    circ = brickwork_circuit(n, 40)
+    inv = Circuit(circ.gate_list + [g.dag() for g in circ.gate_list[::-1]], n=n)
+    out.append(("brickwork 28q x 40 then its inverse, complex64",
+                compile_ops(lower_circuit(inv.build(), n), n, sim_tile_config(n, c64)), False))
+    out.append(("brickwork 28q x 6, complex64",
+                compile_ops(lower_circuit(brickwork_circuit(n, 6).build(), n), n, sim_tile_config(n, c64)), False))
     n = 20
-    pp = ParamProgram(None, tfim_terms(n), 2 * n * 16, cfg=best_adjoint_config(n, c64), dtype=c64,
-                      ops=hea_ops(n, 16), n=n)
+    # the program AdjointModel builds from the circuit (the benchmark's path)
+    pp = ParamProgram(hea_circuit(n, 16, hea_theta(n, 16)), tfim_terms(n), 2 * n * 16,
+                      cfg=best_adjoint_config(n, c64), dtype=c64)
     out.append(("HEA 20q x 16 forward", pp.fwd, False))
     out.append(("HEA 20q x 16 adjoint", pp.adj, True))
     for n in (20, 22, 24, 26):

@@ -15,7 +15,13 @@ from qaravan_amd.monte_carlo import MonteCarloSim, magnetization_terms, trotter_
 from qaravan_amd.program import TileConfig, compile_ops, lower_circuit
 
 pytestmark = pytest.mark.gpu
-TOL = {torch.complex128: 1e-10, torch.complex64: 1e-5}
+# state tolerances (north star); an energy of a Pauli sum H = Σ c_t P_t differs by at most
+# |⟨ψ|H|ψ⟩ − ⟨φ|H|φ⟩| ≤ 2·Σ|c_t|·‖ψ − φ‖₂ for unit vectors, so energies are held to that bound
+TOL = {torch.complex128: 1e-12, torch.complex64: 1e-5}
+
+
+def energy_tol(terms, dtype):
+    return 2.0 * sum(abs(c) for c, _ in terms) * TOL[dtype]
 
 
 class PauliChannel:
@@ -52,7 +58,9 @@ def test_batched_forward_matches_oracle(gpu, dtype):
     BatchedForward(progs, n, gpu, dtype).run(states.reshape(-1))
     for t in range(T):
         ref = O.run(circs[t].gate_list, n, init=ref_in[t])
-        assert np.linalg.norm(states[t].cpu().numpy() - ref) < TOL[dtype] * 10
+        err = np.linalg.norm(states[t].cpu().numpy() - ref)
+        print(f"  trajectory {t}: |psi - psi_ref| = {err:.3e}", flush=True)
+        assert err <= TOL[dtype]
 
 
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
@@ -68,13 +76,13 @@ def test_monte_carlo_samples_match_oracle(gpu, dtype):
     samples = [sim.circuit_sample() for _ in range(12)]
     for v, c in zip(vals, samples):
         sv = O.run([g for layer in c.layers for g in layer], n, layered=False)
-        assert abs(v - O.energy(sv, obs.pauli_terms())) < TOL[dtype] * 10
+        assert abs(v - O.energy(sv, obs.pauli_terms())) <= energy_tol(obs.pauli_terms(), dtype)
     np.random.seed(7)
     one = sim.expectation_sample(obs)
     np.random.seed(7)
     c = sim.circuit_sample()
     sv = O.run([g for layer in c.layers for g in layer], n, layered=False)
-    assert abs(one - O.energy(sv, obs.pauli_terms())) < TOL[dtype] * 10
+    assert abs(one - O.energy(sv, obs.pauli_terms())) <= energy_tol(obs.pauli_terms(), dtype)
 
 
 def test_dense_observable_small_register(gpu):
@@ -99,7 +107,7 @@ def test_trotter_sv_and_mc(gpu):
     for _ in range(steps):
         sv = O.run(circ.gate_list, n, init=sv)
         ref.append(O.energy(sv, op))
-    assert np.allclose(got, ref, atol=1e-10)
+    assert np.allclose(got, ref, rtol=0, atol=energy_tol(op, torch.complex128))
     # noisy trajectories: same seed -> same sampled circuits per step
     T = 6
     ch = PauliChannel(n, 0.1)
@@ -116,4 +124,4 @@ def test_trotter_sv_and_mc(gpu):
             svs[t] = O.run([g for lay in layer[t].layers for g in lay], n, init=svs[t], layered=False)
             vals.append(O.energy(svs[t], op))
         ref.append(np.mean(vals))
-    assert np.allclose(got, ref, atol=1e-10)
+    assert np.allclose(got, ref, rtol=0, atol=energy_tol(op, torch.complex128))
