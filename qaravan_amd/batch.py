@@ -53,6 +53,62 @@ def batch_config(n, dtype, rb=3, remap_adjoint=True):
     return cfg, replace(cfg, remap=bool(remap_adjoint))
 
 
+def batch_programs(ns, circuits, op_lists, terms_list, n_params_list, dtype, tb_of_n, max_cost, keys, lds_split,
+                   rb, remap):
+    """The ParamProgram of every circuit of a round-synchronous batch (host
+    only; circuits with equal ``keys`` share one).  Forward programs keep the
+    natural bit order (the batched Hamiltonian reads ψ in it); adjoint sweeps
+    may remap, their ψ and λ are scratch."""
+    pps, cache = [], {}
+    for c in range(len(ns)):
+        key = None if keys is None else keys[c]
+        if key is not None and key in cache:
+            pps.append(cache[key])
+            continue
+        n = ns[c]
+        cfg = TileConfig(tb=tb_of_n(n), low_bits=4, max_cost=max_cost, io_lane_bits=4,
+                         specialize=bool(lds_split or remap) or rb != RB, lds_split=lds_split, rb=rb)
+        adj_cfg = replace(cfg, remap=remap)
+        if circuits is not None:
+            pp = ParamProgram(circuits[c], terms_list[c], n_params_list[c], cfg=cfg, adj_cfg=adj_cfg, dtype=dtype)
+        else:
+            pp = ParamProgram(None, terms_list[c], n_params_list[c], cfg=cfg, adj_cfg=adj_cfg, dtype=dtype,
+                              ops=op_lists[c], n=n)
+        slots = pp.adj.grad_params.tolist()
+        if len(set(slots)) != len(slots):
+            # the round reductions add slot sums with atomics: a parameter
+            # shared by several gates would get them in a varying order
+            raise NotImplementedError("gates sharing a parameter: use persistent.PersistentBatch, which adds "
+                                      "the slot sums of a parameter in slot order")
+        if key is not None:
+            cache[key] = pp
+        pps.append(pp)
+    return pps
+
+
+def pass_grad_ops(p):
+    """Gradient ops of one adjoint pass (its n_grad)."""
+    return sum(1 for st in p.stages for op, _ in st.ops if op.flags & 2)
+
+
+def gpart_rows(pp, n):
+    """Doubles of gradient partials one circuit needs: per workgroup row, n_grad
+    slots, for its largest adjoint pass."""
+    m = 1
+    for p in pp.adj.passes:
+        if p.kind == PASS_TILE:
+            m = max(m, (2 ** (n - p.tb - p.rb)) * max(pass_grad_ops(p), 1))
+    return m
+
+
+def spb_lds_bytes(tb, rb, split, dual, es, max_ngrad):
+    """Dynamic LDS of one compiled batch-kernel launch: the tile (ψ, and λ in
+    an adjoint pass) in 2^split parts, plus one row of gradient slots per wave."""
+    L = tb + rb - split
+    nw = max(1, 2 ** tb // 64)
+    return (2 if dual else 1) * es * 2 ** L + (8 * nw * max_ngrad if dual else 0)
+
+
 class BatchedAdjoint:
     """Energies and gradients of E_c(θ_c) = ⟨ψ_c(θ_c)|H_c|ψ_c(θ_c)⟩ for a list of
     circuits (rotations carry ``param_index`` into their own θ_c)."""
@@ -88,34 +144,8 @@ class BatchedAdjoint:
         self.ns = [c.num_sites for c in circuits] if circuits is not None else [int(n) for n in ns]
         if min(self.ns) < 10:
             raise NotImplementedError("batched Hamiltonian kernel works on blocks of 1024 amplitudes: n >= 10")
-        self.pps = []
-        cache = {}
-        for c in range(nc):
-            key = None if keys is None else keys[c]
-            if key is not None and key in cache:
-                self.pps.append(cache[key])
-                continue
-            n = self.ns[c]
-            cfg = TileConfig(tb=tb_of_n(n), low_bits=4, max_cost=max_cost, io_lane_bits=4,
-                             specialize=bool(lds_split or remap) or rb != RB, lds_split=lds_split, rb=rb)
-            # forward programs stay in natural order (the batched Hamiltonian
-            # reads ψ in it); adjoint sweeps may remap, their ψ and λ are scratch
-            adj_cfg = replace(cfg, remap=remap)
-            if circuits is not None:
-                pp = ParamProgram(circuits[c], terms_list[c], n_params_list[c], cfg=cfg, adj_cfg=adj_cfg,
-                                  dtype=dtype)
-            else:
-                pp = ParamProgram(None, terms_list[c], n_params_list[c], cfg=cfg, adj_cfg=adj_cfg, dtype=dtype,
-                                  ops=op_lists[c], n=n)
-            slots = pp.adj.grad_params.tolist()
-            if len(set(slots)) != len(slots):
-                # the round reductions add slot sums with atomics: a parameter
-                # shared by several gates would get them in a varying order
-                raise NotImplementedError("gates sharing a parameter: use persistent.PersistentBatch, which adds "
-                                          "the slot sums of a parameter in slot order")
-            if key is not None:
-                cache[key] = pp
-            self.pps.append(pp)
+        self.pps = batch_programs(self.ns, circuits, op_lists, terms_list, n_params_list, dtype, tb_of_n, max_cost,
+                                  keys, lds_split, rb, remap)
         es = 8 if dtype == torch.complex64 else 16
         # ---- flat device buffers
         self.amp_off = np.concatenate([[0], np.cumsum([2 ** n for n in self.ns])]).astype(np.int64)
@@ -148,14 +178,7 @@ class BatchedAdjoint:
         pbase = self.progs.data_ptr()
         self.prog_addr = [(pbase + where[id(pp.fwd)], pbase + where[id(pp.adj)]) for pp in self.pps]
         # gradient partial buffers: the largest adjoint pass of each circuit
-        gp_sizes = []
-        for c, pp in enumerate(self.pps):
-            m = 1
-            for p in pp.adj.passes:
-                if p.kind == PASS_TILE:
-                    ng = sum(1 for st in p.stages for op, _ in st.ops if op.flags & 2)
-                    m = max(m, (2 ** (self.ns[c] - p.tb - p.rb)) * max(ng, 1))
-            gp_sizes.append(m)
+        gp_sizes = [gpart_rows(pp, self.ns[c]) for c, pp in enumerate(self.pps)]
         self.gp_off = np.concatenate([[0], np.cumsum(gp_sizes)]).astype(np.int64)
         self.gpart = torch.zeros(int(self.gp_off[-1]), dtype=torch.float64, device=dev)
         self.ep_off = np.concatenate([[0], np.cumsum([2 ** n // 1024 for n in self.ns])]).astype(np.int64)
@@ -222,7 +245,7 @@ class BatchedAdjoint:
                     L = tb + p.rb
                     n = self.ns[c]
                     ppass, pst, pops, pgrad = self._pass_ptrs(self.prog_addr[c][which], prog, r)
-                    ng = sum(1 for st in p.stages for op, _ in st.ops if op.flags & 2) if which else 0
+                    ng = pass_grad_ops(p) if which else 0
                     gkey = (tb, self.kname[id(prog)][r]) if self.kernels is not None else (tb, None)
                     split = 1 if p.flags & PASSF_HALF_LDS else 2 if p.flags & PASSF_QUARTER_LDS else 0
                     groups.setdefault(gkey, []).append(dict(split=split, rb=p.rb,
@@ -253,9 +276,7 @@ class BatchedAdjoint:
                         self.launches.append(('tile', add_table(arr), len(items), wg, tb, which, mng))
                     else:
                         # one kernel text per group, so one split for all its items
-                        L = tb + items[0]['rb'] - items[0]['split']
-                        nw = max(1, 2 ** tb // 64)
-                        lds = (2 if which else 1) * es * 2 ** L + (8 * nw * mng if which else 0)
+                        lds = spb_lds_bytes(tb, items[0]['rb'], items[0]['split'], bool(which), es, mng)
                         self.launches.append(('spb', add_table(arr), len(items), wg, tb, lds,
                                               self.kernels.index[kname]))
                 if reds:
