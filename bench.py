@@ -494,6 +494,16 @@ def bench_persistent(dev, nc=512, seed=20260004, check=6):
     dt = (time.perf_counter() - t0) / reps
     b.check()
     lat = b.latencies_us()
+    # one more (untimed) run with the workers' clock accounting on
+    b.enable_stats()
+    b._launch()
+    torch.cuda.synchronize(dev)
+    b.check()
+    sched = b.scheduler_stats()
+    worker_s = sched["workgroups"] * dt
+    sched = dict(workers=sched["workgroups"], task_frac=round((sched["task_s"] - sched["wait_s"]) / worker_s, 3),
+                 wait_frac=round(sched["wait_s"] / worker_s, 3), tasks=sched["tasks"], waits_spun=sched["waits_spun"],
+                 sync_mode="xcd-local" if b.sync_mode == 1 else "device")
     grads = b.gradients()
     worst = 0.0
     idx = list(np.argsort(ns)[-2:]) + list(rng.choice(nc, size=check - 2, replace=False))
@@ -525,6 +535,7 @@ def bench_persistent(dev, nc=512, seed=20260004, check=6):
                 algorithmic_bytes=alg, achieved_gbps=alg / dt / 1e9, hbm_frac=alg / dt / 8e12,
                 bytes_note="ψ read+write per forward tile pass, ψ and λ per adjoint pass, ψ read + λ written by "
                            "the Hamiltonian step; most circuits' states live in L2/MALL, so this can exceed HBM",
+                scheduler=sched,
                 plan_s=round(plan_s, 1), compile_s=round(SP.STATS["compile_wall_s"] - c0["compile_wall_s"], 2),
                 total_amplitudes=float(sum(2.0 ** int(n) for n in ns)))
 

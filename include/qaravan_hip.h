@@ -206,11 +206,16 @@ int svh_spb_launch(void *handle, int index, const void *items_dev, int nitems, i
  * once per device and grid shape by a small census kernel with a temporary
  * allocation, outside graph capture), otherwise workgroups help other queues
  * with device-scope release/acquire.  *used_mode (optional host int) = 1 for
- * XCD-local, 0 otherwise.  Tile passes must be generic passes with 3 register
- * bits.  Calls that overlap on one set of counters race. */
+ * XCD-local, 0 otherwise.  stats_dev (optional, 5 uint64 on the device, which
+ * the caller zeroes): every workgroup adds its dependency-wait time, its task
+ * time (claim to counter update, waits included), in 100 MHz ticks, its task
+ * count, the number of its waits that had to spin, and 1 (the grid size).  Tile passes must be
+ * generic passes with 3 register bits.  Calls that overlap on one set of
+ * counters race. */
 int svh_batch_persistent(const void *descs_dev, const void *tasks_dev, const int32_t *qbeg_dev, int nq,
                          int32_t *heads_dev, int32_t *counters_dev, int ncounters, int32_t *err_dev, int dtype,
-                         size_t lds_bytes, int max_workgroups, int xcd_local, int *used_mode, void *stream);
+                         size_t lds_bytes, int max_workgroups, int xcd_local, int *used_mode, uint64_t *stats_dev,
+                         void *stream);
 
 /* out_dev[0] = wall clock (s_memrealtime, 100 MHz ticks), for batch latencies. */
 int svh_timestamp(uint64_t *out_dev, void *stream);

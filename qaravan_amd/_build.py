@@ -45,12 +45,29 @@ def _compile(src, objdir):
     return obj
 
 
-def build(verbose=True):
+def _up_to_date(src, objdir):
+    """The object of ``src`` is newer than the source and every header it
+    may include (csrc/*.h and include/*.h)."""
+    obj = os.path.join(objdir, os.path.basename(src)[:-4] + ".o")
+    if not os.path.exists(obj):
+        return False
+    deps = [src] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    inc = os.path.join(HERE, "..", "include")
+    deps += [os.path.join(inc, f) for f in os.listdir(inc) if f.endswith(".h")]
+    return os.path.getmtime(obj) > max(os.path.getmtime(d) for d in deps)
+
+
+def build(verbose=True, incremental=False):
+    """Compile and link the library.  ``incremental``: reuse objects that are
+    newer than their sources and headers (a development shortcut; the default
+    recompiles everything)."""
     objdir = os.path.join(HERE, "_objs")
     os.makedirs(objdir, exist_ok=True)
     srcs = sources()
-    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(lambda s: _compile(s, objdir), srcs))
+    todo = [s for s in srcs if not (incremental and _up_to_date(s, objdir))]
+    with ThreadPoolExecutor(max_workers=max(1, min(8, len(todo)))) as ex:
+        list(ex.map(lambda s: _compile(s, objdir), todo))
+    objs = [os.path.join(objdir, os.path.basename(s)[:-4] + ".o") for s in srcs]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -63,4 +80,4 @@ def build(verbose=True):
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)

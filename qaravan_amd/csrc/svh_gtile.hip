@@ -98,7 +98,8 @@ constexpr int kSpinLimit = 1 << 24;
 
 // Spin with relaxed device-scope loads (no cache maintenance per poll), then
 // one device-scope acquire once the count is reached.
-__device__ __forceinline__ void wait_counter(const int32_t *c, int need, int32_t *err) {
+// Returns whether the count was not yet reached at the first poll.
+__device__ __forceinline__ bool wait_counter(const int32_t *c, int need, int32_t *err) {
     int spins = 0;
     while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
         __builtin_amdgcn_s_sleep(4);
@@ -108,6 +109,7 @@ __device__ __forceinline__ void wait_counter(const int32_t *c, int need, int32_t
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return spins > 0;
 }
 
 // Fixed-order sum of v over the 512 threads (tree in LDS); valid in thread 0.
@@ -174,7 +176,7 @@ template <typename R>
 __global__ void __launch_bounds__(kPersistThreads, sizeof(R) == 4 ? 4 : 2)
 k_batch_persist(const svh_pdesc_t *__restrict__ descs, const svh_ptask_t *__restrict__ tasks,
                 const int32_t *__restrict__ qbeg, int nq, int32_t *heads, int32_t *ctr, int32_t *err,
-                int xcd_local) {
+                int xcd_local, unsigned long long *stats) {
     extern __shared__ unsigned char smem_raw[];
     __shared__ int s_task;
     cpx<R> *lds = reinterpret_cast<cpx<R> *>(smem_raw);
@@ -186,6 +188,9 @@ k_batch_persist(const svh_pdesc_t *__restrict__ descs, const svh_ptask_t *__rest
     // across XCDs), so every dependency stays inside one L2
     int q = xcc % nq, drained = (xcd_local && xcc >= nq) ? nq : 0;  // used by thread 0
     const int reach = xcd_local ? 1 : nq;
+    // thread 0's clock accounting (100 MHz ticks): dependency waits, everything
+    // from a task's claim to its counter update, tasks, waits that had to spin
+    unsigned long long t_wait = 0, t_task = 0, n_task = 0, n_spun = 0;
     for (;;) {
         if (threadIdx.x == 0) {
             int t = -1;
@@ -206,8 +211,16 @@ k_batch_persist(const svh_pdesc_t *__restrict__ descs, const svh_ptask_t *__rest
         if (t < 0) break;
         const svh_ptask_t T = tasks[t];
         const svh_pdesc_t &D = descs[T.desc];
+        const unsigned long long t0 = stats && threadIdx.x == 0 ? wall_clock64() : 0;
         if (D.dep >= 0) {
-            if (threadIdx.x == 0) wait_counter(ctr + D.dep, D.dep_need, err);
+            if (threadIdx.x == 0) {
+                const bool spun = wait_counter(ctr + D.dep, D.dep_need, err);
+                if (stats) {
+                    const unsigned long long t1 = wall_clock64();
+                    t_wait += t1 - t0;
+                    n_spun += spun ? 1 : 0;
+                }
+            }
             __syncthreads();
         }
         for (int u = T.first; u < T.first + T.count; ++u) {
@@ -256,7 +269,18 @@ k_batch_persist(const svh_pdesc_t *__restrict__ descs, const svh_ptask_t *__rest
             } else if (D.tstamp) {
                 *reinterpret_cast<uint64_t *>(D.tstamp) = wall_clock64();
             }
+            if (stats) {
+                t_task += wall_clock64() - t0;
+                ++n_task;
+            }
         }
+    }
+    if (stats && threadIdx.x == 0) {
+        atomicAdd(stats + 0, t_wait);
+        atomicAdd(stats + 1, t_task);
+        atomicAdd(stats + 2, n_task);
+        atomicAdd(stats + 3, n_spun);
+        atomicAdd(stats + 4, 1ull);
     }
 }
 
@@ -337,7 +361,7 @@ using namespace svh;
 extern "C" int svh_batch_persistent(const void *descs_dev, const void *tasks_dev, const int32_t *qbeg_dev, int nq,
                                     int32_t *heads_dev, int32_t *counters_dev, int ncounters, int32_t *err_dev,
                                     int dtype, size_t lds_bytes, int max_workgroups, int xcd_local, int *used_mode,
-                                    void *stream) {
+                                    uint64_t *stats_dev, void *stream) {
     if (!descs_dev || !tasks_dev || !qbeg_dev || !heads_dev || !err_dev || (ncounters > 0 && !counters_dev))
         return fail(-1, "NULL pointer");
     if (nq < 1 || ncounters < 0) return fail(-4, "bad queue or counter count");
@@ -363,9 +387,9 @@ extern "C" int svh_batch_persistent(const void *descs_dev, const void *tasks_dev
     const auto *T = static_cast<const svh_ptask_t *>(tasks_dev);
     if (dtype == SVH_C64)
         k_batch_persist<float><<<grid, kPersistThreads, need, st>>>(D, T, qbeg_dev, nq, heads_dev, counters_dev, err_dev,
-                                                                    mode);
+                                                                    mode, (unsigned long long *)stats_dev);
     else
         k_batch_persist<double><<<grid, kPersistThreads, need, st>>>(D, T, qbeg_dev, nq, heads_dev, counters_dev,
-                                                                     err_dev, mode);
+                                                                     err_dev, mode, (unsigned long long *)stats_dev);
     return checked_launch("k_batch_persist");
 }

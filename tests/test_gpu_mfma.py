@@ -45,6 +45,8 @@ def test_block5_direct_vs_oracle(gpu, bits):
     (13, (8, 9, 10, 11, 12), (3, 4, 5, 6, 7)),        # three spare bits, a single 8192-amplitude tile
     (17, (3, 9, 1, 13, 6), (16, 0, 11, 2, 8)),        # scrambled bits
     (12, (11, 10, 9, 8, 7), (6, 5, 4, 3, 2)),         # single tile
+    (24, (10, 11, 12, 13, 14), (15, 16, 17, 18, 19)),  # 8 tiles per workgroup: the prefetch pipeline
+    (23, (3, 4, 5, 6, 7), (18, 19, 20, 21, 22)),      # 4 tiles per workgroup, three spare bits
 ])
 def test_block5x2_direct_vs_oracle(gpu, n, ba, bb):
     from qaravan_amd import _lib
