@@ -130,7 +130,7 @@ typedef struct {
     int32_t param;       /* θ index (ROT*)                                           */
     int32_t pauli;       /* 1 X, 2 Y, 3 Z (ROT*)                                     */
     int32_t flags;       /* SVH_OPF_*                                                */
-    int32_t grad_slot;   /* pass-local gradient slot (SVH_OPF_GRAD)                  */
+    int32_t grad_slot;   /* pass-local gradient slot (SVH_OPF_GRAD); consecutive within a stage */
     int32_t pad;
     double m[32];        /* complex row-major matrix (re, im pairs)                  */
     float mf[32];        /* the same matrix rounded to float (used by SVH_C64)       */

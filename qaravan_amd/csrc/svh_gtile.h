@@ -42,6 +42,16 @@ __device__ __forceinline__ void gd_rot2(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << R
     ap_rot2<R, RB, H, L>(v, PAULI, cs, sn);
 }
 
+// The generic kernels' group store: 2·(wave sum) of the first `count` of the
+// 2^K gradient terms into the wave's partial row (gout[slot0 + j]).
+template <int K>
+__device__ __forceinline__ void gd_grad_group(double (&g)[1 << K], double *gout, int slot0, int count) {
+    const double s = wave_sum_many<K, 64>(g);
+    const int lane = threadIdx.x & 63;
+    const int j = rev_low_bits<K>(lane);
+    if (lane < (1 << K) && j < count) gout[slot0 + j] = 2.0 * s;
+}
+
 }  // namespace svh
 
 #include "svh_patterns.h"

@@ -682,885 +682,1452 @@ template <typename R, int RB>
 __device__ __forceinline__ void pat3d_0(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[5], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[5], trig, gout, lane0);
+    { double gv[4] = {gq2, gq3, gq4, gq5}; gd_grad_group<2>(gv, gout, ops[2].grad_slot, 4); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_1(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[6], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[6], trig, gout, lane0);
+    { double gv[8] = {gq2, gq3, gq4, gq5, gq6, 0.0, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[2].grad_slot, 5); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_2(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[5], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[5], trig, gout, lane0);
+    { double gv[4] = {gq2, gq3, gq4, gq5}; gd_grad_group<2>(gv, gout, ops[2].grad_slot, 4); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_3(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[5], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[5], trig, gout, lane0);
+    { double gv[4] = {gq2, gq3, gq4, gq5}; gd_grad_group<2>(gv, gout, ops[2].grad_slot, 4); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_4(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[3], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[3], trig, gout, lane0);
+    { double gv[4] = {gq1, gq2, gq3, 0.0}; gd_grad_group<2>(gv, gout, ops[1].grad_slot, 3); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_5(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    { double gv[8] = {gq2, gq3, gq4, gq5, gq6, 0.0, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[2].grad_slot, 5); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_6(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[6], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[6], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[9], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[10], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[9], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[10], trig, gout, lane0);
+    { double gv[8] = {gq2, gq3, gq4, gq5, gq6, gq8, gq9, gq10}; gd_grad_group<3>(gv, gout, ops[2].grad_slot, 8); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_7(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[5], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[5], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[9], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[11], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[9], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq2, gq3, gq4, gq5, gq8, gq9, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[2].grad_slot, 8); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_8(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[5], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[5], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[8], trig, gout, lane0);
+    { double gv[8] = {gq2, gq3, gq4, gq5, gq7, gq8, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[2].grad_slot, 6); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_9(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[3], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[3], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[9], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[10], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[9], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[10], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq3, gq6, gq7, gq8, gq9, gq10}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_10(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
+    { double gv[2] = {gq1, gq2}; gd_grad_group<1>(gv, gout, ops[1].grad_slot, 2); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_11(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[4], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[4], trig, gout, lane0);
+    { double gv[4] = {gq2, gq3, gq4, 0.0}; gd_grad_group<2>(gv, gout, ops[2].grad_slot, 3); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_12(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[6], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[6], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[9], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[11], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[9], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[10], trig, gout, lane0);
+    { double gv[8] = {gq2, gq3, gq4, gq5, gq6, gq8, gq9, gq10}; gd_grad_group<3>(gv, gout, ops[2].grad_slot, 8); }
+    const double gq11 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[1] = {gq11}; gd_grad_group<0>(gv, gout, ops[11].grad_slot, 1); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_13(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[5], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[5], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[9], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[11], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[12], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[9], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq2, gq3, gq4, gq5, gq8, gq9, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[2].grad_slot, 8); }
+    const double gq12 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[12], trig, gout, lane0);
+    { double gv[1] = {gq12}; gd_grad_group<0>(gv, gout, ops[12].grad_slot, 1); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_14(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[3], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[3], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[9], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[10], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[9], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[10], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq3, gq6, gq7, gq8, gq9, gq10}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[13], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[14], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[15], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[16], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[17], trig, gout, lane0);
+    const double gq13 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[13], trig, gout, lane0);
+    const double gq14 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[14], trig, gout, lane0);
+    const double gq15 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[15], trig, gout, lane0);
+    const double gq16 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[16], trig, gout, lane0);
+    const double gq17 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[17], trig, gout, lane0);
+    { double gv[8] = {gq13, gq14, gq15, gq16, gq17, 0.0, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[13].grad_slot, 5); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_15(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[8], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 6); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_16(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[4], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[4], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[9], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[10], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[9], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[10], trig, gout, lane0);
+    { double gv[8] = {gq2, gq3, gq4, gq7, gq8, gq9, gq10, 0.0}; gd_grad_group<3>(gv, gout, ops[2].grad_slot, 7); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_17(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[3], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[3], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[9], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[10], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[9], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[10], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq3, gq6, gq7, gq8, gq9, gq10}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[13], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[14], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[15], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[16], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[17], trig, gout, lane0);
+    const double gq13 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[13], trig, gout, lane0);
+    const double gq14 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[14], trig, gout, lane0);
+    const double gq15 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[15], trig, gout, lane0);
+    const double gq16 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[16], trig, gout, lane0);
+    const double gq17 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[17], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[19], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[20], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[21], trig, gout, lane0);
+    const double gq19 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[19], trig, gout, lane0);
+    const double gq20 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[20], trig, gout, lane0);
+    const double gq21 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[21], trig, gout, lane0);
+    { double gv[8] = {gq13, gq14, gq15, gq16, gq17, gq19, gq20, gq21}; gd_grad_group<3>(gv, gout, ops[13].grad_slot, 8); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_18(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[8], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[11], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[12], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
+    const double gq12 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[12], trig, gout, lane0);
+    { double gv[1] = {gq12}; gd_grad_group<0>(gv, gout, ops[12].grad_slot, 1); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_19(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[4], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[4], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[9], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[11], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[9], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq2, gq3, gq4, gq7, gq8, gq9, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[2].grad_slot, 8); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_20(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[7], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[7], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, 0.0, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 5); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_21(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[3], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[3], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[9], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[10], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[9], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[10], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq3, gq6, gq7, gq8, gq9, gq10}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[13], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[14], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[15], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[16], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[17], trig, gout, lane0);
+    const double gq13 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[13], trig, gout, lane0);
+    const double gq14 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[14], trig, gout, lane0);
+    const double gq15 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[15], trig, gout, lane0);
+    const double gq16 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[16], trig, gout, lane0);
+    const double gq17 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[17], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[19], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[20], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[21], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[22], trig, gout, lane0);
+    const double gq19 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[19], trig, gout, lane0);
+    const double gq20 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[20], trig, gout, lane0);
+    const double gq21 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[21], trig, gout, lane0);
+    { double gv[8] = {gq13, gq14, gq15, gq16, gq17, gq19, gq20, gq21}; gd_grad_group<3>(gv, gout, ops[13].grad_slot, 8); }
+    const double gq22 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[22], trig, gout, lane0);
+    { double gv[1] = {gq22}; gd_grad_group<0>(gv, gout, ops[22].grad_slot, 1); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_22(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[8], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[11], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[12], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[13], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
+    const double gq12 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[12], trig, gout, lane0);
+    const double gq13 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[13], trig, gout, lane0);
+    { double gv[2] = {gq12, gq13}; gd_grad_group<1>(gv, gout, ops[12].grad_slot, 2); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_23(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[8], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 6); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_24(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[8], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 6); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_25(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[8], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[11], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_26(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[8], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[11], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[12], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
+    const double gq12 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[12], trig, gout, lane0);
+    { double gv[1] = {gq12}; gd_grad_group<0>(gv, gout, ops[12].grad_slot, 1); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_27(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[8], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[11], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[12], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[13], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
+    const double gq12 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[12], trig, gout, lane0);
+    const double gq13 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[13], trig, gout, lane0);
+    { double gv[2] = {gq12, gq13}; gd_grad_group<1>(gv, gout, ops[12].grad_slot, 2); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_28(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[9], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[9], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq9, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 7); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_29(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[9], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[9], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[12], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[13], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[14], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[15], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[16], trig, gout, lane0);
+    const double gq12 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[12], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq9, gq12}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
+    const double gq13 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[13], trig, gout, lane0);
+    const double gq14 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[14], trig, gout, lane0);
+    const double gq15 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[15], trig, gout, lane0);
+    const double gq16 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[16], trig, gout, lane0);
+    { double gv[4] = {gq13, gq14, gq15, gq16}; gd_grad_group<2>(gv, gout, ops[13].grad_slot, 4); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_30(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[9], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[9], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[12], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[13], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[14], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[15], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[16], trig, gout, lane0);
+    const double gq12 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[12], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq9, gq12}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
+    const double gq13 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[13], trig, gout, lane0);
+    const double gq14 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[14], trig, gout, lane0);
+    const double gq15 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[15], trig, gout, lane0);
+    const double gq16 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[16], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[18], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[19], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[20], trig, gout, lane0);
+    const double gq18 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[18], trig, gout, lane0);
+    const double gq19 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[19], trig, gout, lane0);
+    const double gq20 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[20], trig, gout, lane0);
+    { double gv[8] = {gq13, gq14, gq15, gq16, gq18, gq19, gq20, 0.0}; gd_grad_group<3>(gv, gout, ops[13].grad_slot, 7); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_31(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[9], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[9], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[12], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[13], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[14], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[15], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[16], trig, gout, lane0);
+    const double gq12 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[12], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq9, gq12}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
+    const double gq13 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[13], trig, gout, lane0);
+    const double gq14 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[14], trig, gout, lane0);
+    const double gq15 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[15], trig, gout, lane0);
+    const double gq16 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[16], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[18], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[19], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[20], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[21], trig, gout, lane0);
+    const double gq18 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[18], trig, gout, lane0);
+    const double gq19 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[19], trig, gout, lane0);
+    const double gq20 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[20], trig, gout, lane0);
+    const double gq21 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[21], trig, gout, lane0);
+    { double gv[8] = {gq13, gq14, gq15, gq16, gq18, gq19, gq20, gq21}; gd_grad_group<3>(gv, gout, ops[13].grad_slot, 8); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_32(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[9], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[9], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq9, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 7); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_33(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[9], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[9], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[12], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[13], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[14], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[15], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[16], trig, gout, lane0);
+    const double gq12 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[12], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq9, gq12}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
+    const double gq13 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[13], trig, gout, lane0);
+    const double gq14 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[14], trig, gout, lane0);
+    const double gq15 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[15], trig, gout, lane0);
+    const double gq16 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[16], trig, gout, lane0);
+    { double gv[4] = {gq13, gq14, gq15, gq16}; gd_grad_group<2>(gv, gout, ops[13].grad_slot, 4); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_34(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[9], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[9], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[12], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[13], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[14], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[15], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[16], trig, gout, lane0);
+    const double gq12 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[12], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq9, gq12}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
+    const double gq13 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[13], trig, gout, lane0);
+    const double gq14 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[14], trig, gout, lane0);
+    const double gq15 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[15], trig, gout, lane0);
+    const double gq16 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[16], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[18], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[19], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[20], trig, gout, lane0);
+    const double gq18 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[18], trig, gout, lane0);
+    const double gq19 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[19], trig, gout, lane0);
+    const double gq20 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[20], trig, gout, lane0);
+    { double gv[8] = {gq13, gq14, gq15, gq16, gq18, gq19, gq20, 0.0}; gd_grad_group<3>(gv, gout, ops[13].grad_slot, 7); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_35(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[8], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[9], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[8], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[9], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[12], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[13], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[14], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[15], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[16], trig, gout, lane0);
+    const double gq12 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[12], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq9, gq12}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
+    const double gq13 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[13], trig, gout, lane0);
+    const double gq14 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[14], trig, gout, lane0);
+    const double gq15 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[15], trig, gout, lane0);
+    const double gq16 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[16], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[18], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[19], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[20], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[21], trig, gout, lane0);
+    const double gq18 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[18], trig, gout, lane0);
+    const double gq19 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[19], trig, gout, lane0);
+    const double gq20 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[20], trig, gout, lane0);
+    const double gq21 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[21], trig, gout, lane0);
+    { double gv[8] = {gq13, gq14, gq15, gq16, gq18, gq19, gq20, gq21}; gd_grad_group<3>(gv, gout, ops[13].grad_slot, 8); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_36(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[6], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[6], trig, gout, lane0);
+    { double gv[8] = {gq2, gq3, gq4, gq5, gq6, 0.0, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[2].grad_slot, 5); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_37(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[6], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[6], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[9], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[11], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[12], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[13], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[9], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq2, gq3, gq4, gq5, gq6, gq9, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[2].grad_slot, 8); }
+    const double gq12 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[12], trig, gout, lane0);
+    const double gq13 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[13], trig, gout, lane0);
+    { double gv[2] = {gq12, gq13}; gd_grad_group<1>(gv, gout, ops[12].grad_slot, 2); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_38(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[6], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[6], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[9], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[11], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[12], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[13], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[9], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq2, gq3, gq4, gq5, gq6, gq9, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[2].grad_slot, 8); }
+    const double gq12 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[12], trig, gout, lane0);
+    const double gq13 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[13], trig, gout, lane0);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[15], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[16], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[17], trig, gout, lane0);
+    const double gq15 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[15], trig, gout, lane0);
+    const double gq16 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[16], trig, gout, lane0);
+    const double gq17 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[17], trig, gout, lane0);
+    { double gv[8] = {gq12, gq13, gq15, gq16, gq17, 0.0, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[12].grad_slot, 5); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_39(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[2], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[6], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[6], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[9], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[11], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[12], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[13], trig, gout, lane0);
+    const double gq9 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[9], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq2, gq3, gq4, gq5, gq6, gq9, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[2].grad_slot, 8); }
+    const double gq12 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[12], trig, gout, lane0);
+    const double gq13 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[13], trig, gout, lane0);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[15], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[16], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[17], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[18], trig, gout, lane0);
+    const double gq15 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[15], trig, gout, lane0);
+    const double gq16 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[16], trig, gout, lane0);
+    const double gq17 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[17], trig, gout, lane0);
+    const double gq18 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[18], trig, gout, lane0);
+    { double gv[8] = {gq12, gq13, gq15, gq16, gq17, gq18, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[12].grad_slot, 6); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_40(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[2], trig, gout, lane0);
+    { double gv[2] = {gq1, gq2}; gd_grad_group<1>(gv, gout, ops[1].grad_slot, 2); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_41(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[5], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[5], trig, gout, lane0);
+    { double gv[4] = {gq1, gq2, gq4, gq5}; gd_grad_group<2>(gv, gout, ops[1].grad_slot, 4); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_42(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[6], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[6], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq4, gq5, gq6, 0.0, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 5); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_43(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    { double gv[1] = {gq1}; gd_grad_group<0>(gv, gout, ops[1].grad_slot, 1); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_44(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[4], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[4], trig, gout, lane0);
+    { double gv[4] = {gq1, gq3, gq4, 0.0}; gd_grad_group<2>(gv, gout, ops[1].grad_slot, 3); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_45(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[5], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[5], trig, gout, lane0);
+    { double gv[4] = {gq1, gq3, gq4, gq5}; gd_grad_group<2>(gv, gout, ops[1].grad_slot, 4); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_46(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
     ap_cz<R, RB, 2, 1>(v); ap_cz<R, RB, 2, 1>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[3], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[3], trig, gout, lane0);
+    { double gv[2] = {gq1, gq3}; gd_grad_group<1>(gv, gout, ops[1].grad_slot, 2); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_47(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[8], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 6); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_48(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[8], trig, gout, lane0);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[11], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_49(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[8], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 6); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_50(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[8], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[11], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[12], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
+    const double gq12 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[12], trig, gout, lane0);
+    { double gv[1] = {gq12}; gd_grad_group<0>(gv, gout, ops[12].grad_slot, 1); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_51(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[8], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[11], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[12], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[13], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
+    const double gq12 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[12], trig, gout, lane0);
+    const double gq13 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[13], trig, gout, lane0);
+    { double gv[2] = {gq12, gq13}; gd_grad_group<1>(gv, gout, ops[12].grad_slot, 2); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_52(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[6], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[6], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq4, gq5, gq6, 0.0, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 5); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_53(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[7], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[7], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq4, gq5, gq6, gq7, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 6); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_54(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[6], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[6], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq4, gq5, gq6, 0.0, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 5); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_55(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[7], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[7], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq4, gq5, gq6, gq7, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 6); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_56(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[5], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[5], trig, gout, lane0);
+    { double gv[4] = {gq1, gq2, gq4, gq5}; gd_grad_group<2>(gv, gout, ops[1].grad_slot, 4); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_57(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[8], trig, gout, lane0);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[11], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[12], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
+    const double gq12 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[12], trig, gout, lane0);
+    { double gv[1] = {gq12}; gd_grad_group<0>(gv, gout, ops[12].grad_slot, 1); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_58(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[8], trig, gout, lane0);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[10], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[11], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[12], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[13], trig, gout, lane0);
+    const double gq10 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[10], trig, gout, lane0);
+    const double gq11 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[11], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, gq10, gq11}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 8); }
+    const double gq12 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[12], trig, gout, lane0);
+    const double gq13 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[13], trig, gout, lane0);
+    { double gv[2] = {gq12, gq13}; gd_grad_group<1>(gv, gout, ops[12].grad_slot, 2); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_59(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[1], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[1], trig, gout, lane0);
+    { double gv[1] = {gq1}; gd_grad_group<0>(gv, gout, ops[1].grad_slot, 1); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_60(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[1], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[1], trig, gout, lane0);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[4], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[4], trig, gout, lane0);
+    { double gv[4] = {gq1, gq3, gq4, 0.0}; gd_grad_group<2>(gv, gout, ops[1].grad_slot, 3); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_61(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[1], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[1], trig, gout, lane0);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[3], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 2, true>(v, w, ops[4], trig, gout, lane0);
-    gd_rot1<R, RB, 2, 3, true>(v, w, ops[5], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 2>(v, w, 2);
+    gd_rot1<R, RB, 2, 2, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[5], trig, gout, lane0);
+    { double gv[4] = {gq1, gq3, gq4, gq5}; gd_grad_group<2>(gv, gout, ops[1].grad_slot, 4); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_62(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[7], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[7], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, 0.0, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 5); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_63(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[2], trig, gout, lane0);
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
     ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[5], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[6], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[7], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[8], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    const double gq6 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[7], trig, gout, lane0);
+    const double gq8 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[8], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq5, gq6, gq7, gq8, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 6); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_64(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 1, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 1, 3, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[2], trig, gout, lane0);
+    { double gv[2] = {gq1, gq2}; gd_grad_group<1>(gv, gout, ops[1].grad_slot, 2); }
 }
 template <typename R, int RB>
 __device__ __forceinline__ void pat3d_65(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
     ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
-    gd_rot1<R, RB, 0, 2, true>(v, w, ops[1], trig, gout, lane0);
-    gd_rot1<R, RB, 0, 3, true>(v, w, ops[2], trig, gout, lane0);
+    const double gq1 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[2], trig, gout, lane0);
+    { double gv[2] = {gq1, gq2}; gd_grad_group<1>(gv, gout, ops[1].grad_slot, 2); }
 }
 // pattern 3d: returns false when pid is not in the library
 template <typename R, int RB>

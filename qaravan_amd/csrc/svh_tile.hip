@@ -484,8 +484,15 @@ static int check_program(const void *prog_host, int n, const svh_prog_header_t *
                     (s >= P.stage_begin + P.n_stages ||
                      !gtile_pattern_matches(P.rb, mode == 1, S.pattern, ops + S.op_begin, S.op_end - S.op_begin)))
                     return fail(-8, "stage " + std::to_string(s) + ": pattern id does not match its op sequence");
+                int last_slot = -1;  // gradient terms of a stage are stored as one group
                 for (int o = S.op_begin; o < S.op_end; ++o) {
                     const svh_op_t &O = ops[o];
+                    if (O.flags & SVH_OPF_GRAD) {
+                        if (last_slot >= 0 && O.grad_slot != last_slot + 1)
+                            return fail(-8, "op " + std::to_string(o) +
+                                                ": the gradient slots of a stage must be consecutive");
+                        last_slot = O.grad_slot;
+                    }
                     const bool two = O.kind == SVH_OP_MAT2 || O.kind == SVH_OP_CX ||
                                      O.kind == SVH_OP_CZ || O.kind == SVH_OP_ROT2;
                     if (O.kind < SVH_OP_MAT1 || O.kind > SVH_OP_ROT2)

@@ -211,6 +211,90 @@ __device__ __forceinline__ double wave_sum(double v) {
                             __builtin_amdgcn_readlane(__double2loint(v), W - 1));
 }
 
+// ---- several gradient terms of one stage summed over the wave together
+// Lane L's partner across lane bit MASK (L ^ MASK), through the DPP row
+// network for bits 0, 1 and 3 and a lane permute for bits 4 and 5.
+template <int MASK>
+__device__ __forceinline__ double lane_xor(double x) {
+    if constexpr (MASK == 1) return __builtin_amdgcn_update_dpp(0.0, x, 0xb1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
+    else if constexpr (MASK == 2) return __builtin_amdgcn_update_dpp(0.0, x, 0x4e, 0xf, 0xf, false);  // quad_perm [2,3,0,1]
+    else if constexpr (MASK == 8) return __builtin_amdgcn_update_dpp(0.0, x, 0x128, 0xf, 0xf, false);  // row_ror:8
+    else return __shfl_xor(x, MASK, 64);
+}
+
+// One transposed step over lane bit B: M values per lane become M/2.  A lane
+// whose bit B is 0 keeps the lower half of the values and gives its partner
+// (lane ^ 2^B) the upper half; a lane whose bit is 1 does the opposite; each
+// adds the partner's copy of the half it keeps.  Bit 2 has no single DPP
+// partner pattern: row_ror:4 reaches L + 4 and row_ror:12 reaches L - 4 inside
+// the row of 16, and the lane takes the one that is L ^ 4.
+template <int B, int M>
+__device__ __forceinline__ void xpose_step(double *v, int lane) {
+    const bool up = (lane >> B) & 1;
+#pragma unroll
+    for (int j = 0; j < M / 2; ++j) {
+        const double keep = up ? v[j + M / 2] : v[j];
+        const double send = up ? v[j] : v[j + M / 2];
+        double recv;
+        if constexpr (B == 2) {
+            const double fwd = __builtin_amdgcn_update_dpp(0.0, send, 0x124, 0xf, 0xf, false);  // row_ror:4
+            const double bwd = __builtin_amdgcn_update_dpp(0.0, send, 0x12c, 0xf, 0xf, false);  // row_ror:12
+            recv = up ? bwd : fwd;
+        } else {
+            recv = lane_xor<1 << B>(send);
+        }
+        v[j] = keep + recv;
+    }
+}
+
+// Sums each of the 2^K values (K <= 3) of every lane over the W lanes of its
+// wave (W = 16, 32 or 64: the workgroup when it is smaller than a wave).
+// Transposed steps over lane bits 0..K-1 (2^K - 1 exchanges instead of 6·2^K),
+// then plain sums over the other lane bits.  On return every lane holds the
+// wave total of value rev_K(lane mod 2^K) (its low K lane bits reversed).
+// Every sum is FP64 in a fixed order, so results do not vary from run to run.
+template <int K, int W>
+__device__ __forceinline__ double wave_sum_many(double (&v)[1 << K]) {
+    static_assert(K >= 0 && K <= 3, "wave_sum_many: at most 8 values");
+    static_assert(W == 16 || W == 32 || W == 64, "wave_sum_many: 16, 32 or 64 lanes");
+    const int lane = threadIdx.x & 63;
+    if constexpr (K >= 1) xpose_step<0, (1 << K)>(v, lane);
+    if constexpr (K >= 2) xpose_step<1, (1 << K) / 2>(v, lane);
+    if constexpr (K >= 3) xpose_step<2, 2>(v, lane);
+    double s = v[0];
+    if constexpr (K < 1) s += lane_xor<1>(s);
+    if constexpr (K < 2) s += lane_xor<2>(s);
+    if constexpr (K < 3) {
+        // bits 2 and 3 together: lanes L, L + 4, L + 8, L + 12 of the row
+        s += __builtin_amdgcn_update_dpp(0.0, s, 0x124, 0xf, 0xf, false);  // row_ror:4
+        s += __builtin_amdgcn_update_dpp(0.0, s, 0x128, 0xf, 0xf, false);  // row_ror:8
+    } else {
+        s += lane_xor<8>(s);
+    }
+    if constexpr (W >= 32) s += lane_xor<16>(s);
+    if constexpr (W == 64) s += lane_xor<32>(s);
+    return s;
+}
+
+// Index of the value a lane holds after wave_sum_many<K>: its low K bits reversed.
+template <int K>
+__device__ __forceinline__ int rev_low_bits(int lane) {
+    int j = 0;
+#pragma unroll
+    for (int t = 0; t < K; ++t) j |= ((lane >> t) & 1) << (K - 1 - t);
+    return j;
+}
+
+// Store 2·(wave sum) of the first `count` of the 2^K gradient terms g into
+// gacc[wave][slot0 + j] (LDS, one row of ng slots per wave).
+template <int K, int W>
+__device__ __forceinline__ void sp_grad_group(double (&g)[1 << K], double *gacc, int ng, int slot0, int count) {
+    const double s = wave_sum_many<K, W>(g);
+    const int lane = threadIdx.x & 63;
+    const int j = rev_low_bits<K>(lane);
+    if (lane < (1 << K) && j < count) gacc[(threadIdx.x >> 6) * ng + slot0 + j] = 2.0 * s;
+}
+
 template <typename R>
 __device__ __forceinline__ cpx<R> ld_m(const svh_op_t &op, int i) {
     if constexpr (std::is_same<R, float>::value) return cpx<R>{op.mf[2 * i], op.mf[2 * i + 1]};

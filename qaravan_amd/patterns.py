@@ -87,13 +87,41 @@ def _op_line(k, op, dual):
         return f"    {one}(v);" + (f" {one}(w);" if dual else "")
     if kind == P.OP_ROT1:
         if dual:
-            return f"    gd_rot1<R, RB, {a}, {pauli}, {g}>(v, w, ops[{k}], trig, gout, lane0);"
+            # the gradient term is taken by the caller (see _pattern_lines)
+            pre = f"    const double gq{k} = gr_rot1<R, RB, {a}>(v, w, {pauli});\n" if grad else ""
+            return pre + f"    gd_rot1<R, RB, {a}, {pauli}, false>(v, w, ops[{k}], trig, gout, lane0);"
         return f"    sp_rot1<R, {a}, {pauli}, RB>(v, ops[{k}], trig);"
     if kind == P.OP_ROT2:
         if dual:
-            return f"    gd_rot2<R, RB, {hi}, {lo}, {pauli}, {g}>(v, w, ops[{k}], trig, gout, lane0);"
+            pre = f"    const double gq{k} = gr_rot2<R, RB, {hi}, {lo}>(v, w, {pauli});\n" if grad else ""
+            return pre + f"    gd_rot2<R, RB, {hi}, {lo}, {pauli}, false>(v, w, ops[{k}], trig, gout, lane0);"
         return f"    sp_rot2<R, {a}, {b}, {pauli}, RB>(v, ops[{k}], trig);"
     raise ValueError(kind)
+
+
+def _pattern_lines(key, dual):
+    """Body of one pattern.  In an adjoint stage each gradient term is
+    computed where its rotation is undone and kept in a register; the terms
+    (consecutive slots) are then summed over the wave together, up to 8 at a
+    time, and stored to the wave's partial row."""
+    lines, grads = [], []
+
+    def flush():
+        if grads:
+            cnt = len(grads)
+            K = (cnt - 1).bit_length()
+            vals = ", ".join([f"gq{k}" for k in grads] + ["0.0"] * ((1 << K) - cnt))
+            lines.append(f"    {{ double gv[{1 << K}] = {{{vals}}}; "
+                         f"gd_grad_group<{K}>(gv, gout, ops[{grads[0]}].grad_slot, {cnt}); }}")
+            grads.clear()
+    for k, op in enumerate(key):
+        lines.append(_op_line(k, op, dual))
+        if dual and op[4] and op[0] in (P.OP_ROT1, P.OP_ROT2):
+            grads.append(k)
+            if len(grads) == 8:
+                flush()
+    flush()
+    return lines
 
 
 def generate(keys_by_lib):
@@ -108,7 +136,7 @@ def generate(keys_by_lib):
                "cpx<R> (&v)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig"
         for i, key in enumerate(keys):
             out.append(f"template <typename R, int RB>\n__device__ __forceinline__ void pat{tag}_{i}({args}) {{")
-            out += [_op_line(k, op, dual) for k, op in enumerate(key)]
+            out += _pattern_lines(key, dual)
             out.append("}")
         call = "v, w, ops, trig, gout, lane0" if dual else "v, ops, trig"
         out.append(f"// pattern {tag}: returns false when pid is not in the library")

@@ -155,6 +155,24 @@ def _body(R, p, stages, ops, dual, store_to="psi"):
         out += [f"    cpx<{R}> v[{1 << rb}];",
                 f"    const uint64_t base = sp_base_wg<{R}, {tb}, {rb}>(pass, n, wg);",
                 f"    sp_load<{R}, {tb}, {rb}>(psi, base, st[0], v);"]
+    W = min(1 << tb, 64)
+    grads = []          # (variable, op index) of this stage's gradient terms not yet stored
+
+    def flush():
+        # the stage's gradient terms (consecutive slots) summed over the wave
+        # together, up to 8 at a time
+        if not grads:
+            return
+        cnt = len(grads)
+        slots = [int(ops[op0 + k]['grad_slot']) for _, k in grads]
+        if slots != list(range(slots[0], slots[0] + cnt)):
+            raise ValueError(f"gradient slots of a stage are not consecutive: {slots}")
+        K = (cnt - 1).bit_length()
+        vals = ", ".join([g for g, _ in grads] + ["0.0"] * ((1 << K) - cnt))
+        out.append(f"    {{ double gv[{1 << K}] = {{{vals}}}; "
+                   f"sp_grad_group<{K}, {W}>(gv, gacc, {ng}, ops[{grads[0][1]}].grad_slot, {cnt}); }}")
+        grads.clear()
+
     for s in range(ns):
         S = stages[s0 + s]
         if s > 0:
@@ -182,11 +200,19 @@ def _body(R, p, stages, ops, dual, store_to="psi"):
                     out.append(f"    ap_cx_t<{R}, {rb}, {hi}, {lo}, {th}>(v); ap_cx_t<{R}, {rb}, {hi}, {lo}, {th}>(w);")
                 elif kind == P.OP_CZ:
                     out.append(f"    ap_cz<{R}, {rb}, {hi}, {lo}>(v); ap_cz<{R}, {rb}, {hi}, {lo}>(w);")
-                elif kind == P.OP_ROT1:
-                    out.append(f"    spd_rot1<{R}, {tb}, {a}, {pauli}, {g}, {rb}>(v, w, ops[{k}], trig, gacc, {ng});")
-                elif kind == P.OP_ROT2:
-                    out.append(f"    spd_rot2<{R}, {tb}, {a}, {b}, {pauli}, {g}, {rb}>(v, w, ops[{k}], trig, gacc, "
-                               f"{ng});")
+                elif kind in (P.OP_ROT1, P.OP_ROT2):
+                    if g == "true":
+                        var = f"gq{oi - op0}"
+                        gr = (f"gr_rot1<{R}, {rb}, {a}>" if kind == P.OP_ROT1 else f"gr_rot2<{R}, {rb}, {a}, {b}>")
+                        out.append(f"    const double {var} = {gr}(v, w, {pauli});")
+                        grads.append((var, k))
+                    if kind == P.OP_ROT1:
+                        out.append(f"    spd_rot1<{R}, {tb}, {a}, {pauli}, false, {rb}>(v, w, ops[{k}], trig, gacc, {ng});")
+                    else:
+                        out.append(f"    spd_rot2<{R}, {tb}, {a}, {b}, {pauli}, false, {rb}>(v, w, ops[{k}], trig, "
+                                   f"gacc, {ng});")
+                    if len(grads) == 8:
+                        flush()
                 else:
                     raise ValueError(kind)
             else:
@@ -205,6 +231,7 @@ def _body(R, p, stages, ops, dual, store_to="psi"):
                     out.append(f"    sp_rot2<{R}, {a}, {b}, {pauli}, {rb}>(v, ops[{k}], trig);")
                 else:
                     raise ValueError(kind)
+        flush()
     if dual:
         remap = int(p['flags']) & P.PASSF_REMAP_STORE
         if remap and ns == 1:
