@@ -56,6 +56,8 @@ def parse():
                     help="only the timed forward leg (for rocprofv3 runs)")
     ap.add_argument("--profile-blocks", action="store_true", help="only the MFMA block leg")
     ap.add_argument("--profile-adjoint", action="store_true", help="only 20 adjoint gradients (no torch tapes)")
+    ap.add_argument("--profile-batch", action="store_true",
+                    help="only the 512-circuit persistent batch (3 launches, no checks)")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only a 2 GiB device copy (calibrates the HBM byte counters)")
     ap.add_argument("--tape-host", action="store_true",
@@ -278,6 +280,28 @@ def profile_adjoint(dev, reps=20):
     torch.cuda.synchronize(dev)
     return dict(ms=(time.perf_counter() - t0) / reps * 1e3, fwd_passes=model.pp.fwd.n_tile_passes(),
                 adj_passes=model.pp.adj.n_tile_passes())
+
+
+def profile_batch(dev, nc=512, seed=20260004, reps=3):
+    """The persistent 512-circuit batch alone (eager launches), for kernel traces."""
+    import torch
+
+    from qaravan_amd.persistent import BatchPlan, PersistentBatch
+    from qaravan_amd.workloads import batch_specs, hea_ops, tfim_terms
+    ns, ls, thetas, _ = batch_specs(nc, seed)
+    plan = BatchPlan(ns.tolist(), [hea_ops(int(n), int(l)) for n, l in zip(ns, ls)],
+                     [tfim_terms(int(n)) for n in ns], [len(t) for t in thetas],
+                     keys=list(zip(ns.tolist(), ls.tolist())))
+    b = PersistentBatch(None, None, None, None, device=dev, plan=plan)
+    b.set_theta(thetas)
+    b.run()
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.run()
+    torch.cuda.synchronize(dev)
+    b.check()
+    return dict(ms=(time.perf_counter() - t0) / reps * 1e3, **plan.describe())
 
 
 def bench_blocks(dev, n=30, layers=24, reps=3):
@@ -720,6 +744,9 @@ def main():
         return
     if args.profile_adjoint:
         print(json.dumps({"profile": "adjoint", **profile_adjoint(dev)}))
+        return
+    if args.profile_batch:
+        print(json.dumps({"profile": "batch", **profile_batch(dev)}))
         return
     fwd = bench_forward(args, dev)
     if args.profile_forward:
