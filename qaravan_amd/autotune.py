@@ -189,3 +189,64 @@ def tune_adjoint(n, layers, dtype, device, space=None, repeats=5, log=print, out
     with open(out_path or CACHE, "w") as f:
         json.dump(cache, f, indent=1, sort_keys=True)
     return cfg, results
+
+
+def batch_tuning(dtype):
+    """The persistent batch's tuned settings for ``dtype`` (empty if untuned):
+    {"tasks_per_step": int, "remap_adjoint": {str(n): bool}}."""
+    return _load().get(f"batch:{str(dtype).replace('torch.', '')}", {})
+
+
+def tune_batch(device, dtype=None, nc=512, seed=None, reps=3, log=print, out_path=None):
+    """Tune the persistent batch on the 512-circuit workload: for each qubit
+    count, whether adjoint sweeps remap qubits (timed on the sub-batch of the
+    circuits of that size), then the task granularity (tasks per step) on the
+    whole batch with those choices.  The tile shape itself is fixed by the
+    pattern library the generic kernels are compiled with (tiles of 2^12
+    amplitudes, 3 register bits).  Writes "batch:<dtype>" to the cache."""
+    import numpy as np
+    import torch
+
+    from .persistent import BatchPlan, PersistentBatch
+    from .workloads import SEED_BASE, batch_specs, hea_ops, tfim_terms
+    dtype = dtype or torch.complex64
+    seed = SEED_BASE + 4 if seed is None else seed
+    ns, ls, thetas, _ = batch_specs(nc, seed)
+    ns = [int(x) for x in ns]
+    ls = [int(x) for x in ls]
+
+    def timed(idx, remap, tps):
+        plan = BatchPlan([ns[i] for i in idx], [hea_ops(ns[i], ls[i]) for i in idx],
+                         [tfim_terms(ns[i]) for i in idx], [len(thetas[i]) for i in idx], dtype,
+                         keys=[(ns[i], ls[i]) for i in idx], remap_adjoint=remap, tasks_per_step=tps)
+        b = PersistentBatch(None, None, None, None, device=device, dtype=dtype, plan=plan)
+        b.set_theta([thetas[i] for i in idx])
+        b.capture()
+        b.run()
+        torch.cuda.synchronize(device)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b.run()
+        torch.cuda.synchronize(device)
+        b.check()
+        return (time.perf_counter() - t0) / reps
+
+    remap = {}
+    for n in sorted(set(ns)):
+        idx = [i for i in range(nc) if ns[i] == n]
+        t = {rm: timed(idx, rm, 128) for rm in (True, False)}
+        remap[str(n)] = bool(min(t, key=t.get))
+        log(f"[tune-batch] n={n} ({len(idx)} circuits): remapped adjoint {t[True] * 1e3:.2f} ms, "
+            f"natural {t[False] * 1e3:.2f} ms -> remap={remap[str(n)]}")
+    full = list(range(nc))
+    tt = {}
+    for tps in (64, 128, 256):
+        tt[tps] = timed(full, remap, tps)
+        log(f"[tune-batch] {nc} circuits, {tps} tasks per step: {tt[tps] * 1e3:.1f} ms")
+    best = min(tt, key=tt.get)
+    cache = _load()
+    cache[f"batch:{str(dtype).replace('torch.', '')}"] = dict(tasks_per_step=best, remap_adjoint=remap,
+                                                               ms=round(tt[best] * 1e3, 2))
+    with open(out_path or CACHE, "w") as f:
+        json.dump(cache, f, indent=1, sort_keys=True)
+    return best, remap

@@ -70,9 +70,12 @@ class BatchPlan:
     t_end."""
 
     def __init__(self, ns, op_lists, terms_list, n_params_list, dtype=torch.complex64, keys=None, nq=8,
-                 remap_adjoint=True, workers_per_queue=WORKERS_PER_QUEUE, tasks_per_step=128):
+                 remap_adjoint=None, workers_per_queue=WORKERS_PER_QUEUE, tasks_per_step=None):
         self.dtype = dtype
-        self.tasks_per_step = int(tasks_per_step)
+        from .autotune import batch_tuning
+        tuned = batch_tuning(dtype)
+        # at most this many tasks per step (the tuned value, else 128)
+        self.tasks_per_step = int(tasks_per_step if tasks_per_step is not None else tuned.get("tasks_per_step", 128))
         self.es = 8 if dtype == torch.complex64 else 16
         self.ns = [int(n) for n in ns]
         self.nc = len(self.ns)
@@ -86,7 +89,11 @@ class BatchPlan:
             if key is not None and key in cache:
                 self.pps.append(cache[key])
                 continue
-            cfg, adj_cfg = batch_config(self.ns[c], dtype, 3, remap_adjoint)
+            # remapped adjoint sweeps: as asked (a bool, or {str(n): bool}), else
+            # the tuned choice for this n, else on
+            per_n = remap_adjoint if isinstance(remap_adjoint, dict) else tuned.get("remap_adjoint", {})
+            rm = remap_adjoint if isinstance(remap_adjoint, bool) else per_n.get(str(self.ns[c]), True)
+            cfg, adj_cfg = batch_config(self.ns[c], dtype, 3, rm)
             pp = ParamProgram(None, terms_list[c], n_params_list[c], cfg=cfg, adj_cfg=adj_cfg, dtype=dtype,
                               ops=op_lists[c], n=self.ns[c])
             for prog in (pp.fwd, pp.adj):
@@ -353,7 +360,7 @@ class PersistentBatch:
     ``BatchPlan``; ``set_theta`` then ``run()`` → (energies, flat gradient)."""
 
     def __init__(self, ns, op_lists, terms_list, n_params_list, device="cuda", dtype=torch.complex64, keys=None,
-                 nq=8, remap_adjoint=True, max_workgroups=0, plan=None, xcd_local=True):
+                 nq=8, remap_adjoint=None, max_workgroups=0, plan=None, xcd_local=True):
         _lib.require_gpu()
         self.device = torch.device(device)
         self.dtype = dtype
