@@ -11,6 +11,9 @@
   the pair's backward light cone (at most 14 qubits), which the oracle
   simulates exactly; the GPU's 28-qubit rdm must match it (≤ 1e-12 complex128,
   ≤ 1e-6 complex64).
+* C2 at 30 qubits: the complex64 MFMA blocks against complex128 through the
+  dense kernel (‖Δψ‖ ≤ 1e-5), and light-cone rdms of a 3-layer circuit against
+  the oracle.
 * The 512-circuit batch (10..22 qubits, 4..32 layers) in one persistent launch,
   complex64: energies and gradients of 8 seeded circuits (up to 16 qubits) against the oracle's
   adjoint (relative gradient error ≤ 1e-5, |ΔE| ≤ 1e-5), and of the two largest
@@ -101,6 +104,40 @@ def test_brickwork_28q_light_cone_rdms_vs_oracle(gpu, dtype, tol):
         err = np.abs(r_gpu - r_ref).max()
         print(f"  pair {pair}: cone of {len(cone)} qubits, {len(kept)} gates, max |drho| = {err:.3e}", flush=True)
         assert err <= tol
+
+
+def test_haar_blocks_30q_c64_vs_c128_and_light_cone(gpu):
+    """C2 at its full size (30 qubits, 24 layers of Haar 5-qubit blocks): the
+    complex64 MFMA path against complex128 through the dense kernel (another
+    code path), ‖Δψ‖ ≤ 1e-5; and after 3 layers the 30-qubit rdms of three
+    qubit pairs against the oracle's simulation of their light cones."""
+    from qaravan_amd.measure import rdm_from_sv
+    from qaravan_amd.workloads import haar_block_circuit
+    n = 30
+    circ = haar_block_circuit(n, 24)
+    print("30q x 24 Haar 5-qubit blocks: complex64 (MFMA) vs complex128 (dense kernel)", flush=True)
+    a = _run_28q(gpu, circ, C64).get_statevector()
+    torch.cuda.empty_cache()
+    b = _run_28q(gpu, circ, C128).get_statevector()
+    err = torch.linalg.vector_norm(a.to(C128) - b).item()
+    print(f"  |psi64 - psi128| = {err:.3e}", flush=True)
+    assert abs(torch.linalg.vector_norm(b).item() - 1) < 1e-12
+    assert err <= 1e-5
+    del a, b
+    torch.cuda.empty_cache()
+    circ = haar_block_circuit(n, 3)
+    for dtype, tol in ((C128, 1e-12), (C64, 1e-6)):
+        psi = _run_28q(gpu, circ, dtype).get_statevector()
+        for pair in ([0, 1], [14, 15], [28, 29]):
+            kept, cone = _light_cone(circ.gate_list, pair)
+            assert len(cone) <= 16
+            where = {q: i for i, q in enumerate(cone)}
+            ref = O.run([_Relabelled(g, where) for g in kept], len(cone))
+            err = np.abs(rdm_from_sv(psi, pair).cpu().numpy() - O.rdm_from_sv(ref, [where[q] for q in pair])).max()
+            print(f"  {dtype} pair {pair}: cone of {len(cone)} qubits, max |drho| = {err:.3e}", flush=True)
+            assert err <= tol
+        del psi
+        torch.cuda.empty_cache()
 
 
 def _hea_spec(n, layers):
