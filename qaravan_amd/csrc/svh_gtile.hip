@@ -96,6 +96,24 @@ bool gtile_pattern_matches(int rb, bool dual, int pid, const svh_op_t *ops, int 
 constexpr int kPersistThreads = 512;
 constexpr int kSpinLimit = 1 << 24;
 
+// A descriptor field holding a device address, as a pointer into the global
+// address space.  Converting through address space 1 (not straight to a
+// generic pointer) lets the compiler load the uniform records behind it (pass,
+// stages, ops, angles) with scalar loads, as it does for kernel arguments; from
+// a generic pointer it would issue per-lane flat loads.
+template <typename T>
+__device__ __forceinline__ T *gptr(uint64_t a) {
+    return (T *)(__attribute__((address_space(1))) T *)a;
+}
+// The same for records nothing writes while the kernel runs (pass, stage and
+// op records, angle and term tables, slot maps): through the constant address
+// space, whose loads the compiler may serve from the scalar cache even though
+// the kernel stores to other global memory.
+template <typename T>
+__device__ __forceinline__ const T *cptr(uint64_t a) {
+    return (const T *)(__attribute__((address_space(4))) const T *)a;
+}
+
 // Spin with relaxed device-scope loads (no cache maintenance per poll), then
 // one device-scope acquire once the count is reached.
 // Returns whether the count was not yet reached at the first poll.
@@ -134,9 +152,9 @@ __device__ __forceinline__ double pt_sign(uint64_t v) { return (__popcll(v) & 1)
 // block blk (two per thread), and the block's Σ Re(conj(ψ_i) λ_i) into part[blk].
 template <typename R>
 __device__ __forceinline__ void ham_block(const svh_pdesc_t &D, int64_t blk, double *red) {
-    const cpx<R> *psi = reinterpret_cast<const cpx<R> *>(D.psi);
-    cpx<R> *lam = reinterpret_cast<cpx<R> *>(D.lam);
-    const PTerm *terms = reinterpret_cast<const PTerm *>(D.out);
+    const cpx<R> *psi = gptr<const cpx<R>>(D.psi);
+    cpx<R> *lam = gptr<cpx<R>>(D.lam);
+    const PTerm *terms = cptr<PTerm>(D.out);
     const uint64_t i0 = (uint64_t)blk * 1024 + threadIdx.x, i1 = i0 + 512;
     double ar0 = 0.0, ai0 = 0.0, ar1 = 0.0, ai1 = 0.0;
     for (int t = 0; t < D.ng; ++t) {
@@ -154,18 +172,18 @@ __device__ __forceinline__ void ham_block(const svh_pdesc_t &D, int64_t blk, dou
     const cpx<R> p0 = psi[i0], p1 = psi[i1];
     const double e = ((double)p0.x * ar0 + (double)p0.y * ai0) + ((double)p1.x * ar1 + (double)p1.y * ai1);
     const double tot = block_sum(e, red);
-    if (threadIdx.x == 0) reinterpret_cast<double *>(D.part)[blk] = tot;
+    if (threadIdx.x == 0) gptr<double>(D.part)[blk] = tot;
 }
 
 // Slot s of a reduction: Σ_r part[r·ng + s] in a fixed order.
 __device__ __forceinline__ void red_slot(const svh_pdesc_t &D, int s, double *red) {
-    const double *part = reinterpret_cast<const double *>(D.part);
+    const double *part = gptr<const double>(D.part);
     double acc = 0.0;
     for (int r = threadIdx.x; r < D.rows; r += kPersistThreads) acc += part[(int64_t)r * D.ng + s];
     const double tot = block_sum(acc, red);
     if (threadIdx.x == 0) {
-        if (D.slotsum) reinterpret_cast<double *>(D.slotsum)[s] = tot;
-        else reinterpret_cast<double *>(D.out)[reinterpret_cast<const int32_t *>(D.slot_param)[s]] = tot;
+        if (D.slotsum) gptr<double>(D.slotsum)[s] = tot;
+        else gptr<double>(D.out)[cptr<int32_t>(D.slot_param)[s]] = tot;
     }
 }
 
@@ -206,7 +224,9 @@ k_batch_persist(const svh_pdesc_t *__restrict__ descs, const svh_ptask_t *__rest
             s_task = t;
         }
         __syncthreads();
-        const int t = s_task;
+        // the same value in every lane: say so, so that the task record, its
+        // descriptor and every record they point to load through the scalar unit
+        const int t = __builtin_amdgcn_readfirstlane(s_task);
         __syncthreads();  // every thread has read s_task before thread 0 writes it again
         if (t < 0) break;
         const svh_ptask_t T = tasks[t];
@@ -226,28 +246,28 @@ k_batch_persist(const svh_pdesc_t *__restrict__ descs, const svh_ptask_t *__rest
         for (int u = T.first; u < T.first + T.count; ++u) {
             if (u > T.first) __syncthreads();  // the previous unit is done with LDS
             if (D.kind == SVH_PT_FWD || D.kind == SVH_PT_ADJ) {
-                const auto *pass = reinterpret_cast<const svh_pass_t *>(D.pass);
-                cpx<R> *psi = reinterpret_cast<cpx<R> *>(D.psi);
+                const auto *pass = cptr<svh_pass_t>(D.pass);
+                cpx<R> *psi = gptr<cpx<R>>(D.psi);
                 cpx<R> *lds2 = lds + (1 << pass->L);
                 if (D.kind == SVH_PT_ADJ)
-                    gtile<R, 3, true>(psi, psi, reinterpret_cast<cpx<R> *>(D.lam), D.n, pass,
-                                      reinterpret_cast<const svh_stage_t *>(D.stages),
-                                      reinterpret_cast<const svh_op_t *>(D.ops),
-                                      reinterpret_cast<const cpx<R> *>(D.trig), reinterpret_cast<double *>(D.part),
+                    gtile<R, 3, true>(psi, psi, gptr<cpx<R>>(D.lam), D.n, pass,
+                                      cptr<svh_stage_t>(D.stages),
+                                      cptr<svh_op_t>(D.ops),
+                                      cptr<cpx<R>>(D.trig), gptr<double>(D.part),
                                       (uint64_t)u, lds, lds2);
                 else
-                    gtile<R, 3, false>(psi, psi, nullptr, D.n, pass, reinterpret_cast<const svh_stage_t *>(D.stages),
-                                       reinterpret_cast<const svh_op_t *>(D.ops),
-                                       reinterpret_cast<const cpx<R> *>(D.trig), nullptr, (uint64_t)u, lds, lds2);
+                    gtile<R, 3, false>(psi, psi, nullptr, D.n, pass, cptr<svh_stage_t>(D.stages),
+                                       cptr<svh_op_t>(D.ops),
+                                       cptr<cpx<R>>(D.trig), nullptr, (uint64_t)u, lds, lds2);
             } else if (D.kind == SVH_PT_HAM) {
                 ham_block<R>(D, u, red);
             } else if (D.kind == SVH_PT_RED) {
                 red_slot(D, u, red);
             } else if (D.kind == SVH_PT_SCT) {
                 if (threadIdx.x == 0) {
-                    const double *ss = reinterpret_cast<const double *>(D.slotsum);
-                    const int32_t *sp = reinterpret_cast<const int32_t *>(D.slot_param);
-                    double *out = reinterpret_cast<double *>(D.out);
+                    const double *ss = gptr<const double>(D.slotsum);
+                    const int32_t *sp = cptr<int32_t>(D.slot_param);
+                    double *out = gptr<double>(D.out);
                     for (int s = 0; s < D.ng; ++s) out[sp[s]] += ss[s];
                 }
             }
@@ -265,9 +285,9 @@ k_batch_persist(const svh_pdesc_t *__restrict__ descs, const svh_ptask_t *__rest
                 const int before =
                     __hip_atomic_fetch_add(ctr + D.fin, T.count, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
                 if (before + T.count == D.fin_need && D.tstamp)
-                    *reinterpret_cast<uint64_t *>(D.tstamp) = wall_clock64();
+                    *gptr<uint64_t>(D.tstamp) = wall_clock64();
             } else if (D.tstamp) {
-                *reinterpret_cast<uint64_t *>(D.tstamp) = wall_clock64();
+                *gptr<uint64_t>(D.tstamp) = wall_clock64();
             }
             if (stats) {
                 t_task += wall_clock64() - t0;
