@@ -32,12 +32,18 @@ def sources():
 # VGPRs only if LLVM does not merge look-alike switch arms into one block that
 # addresses the array through a phi of pointers (SimplifyCFG sinking), which
 # would force the array into scratch memory.
-FILE_FLAGS = {"svh_gtile.hip": ["-mllvm", "-simplifycfg-sink-common=false"]}
+GTILE_FLAGS = ["-mllvm", "-simplifycfg-sink-common=false"]
+
+
+def file_flags(name):
+    """Extra flags of one source: the generic tile kernels (svh_gtile.hip and
+    its per-instantiation files svh_gt_*.hip, svh_pb_*.hip)."""
+    return GTILE_FLAGS if name == "svh_gtile.hip" or name.startswith(("svh_gt_", "svh_pb_")) else []
 
 
 def _compile(src, objdir):
     obj = os.path.join(objdir, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC, *FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, *file_flags(os.path.basename(src)), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:

@@ -76,6 +76,7 @@ class BatchPlan:
         tuned = batch_tuning(dtype)
         # at most this many tasks per step (the tuned value, else 128)
         self.tasks_per_step = int(tasks_per_step if tasks_per_step is not None else tuned.get("tasks_per_step", 128))
+
         self.es = 8 if dtype == torch.complex64 else 16
         self.ns = [int(n) for n in ns]
         self.nc = len(self.ns)
@@ -128,7 +129,7 @@ class BatchPlan:
         self.gpart_doubles = 0
         self.slotsum_doubles = 0
         self.ncounters = 0
-        self.max_lds = 512 * 8
+        self.max_lds = 512 * 8      # the reductions' tree (at most 512 doubles)
         self.cost = np.zeros(self.nc)
 
         def counter():
