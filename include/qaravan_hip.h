@@ -195,9 +195,16 @@ int svh_spb_launch(void *handle, int index, const void *items_dev, int nitems, i
  * back, queue q = tasks [qbeg[q], qbeg[q+1]) (qbeg_dev: nq + 1 int32, device);
  * heads_dev: nq int32 scratch; counters_dev: ncounters int32 scratch named by
  * the descriptors' dep/done/fin fields; err_dev: one int32, set to 1 when a
- * dependency wait exceeded its bound (the results are then invalid).  Every
+ * dependency wait exceeded its bound, 2 when a tile pass does not fit the
+ * workgroup (more than `threads` threads, register bits other than 3, or more
+ * amplitudes than the λ offset; its units are skipped; the results are then
+ * invalid).  Every
  * descriptor's dependencies must come earlier in its own queue.  lds_bytes:
- * the largest tile a task needs (2·2^L amplitudes for an adjoint tile);
+ * the LDS a task needs; the launch takes at least the ψ and λ tiles of the
+ * largest adjoint tile (λ at a fixed offset of threads·8 complex64 or
+ * threads·4 complex128 amplitudes; tiles with more amplitudes are refused);
+ * threads: workgroup size, 512 (0 means 512) or, for complex64, 256; a tile
+ * pass's 2^tb must not exceed it.
  * max_workgroups: cap on the resident grid (0 = fill the device).
  * xcd_local != 0 asks for XCD-local synchronisation: a workgroup serves only
  * the queue of the XCD it runs on and dependencies are made visible through
@@ -214,8 +221,8 @@ int svh_spb_launch(void *handle, int index, const void *items_dev, int nitems, i
  * counters race. */
 int svh_batch_persistent(const void *descs_dev, const void *tasks_dev, const int32_t *qbeg_dev, int nq,
                          int32_t *heads_dev, int32_t *counters_dev, int ncounters, int32_t *err_dev, int dtype,
-                         size_t lds_bytes, int max_workgroups, int xcd_local, int *used_mode, uint64_t *stats_dev,
-                         void *stream);
+                         size_t lds_bytes, int threads, int max_workgroups, int xcd_local, int *used_mode,
+                         uint64_t *stats_dev, void *stream);
 
 /* out_dev[0] = wall clock (s_memrealtime, 100 MHz ticks), for batch latencies. */
 int svh_timestamp(uint64_t *out_dev, void *stream);

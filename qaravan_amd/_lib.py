@@ -60,7 +60,7 @@ SIGNATURES = {
     "svh_batch_init": (_c_int, [_c_void_p, ctypes.c_int64, _c_void_p, _c_int, _c_int, _c_void_p]),
     "svh_batch_trig": (_c_int, [_c_void_p, ctypes.c_int64, _c_void_p, _c_int, _c_void_p]),
     "svh_batch_persistent": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_void_p, _c_void_p, _c_int,
-                                      _c_void_p, _c_int, _c_size_t, _c_int, _c_int, ctypes.POINTER(_c_int),
+                                      _c_void_p, _c_int, _c_size_t, _c_int, _c_int, _c_int, ctypes.POINTER(_c_int),
                                       _c_void_p, _c_void_p]),
     "svh_timestamp": (_c_int, [_c_void_p, _c_void_p]),
     "svh_spb_load": (_c_int, [ctypes.POINTER(ctypes.c_char_p), _c_int, ctypes.POINTER(ctypes.c_char_p), _c_int,

@@ -40,14 +40,15 @@ def default_tb(n, dtype, rb=RB):
     return max(4, min(12 - rb, n - rb))
 
 
-def batch_config(n, dtype, rb=3, remap_adjoint=True):
+def batch_config(n, dtype, rb=3, remap_adjoint=True, threads=512):
     """(forward, adjoint) tile configurations of one circuit in the persistent
-    batch: generic kernels, tiles of 2^12 amplitudes (2^11 for complex128, so
-    ψ and λ fit twice per CU's LDS) or the whole state when it is smaller, no
-    pass cap.  The forward program keeps the natural bit order (the
-    Hamiltonian step reads it); the adjoint sweep may remap, since its ψ and λ
-    are scratch afterwards."""
+    batch: generic kernels, tiles of 2^12 amplitudes with 512-thread
+    workgroups (2^11 with 256, and for complex128, so ψ and λ fit twice per
+    CU's LDS) or the whole state when it is smaller, no pass cap.  The forward
+    program keeps the natural bit order (the Hamiltonian step reads it); the
+    adjoint sweep may remap, since its ψ and λ are scratch afterwards."""
     top = 12 if dtype == torch.complex64 else 11
+    top = min(top, threads.bit_length() - 1 + rb)
     tb = max(6, min(top - rb, n - rb))
     cfg = TileConfig(tb=tb, low_bits=4, max_cost=1e9, io_lane_bits=4, rb=rb, generic=True)
     return cfg, replace(cfg, remap=bool(remap_adjoint))

@@ -92,15 +92,26 @@ __device__ __forceinline__ bool run_pattern(int pid, cpx<R> (&v)[1 << RB], cpx<R
 }
 
 // One tile of one pass.  st = the pass's first stage record, ops = the
-// program's op array; lds (and lds2 for λ) hold 2^L amplitudes each.  The
-// tile is loaded from psi and stored to out (the same buffer except for the
-// out-of-place last pass of a program that ends with a bit permutation).
-template <typename R, int RB, bool DUAL>
+// program's op array; lds (and, for λ, lds2, or lds + LOFF amplitudes when
+// LOFF > 0) hold 2^L amplitudes each.  The tile is loaded from psi and stored
+// to out (the same buffer except for the out-of-place last pass of a program
+// that ends with a bit permutation).
+//
+// LDS addressing: a layout puts register j of thread t at slot
+// swz(local(t)) ^ lsw[j].  The thread's swizzled base (in bytes) is computed
+// once per layout and serves both the exchange that reads a layout and the
+// next one that writes it; slot j then costs one XOR with the uniform
+// lsw[j]·sizeof(amplitude), and a constant λ offset goes into the LDS
+// instruction's offset field.
+template <typename R, int RB, bool DUAL, int LOFF = 0>
 __device__ __forceinline__ void gtile(const cpx<R> *psi, cpx<R> *out, cpx<R> *__restrict__ lam, int n,
                                       const svh_pass_t *__restrict__ pass, const svh_stage_t *__restrict__ st,
                                       const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig,
                                       double *__restrict__ gpart, uint64_t wg, cpx<R> *lds, cpx<R> *lds2) {
     constexpr int NR = 1 << RB;
+    constexpr int ES = (int)sizeof(cpx<R>);
+    char *const lb = reinterpret_cast<char *>(lds);
+    char *const lb2 = LOFF > 0 ? lb + LOFF * ES : reinterpret_cast<char *>(lds2);
     const int tid = threadIdx.x;
     const int tb = pass->tb;
     const bool active = tid < (1 << tb);
@@ -121,26 +132,29 @@ __device__ __forceinline__ void gtile(const cpx<R> *psi, cpx<R> *out, cpx<R> *__
             if constexpr (DUAL) w[j] = lam[t + S.goff[j]];
         }
     }
+    // byte offset of this thread's slot 0 in the current stage's layout
+    int tcur = ns > 1 ? lds_swz(thr_local<RB>(st[0], tid)) * ES : 0;
     for (int s = 0; s < ns; ++s) {
         const svh_stage_t &S = st[s];
         if (s > 0) {
             const svh_stage_t &Pv = st[s - 1];
             if (s > 1) __syncthreads();  // the previous exchange's reads are done
             if (active) {
-                const int tp = lds_swz(thr_local<RB>(Pv, tid));
 #pragma unroll
                 for (int j = 0; j < NR; ++j) {
-                    lds[tp ^ Pv.lsw[j]] = v[j];
-                    if constexpr (DUAL) lds2[tp ^ Pv.lsw[j]] = w[j];
+                    const int o = tcur ^ (Pv.lsw[j] * ES);
+                    *reinterpret_cast<cpx<R> *>(lb + o) = v[j];
+                    if constexpr (DUAL) *reinterpret_cast<cpx<R> *>(lb2 + o) = w[j];
                 }
             }
             __syncthreads();
+            tcur = lds_swz(thr_local<RB>(S, tid)) * ES;
             if (active) {
-                const int tn = lds_swz(thr_local<RB>(S, tid));
 #pragma unroll
                 for (int j = 0; j < NR; ++j) {
-                    v[j] = lds[tn ^ S.lsw[j]];
-                    if constexpr (DUAL) w[j] = lds2[tn ^ S.lsw[j]];
+                    const int o = tcur ^ (S.lsw[j] * ES);
+                    v[j] = *reinterpret_cast<const cpx<R> *>(lb + o);
+                    if constexpr (DUAL) w[j] = *reinterpret_cast<const cpx<R> *>(lb2 + o);
                 }
             }
         }

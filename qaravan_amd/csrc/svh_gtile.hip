@@ -75,7 +75,10 @@ struct PersistShape {
     uint32_t xcc_mask;  // XCDs the grid's workgroups ran on (census), 0 = unknown
 };
 
-static int persist_grid(const void *kern, int threads, size_t lds, int *grid, uint32_t *xcc_mask) {
+// lds: the kernel's dynamic LDS; census_lds: the LDS the census kernel takes
+// so that its workgroups are resident in the same numbers per CU.
+static int persist_grid(const void *kern, int threads, size_t lds, size_t census_lds, int *grid,
+                        uint32_t *xcc_mask) {
     static std::mutex mu;
     static std::unordered_map<uint64_t, PersistShape> cache;
     int dev = 0;
@@ -105,11 +108,12 @@ static int persist_grid(const void *kern, int threads, size_t lds, int *grid, ui
     // device and shape; skipped while a graph is being captured)
     uint32_t mask = 0;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(nullptr, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+    if (hipStreamIsCapturing(nullptr, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone &&
+        ensure_lds_limit((const void *)k_xcc_census, 160 * 1024, "hipFuncSetAttribute(k_xcc_census)") == 0) {
         int32_t *d = nullptr;
         if (hipMalloc(&d, sizeof(int32_t) * (size_t)*grid) == hipSuccess) {
             std::vector<int32_t> h((size_t)*grid, -1);
-            k_xcc_census<<<*grid, threads, lds, 0>>>(d);
+            k_xcc_census<<<*grid, threads, census_lds, 0>>>(d);
             if (hipMemcpy(h.data(), d, sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost) == hipSuccess)
                 for (int32_t x : h)
                     if (x >= 0 && x < 32) mask |= 1u << x;
@@ -128,19 +132,30 @@ using namespace svh;
 
 extern "C" int svh_batch_persistent(const void *descs_dev, const void *tasks_dev, const int32_t *qbeg_dev, int nq,
                                     int32_t *heads_dev, int32_t *counters_dev, int ncounters, int32_t *err_dev,
-                                    int dtype, size_t lds_bytes, int max_workgroups, int xcd_local, int *used_mode,
-                                    uint64_t *stats_dev, void *stream) {
+                                    int dtype, size_t lds_bytes, int threads, int max_workgroups, int xcd_local,
+                                    int *used_mode, uint64_t *stats_dev, void *stream) {
     if (!descs_dev || !tasks_dev || !qbeg_dev || !heads_dev || !err_dev || (ncounters > 0 && !counters_dev))
         return fail(-1, "NULL pointer");
     if (nq < 1 || ncounters < 0) return fail(-4, "bad queue or counter count");
     if (dtype != SVH_C64 && dtype != SVH_C128) return fail(-3, "bad dtype");
     if (lds_bytes > 159 * 1024) return fail(-7, "persistent batch needs more than 159 KiB of LDS");
-    const int threads = 512;  // tiles of 3 register bits (generic passes)
-    const size_t need = lds_bytes > threads * sizeof(double) ? lds_bytes : threads * sizeof(double);
-    const void *kern = dtype == SVH_C128 ? persist_kernel_d3() : persist_kernel_f3();
+    // workgroup size: 512 threads (two resident per CU), or for complex64 256
+    // (four per CU, tiles of at most 2^11 amplitudes)
+    if (threads == 0) threads = 512;
+    if (threads != 512 && !(threads == 256 && dtype == SVH_C64))
+        return fail(-4, "persistent batch: workgroups of 512 threads, or 256 for complex64");
+    const bool small = threads == 256;
+    // the kernel's static LDS holds the ψ and λ tiles of an adjoint task (λ
+    // at its fixed offset of threads·8 complex64 or threads·4 complex128
+    // amplitudes), the reduction tree and the task index; no dynamic LDS
+    const size_t lam_bytes = dtype == SVH_C64 ? (size_t)threads * 8 * 8 : (size_t)threads * 4 * 16;
+    if (lds_bytes > 2 * lam_bytes)
+        return fail(-7, "persistent batch: tiles larger than the workgroup's LDS (λ offset)");
+    const size_t need = 0;
+    const void *kern = dtype == SVH_C128 ? persist_kernel_d3() : small ? persist_kernel_f3_256() : persist_kernel_f3();
     int grid = 0;
     uint32_t xcc_mask = 0;
-    if (int e = persist_grid(kern, threads, need, &grid, &xcc_mask)) return e;
+    if (int e = persist_grid(kern, threads, need, 2 * lam_bytes, &grid, &xcc_mask)) return e;
     if (max_workgroups > 0 && grid > max_workgroups) grid = max_workgroups;
     // XCD-local synchronisation only when the full grid covers exactly the
     // XCDs 0..nq-1, so every queue is served from inside its own XCD
@@ -156,7 +171,7 @@ extern "C" int svh_batch_persistent(const void *descs_dev, const void *tasks_dev
     const auto *D = static_cast<const svh_pdesc_t *>(descs_dev);
     const auto *T = static_cast<const svh_ptask_t *>(tasks_dev);
     auto *S = (unsigned long long *)stats_dev;
-    auto launch = dtype == SVH_C128 ? persist_launch_d3 : persist_launch_f3;
+    auto launch = dtype == SVH_C128 ? persist_launch_d3 : small ? persist_launch_f3_256 : persist_launch_f3;
     return launch((unsigned)grid, (unsigned)threads, need, st, D, T, qbeg_dev, nq, heads_dev, counters_dev, err_dev,
                   mode, S);
 }

@@ -23,6 +23,7 @@ SVH_GT_DECL(d4)
                            int32_t *err, int mode, unsigned long long *stats);
 SVH_PB_DECL(f3)
 SVH_PB_DECL(d3)
+SVH_PB_DECL(f3_256)
 #undef SVH_PB_DECL
 
 }  // namespace svh

@@ -159,15 +159,29 @@ constexpr int persist_waves_per_eu() {
     return sizeof(R) == 4 ? 4 : 2;
 }
 
+// The λ tile's place in LDS: right after the largest ψ tile the batch plans
+// for this workgroup size (2^12 complex64 amplitudes with 512 threads, 2^11
+// with 256, and 2^11 complex128), a constant the LDS instructions take as
+// their offset (at most 32 KiB).
+template <typename R, int TH, int RB>
+constexpr int persist_lam_offset() {
+    return sizeof(R) == 4 ? (TH << RB) : (TH << (RB - 1));
+}
+
 template <typename R, int TH, int RB>
 __global__ void __launch_bounds__(TH, (persist_waves_per_eu<R, TH, RB>()))
 k_batch_persist(const svh_pdesc_t *__restrict__ descs, const svh_ptask_t *__restrict__ tasks,
                 const int32_t *__restrict__ qbeg, int nq, int32_t *heads, int32_t *ctr, int32_t *err,
                 int xcd_local, unsigned long long *stats) {
-    extern __shared__ unsigned char smem_raw[];
-    __shared__ int s_task;
-    cpx<R> *lds = reinterpret_cast<cpx<R> *>(smem_raw);
-    double *red = reinterpret_cast<double *>(smem_raw);
+    // One static LDS array, at a compile-time address, so that the tile
+    // slots' addresses fold into the LDS instructions: the ψ tile, the λ tile
+    // at its fixed offset, and over the same bytes the reduction tree and the
+    // claimed task index (used only between barriers where no tile is live).
+    constexpr int LOFF = persist_lam_offset<R, TH, RB>();
+    __shared__ cpx<R> tiles[2 * LOFF];
+    cpx<R> *lds = tiles;
+    double *red = reinterpret_cast<double *>(tiles);
+    int &s_task = *reinterpret_cast<int *>(tiles);
     // home queue: the XCD this workgroup runs on (hardware register XCC_ID,
     // bits 3:0), so a queue's circuits stay in one L2
     const int xcc = (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) & 15);
@@ -216,18 +230,23 @@ k_batch_persist(const svh_pdesc_t *__restrict__ descs, const svh_ptask_t *__rest
             if (u > T.first) __syncthreads();  // the previous unit is done with LDS
             if (D.kind == SVH_PT_FWD || D.kind == SVH_PT_ADJ) {
                 const auto *pass = cptr<svh_pass_t>(D.pass);
+                // a tile wider than the workgroup, with other register bits, or
+                // larger than the λ offset: refuse it
+                if ((1 << pass->tb) > TH || pass->rb != RB || (1 << pass->L) > persist_lam_offset<R, TH, RB>()) {
+                    if (threadIdx.x == 0) atomicExch(err, 2);
+                    break;
+                }
                 cpx<R> *psi = gptr<cpx<R>>(D.psi);
-                cpx<R> *lds2 = lds + (1 << pass->L);
                 if (D.kind == SVH_PT_ADJ)
-                    gtile<R, RB, true>(psi, psi, gptr<cpx<R>>(D.lam), D.n, pass,
-                                      cptr<svh_stage_t>(D.stages),
-                                      cptr<svh_op_t>(D.ops),
-                                      cptr<cpx<R>>(D.trig), gptr<double>(D.part),
-                                      (uint64_t)u, lds, lds2);
+                    gtile<R, RB, true, LOFF>(psi, psi, gptr<cpx<R>>(D.lam), D.n, pass,
+                                             cptr<svh_stage_t>(D.stages),
+                                             cptr<svh_op_t>(D.ops),
+                                             cptr<cpx<R>>(D.trig), gptr<double>(D.part),
+                                             (uint64_t)u, lds, nullptr);
                 else
                     gtile<R, RB, false>(psi, psi, nullptr, D.n, pass, cptr<svh_stage_t>(D.stages),
                                        cptr<svh_op_t>(D.ops),
-                                       cptr<cpx<R>>(D.trig), nullptr, (uint64_t)u, lds, lds2);
+                                       cptr<cpx<R>>(D.trig), nullptr, (uint64_t)u, lds, nullptr);
             } else if (D.kind == SVH_PT_HAM) {
                 ham_block<R, TH>(D, u, red);
             } else if (D.kind == SVH_PT_RED) {

@@ -12,7 +12,8 @@ dependency chain of steps:
 
 Each step is a descriptor (svh_pdesc_t) whose units (tiles, Hamiltonian
 blocks, reduction slots) are cut into tasks.  ``svh_batch_persistent`` runs a
-resident grid of 512-thread workgroups that take tasks from per-queue lists
+resident grid of workgroups (512 threads, or 256 for complex64) that take
+tasks from per-queue lists
 and wait, per task, only for the one step it reads (a device counter), so a
 circuit advances as soon as its own previous step is done: small circuits
 finish early instead of waiting for the slowest pass of a round, and no launch
@@ -52,7 +53,13 @@ PDESC_DT = np.dtype([('kind', '<i4'), ('n', '<i4'), ('dep', '<i4'), ('dep_need',
 PTASK_DT = np.dtype([('desc', '<i4'), ('first', '<i4'), ('count', '<i4'), ('pad', '<i4')])
 assert PDESC_DT.itemsize == 128 and PTASK_DT.itemsize == 16
 _PTR_FIELDS = ('psi', 'lam', 'pass', 'stages', 'ops', 'trig', 'part', 'out', 'slot_param', 'slotsum', 'tstamp')
-WORKERS_PER_QUEUE = 64   # resident 512-thread workgroups per XCD (2 per CU): the simulation's estimate
+CUS_PER_XCD = 32         # MI355X: 256 CUs in 8 XCDs
+
+
+def queue_workers(threads):
+    """Resident workgroups per XCD (the schedule simulation's worker count):
+    2 per CU of 512 threads, 4 of 256."""
+    return CUS_PER_XCD * (1024 // threads)
 
 
 def _hdr(blob):
@@ -70,10 +77,15 @@ class BatchPlan:
     t_end."""
 
     def __init__(self, ns, op_lists, terms_list, n_params_list, dtype=torch.complex64, keys=None, nq=8,
-                 remap_adjoint=None, workers_per_queue=WORKERS_PER_QUEUE, tasks_per_step=None):
+                 remap_adjoint=None, workers_per_queue=None, tasks_per_step=None, threads=None):
         self.dtype = dtype
         from .autotune import batch_tuning
         tuned = batch_tuning(dtype)
+        # workgroup size (the tuned value, else 512; 256 only for complex64):
+        # every tile pass has at most this many threads
+        self.threads = int(threads if threads is not None else tuned.get("threads", 512))
+        if self.threads not in (256, 512) or (self.threads == 256 and dtype != torch.complex64):
+            raise ValueError("persistent batch workgroups: 512 threads, or 256 for complex64")
         # at most this many tasks per step (the tuned value, else 128)
         self.tasks_per_step = int(tasks_per_step if tasks_per_step is not None else tuned.get("tasks_per_step", 128))
 
@@ -94,7 +106,7 @@ class BatchPlan:
             # the tuned choice for this n, else on
             per_n = remap_adjoint if isinstance(remap_adjoint, dict) else tuned.get("remap_adjoint", {})
             rm = remap_adjoint if isinstance(remap_adjoint, bool) else per_n.get(str(self.ns[c]), True)
-            cfg, adj_cfg = batch_config(self.ns[c], dtype, 3, rm)
+            cfg, adj_cfg = batch_config(self.ns[c], dtype, 3, rm, self.threads)
             pp = ParamProgram(None, terms_list[c], n_params_list[c], cfg=cfg, adj_cfg=adj_cfg, dtype=dtype,
                               ops=op_lists[c], n=self.ns[c])
             for prog in (pp.fwd, pp.adj):
@@ -121,7 +133,7 @@ class BatchPlan:
         self.terms_host = np.concatenate(tt).view(np.uint8).copy()
         self.ep_off = np.concatenate([[0], np.cumsum([2 ** n // 1024 for n in self.ns])]).astype(np.int64)
         self._build_descs()
-        self._build_tasks(workers_per_queue)
+        self._build_tasks(workers_per_queue or queue_workers(self.threads))
 
     # ------------------------------------------------------------ descriptors
     def _build_descs(self):
@@ -129,7 +141,8 @@ class BatchPlan:
         self.gpart_doubles = 0
         self.slotsum_doubles = 0
         self.ncounters = 0
-        self.max_lds = 512 * 8      # the reductions' tree (at most 512 doubles)
+        self.max_lds = self.threads * 8      # the reductions' tree (one double per thread)
+        self.lam_off = self.threads * (8 if self.dtype == torch.complex64 else 4)
         self.cost = np.zeros(self.nc)
 
         def counter():
@@ -154,6 +167,7 @@ class BatchPlan:
                 h = _hdr(prog.blob)
                 p = prog.passes[pi]
                 L = p.tb + p.rb
+                assert 2 ** p.tb <= self.threads, "a tile pass wider than the workgroup"
                 nwg = 2 ** (n - L)
                 ps = _pass_row(prog, pi)
                 ng = int(ps['n_grad'])
@@ -163,7 +177,8 @@ class BatchPlan:
                         **{'pass': ('progs', base + int(h['off_passes']) + pi * PASS_DT.itemsize)},
                         stages=('progs', base + int(h['off_stages']) + int(ps['stage_begin']) * STAGE_DT.itemsize),
                         ops=('progs', base + int(h['off_ops'])), trig=trig, part=part or 0)
-                self.max_lds = max(self.max_lds, (2 if dual else 1) * self.es * 2 ** L)
+                # λ sits at a fixed offset (threads·8 complex64 / threads·4 complex128 amplitudes)
+                self.max_lds = max(self.max_lds, self.es * ((self.lam_off if dual else 0) + 2 ** L))
                 unit = (1.0 + 0.12 * len(p.stages) + 0.04 * len(p.ops) * (2.5 if dual else 1.0)) * 2 ** (L - 12)
                 self.cost[c] += unit * nwg
                 chain.append((d, [dep_desc] if dep_desc is not None else [], unit))
@@ -361,12 +376,13 @@ class PersistentBatch:
     ``BatchPlan``; ``set_theta`` then ``run()`` → (energies, flat gradient)."""
 
     def __init__(self, ns, op_lists, terms_list, n_params_list, device="cuda", dtype=torch.complex64, keys=None,
-                 nq=8, remap_adjoint=None, max_workgroups=0, plan=None, xcd_local=True):
+                 nq=8, remap_adjoint=None, max_workgroups=0, plan=None, xcd_local=True, threads=None):
         _lib.require_gpu()
         self.device = torch.device(device)
         self.dtype = dtype
         self.code = _lib.dtype_code(dtype)
-        self.plan = plan or BatchPlan(ns, op_lists, terms_list, n_params_list, dtype, keys, nq, remap_adjoint)
+        self.plan = plan or BatchPlan(ns, op_lists, terms_list, n_params_list, dtype, keys, nq, remap_adjoint,
+                                      threads=threads)
         P = self.plan
         dev = self.device
         self.nc = P.nc
@@ -430,7 +446,8 @@ class PersistentBatch:
         mode = ctypes.c_int(-1)
         _lib.check(lib.svh_batch_persistent(_lib.ptr(self.descs), _lib.ptr(self.tasks), _lib.ptr(self.qbeg), P.nq,
                                             _lib.ptr(self.heads), _lib.ptr(self.counters), P.ncounters,
-                                            _lib.ptr(self.err), self.code, P.max_lds, self.max_workgroups,
+                                            _lib.ptr(self.err), self.code, P.max_lds, P.threads,
+                                            self.max_workgroups,
                                             int(self.xcd_local), ctypes.byref(mode),
                                             _lib.ptr(self.stats) if self.stats is not None else None, st),
                    "svh_batch_persistent")
@@ -457,7 +474,9 @@ class PersistentBatch:
         """Raise if a device dependency wait ran past its bound, or a queue was
         not fully taken (synchronises)."""
         if int(self.err.item()) != 0:
-            raise _lib.HipLibraryError("persistent batch: a dependency wait exceeded its bound; results invalid")
+            what = ("a dependency wait exceeded its bound" if int(self.err.item()) == 1
+                    else "a tile pass is wider than the workgroup")
+            raise _lib.HipLibraryError(f"persistent batch: {what}; results invalid")
         heads = self.heads.cpu().numpy()
         qlen = np.diff(self.plan.qbeg)
         if (heads < qlen).any():

@@ -100,7 +100,9 @@ def _check_bounds(P):
             rec = np.frombuffer(P.progs_host[off:off + PASS_DT.itemsize].tobytes(), dtype=PASS_DT)[0]
             assert d['units'] == 2 ** (n - int(rec['L'])) and d['ng'] == int(rec['n_grad'])
             assert 6 <= int(rec['tb']) <= 9 and int(rec['rb']) == 3
-            assert P.max_lds >= (2 if d['kind'] == PT_ADJ else 1) * es * 2 ** int(rec['L'])
+            # λ at its fixed offset, past every ψ tile
+            assert 2 ** int(rec['L']) <= P.lam_off
+            assert P.max_lds >= es * ((P.lam_off if d['kind'] == PT_ADJ else 0) + 2 ** int(rec['L']))
             if d['kind'] == PT_ADJ:        # one partial row per wave of every tile
                 inside(d['part'], 8 * (d['units'] << (int(rec['tb']) - 6)) * max(d['ng'], 1))
         if d['kind'] == PT_HAM:
@@ -112,13 +114,20 @@ def _check_bounds(P):
                 inside(d['slotsum'], 8 * d['units'])
 
 
+@pytest.mark.parametrize("threads", [512, 256])
 @pytest.mark.parametrize("shared", [False, True])
-def test_plan_dependencies_and_coverage(shared):
+def test_plan_dependencies_and_coverage(shared, threads):
     rng = np.random.default_rng(7)
     ns, ls, ops, npar, _ = _batch(rng, 10, shared=shared)
-    P = BatchPlan(ns, ops, [tfim_terms(n) for n in ns], npar, dtype=C64, nq=3)
+    P = BatchPlan(ns, ops, [tfim_terms(n) for n in ns], npar, dtype=C64, nq=3, threads=threads)
     _check_plan(P)
     _check_bounds(P)
+    assert P.max_lds >= 8 * threads
+    for d in P.descs:            # every tile fits the workgroup
+        if d['kind'] in (PT_FWD, PT_ADJ):
+            off = d['pass'][1]
+            rec = np.frombuffer(P.progs_host[off:off + PASS_DT.itemsize].tobytes(), dtype=PASS_DT)[0]
+            assert 2 ** int(rec['tb']) <= threads
     kinds = [d['kind'] for d in P.descs]
     assert kinds.count(PT_HAM) == len(ns) and kinds.count(PT_FWD) > 0 and kinds.count(PT_ADJ) > 0
     assert (kinds.count(PT_SCT) == len(ns)) == shared
@@ -136,14 +145,14 @@ def _run(gpu, ns, ops, npar, thetas, dtype, **kw):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("dtype,threads", [(C128, 512), (C64, 512), (C64, 256)])
 @pytest.mark.parametrize("shared", [False, True])
-def test_persistent_vs_oracle(gpu, dtype, shared):
+def test_persistent_vs_oracle(gpu, dtype, threads, shared):
     rng = np.random.default_rng(11 + shared)
     ns, ls, ops, npar, params = _batch(rng, 8, shared=shared)
     thetas = [rng.uniform(-np.pi, np.pi, size=p) for p in npar]
-    print(f"persistent batch ns={ns} layers={ls} shared={shared} {dtype}", flush=True)
-    b, e, g = _run(gpu, ns, ops, npar, thetas, dtype, nq=3)
+    print(f"persistent batch ns={ns} layers={ls} shared={shared} {dtype} threads={threads}", flush=True)
+    b, e, g = _run(gpu, ns, ops, npar, thetas, dtype, nq=3, threads=threads)
     gl = b.gradients()
     tol = 1e-10 if dtype == C128 else 1e-5
     for c, (n, L) in enumerate(zip(ns, ls)):
