@@ -193,7 +193,7 @@ def tune_adjoint(n, layers, dtype, device, space=None, repeats=5, log=print, out
 
 def batch_tuning(dtype):
     """The persistent batch's tuned settings for ``dtype`` (empty if untuned):
-    {"tasks_per_step": int, "remap_adjoint": {str(n): bool}}."""
+    {"tasks_per_step": int, "remap_adjoint": {str(n): bool}, "threads": int}."""
     return _load().get(f"batch:{str(dtype).replace('torch.', '')}", {})
 
 
@@ -201,9 +201,10 @@ def tune_batch(device, dtype=None, nc=512, seed=None, reps=3, log=print, out_pat
     """Tune the persistent batch on the 512-circuit workload: for each qubit
     count, whether adjoint sweeps remap qubits (timed on the sub-batch of the
     circuits of that size), then the task granularity (tasks per step) on the
-    whole batch with those choices.  The tile shape itself is fixed by the
-    pattern library the generic kernels are compiled with (tiles of 2^12
-    amplitudes, 3 register bits).  Writes "batch:<dtype>" to the cache."""
+    whole batch with those choices, then (complex64) the workgroup size,
+    512 or 256 threads (tiles of 2^12 or 2^11 amplitudes; 3 register bits,
+    the pattern library the generic kernels are compiled with).  Writes
+    "batch:<dtype>" to the cache."""
     import numpy as np
     import torch
 
@@ -215,10 +216,11 @@ def tune_batch(device, dtype=None, nc=512, seed=None, reps=3, log=print, out_pat
     ns = [int(x) for x in ns]
     ls = [int(x) for x in ls]
 
-    def timed(idx, remap, tps):
+    def timed(idx, remap, tps, threads=512):
         plan = BatchPlan([ns[i] for i in idx], [hea_ops(ns[i], ls[i]) for i in idx],
                          [tfim_terms(ns[i]) for i in idx], [len(thetas[i]) for i in idx], dtype,
-                         keys=[(ns[i], ls[i]) for i in idx], remap_adjoint=remap, tasks_per_step=tps)
+                         keys=[(ns[i], ls[i]) for i in idx], remap_adjoint=remap, tasks_per_step=tps,
+                         threads=threads)
         b = PersistentBatch(None, None, None, None, device=device, dtype=dtype, plan=plan)
         b.set_theta([thetas[i] for i in idx])
         b.capture()
@@ -244,9 +246,15 @@ def tune_batch(device, dtype=None, nc=512, seed=None, reps=3, log=print, out_pat
         tt[tps] = timed(full, remap, tps)
         log(f"[tune-batch] {nc} circuits, {tps} tasks per step: {tt[tps] * 1e3:.1f} ms")
     best = min(tt, key=tt.get)
+    threads, ms = 512, tt[best]
+    if dtype == torch.complex64:
+        t256 = timed(full, remap, best, 256)
+        log(f"[tune-batch] {nc} circuits, {best} tasks per step, 256-thread workgroups: {t256 * 1e3:.1f} ms")
+        if t256 < ms:
+            threads, ms = 256, t256
     cache = _load()
     cache[f"batch:{str(dtype).replace('torch.', '')}"] = dict(tasks_per_step=best, remap_adjoint=remap,
-                                                               ms=round(tt[best] * 1e3, 2))
+                                                               threads=threads, ms=round(ms * 1e3, 2))
     with open(out_path or CACHE, "w") as f:
         json.dump(cache, f, indent=1, sort_keys=True)
     return best, remap
