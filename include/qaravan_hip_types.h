@@ -63,7 +63,12 @@ typedef struct {
      * logical address bit q to physical bit perm[q]; the run ends with one
      * permutation back to the natural order (needs scratch for one state). */
     int64_t off_perm;
-    int64_t reserved;
+    /* 0, or (forward programs with off_perm set) the offset of int32 count
+     * (1..8) followed by count (control, target) pairs of physical address
+     * bits: CNOTs that run, in this order, after the last pass and before the
+     * final permutation, which applies them while it gathers (a trailing pass
+     * of CNOTs only costs no sweep of its own). */
+    int64_t off_cx;
 } svh_prog_header_t; /* 96 bytes */
 
 typedef struct {

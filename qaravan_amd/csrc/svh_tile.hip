@@ -21,6 +21,7 @@
 namespace svh {
 
 constexpr int kPermTileBits = 11;  // bit-permutation tiles: 2048 amplitudes
+constexpr int kMaxFinalCx = 8;     // CNOTs a program's final permutation may apply
 
 // ------------------------------------------------ the pass kernel
 // One tile of one pass: workgroup-local work shared by k_tile (one circuit,
@@ -525,6 +526,16 @@ static int check_program(const void *prog_host, int n, const svh_prog_header_t *
             seen |= 1ull << perm[q];
         }
     }
+    if (h->off_cx) {
+        if (!h->off_perm || mode == 1) return fail(-8, "final CNOTs need a final permutation (forward programs only)");
+        const int32_t *cx = reinterpret_cast<const int32_t *>(b + h->off_cx);
+        if (cx[0] < 1 || cx[0] > kMaxFinalCx) return fail(-8, "bad final CNOT count");
+        for (int k = 0; k < cx[0]; ++k) {
+            const int c = cx[1 + 2 * k], t = cx[2 + 2 * k];
+            if (c < 0 || c >= h->n_qubits || t < 0 || t >= h->n_qubits || c == t)
+                return fail(-8, "bad final CNOT bits");
+        }
+    }
     *hdr = h;
     return 0;
 }
@@ -543,6 +554,11 @@ struct PermArgs {
     int32_t ndst[64];             // their destination bits
     int32_t nnon;
     int32_t tbits;
+    // CNOTs applied before the permutation, in gate order: the source of
+    // destination element j is G1(G2(...Gk(j))) with Gi: a ^= bit(a, cxc[i]) << cxt[i]
+    int32_t ncx;
+    int32_t cxc[kMaxFinalCx];
+    int32_t cxt[kMaxFinalCx];
 };
 
 template <typename R>
@@ -572,7 +588,9 @@ __global__ void __launch_bounds__(256) k_permute_bits(const cpx<R> *__restrict__
         uint64_t so = so_t;
         for (int b = 8; b < TB; ++b) so |= (uint64_t)((r >> (b - 8)) & 1) << A.tsrc[b];
         const int e = r * 256 + tid;
-        tile[e ^ ((e >> 5) & 31)] = src[sb | so];
+        uint64_t a = sb | so;
+        for (int k = A.ncx - 1; k >= 0; --k) a ^= ((a >> A.cxc[k]) & 1ull) << A.cxt[k];
+        tile[e ^ ((e >> 5) & 31)] = src[a];
     }
     __syncthreads();
     for (int r = 0; r < per; ++r) {
@@ -587,7 +605,7 @@ __global__ void __launch_bounds__(256) k_permute_bits(const cpx<R> *__restrict__
     }
 }
 
-static int permute_args(int n, const int *perm, PermArgs *A) {
+static int permute_args(int n, const int *perm, PermArgs *A, int ncx = 0, const int32_t *cx = nullptr) {
     const int TB = n < kPermTileBits ? n : kPermTileBits;
     if (TB < 8) return fail(-2, "bit permutation needs n >= 8");
     std::vector<int> inv(n, -1);
@@ -603,8 +621,19 @@ static int permute_args(int n, const int *perm, PermArgs *A) {
             ++cnt;
         }
     };
+    if (ncx < 0 || ncx > kMaxFinalCx) return fail(-4, "too many final CNOTs");
+    for (int k = 0; k < ncx; ++k) {
+        const int c = cx[2 * k], t = cx[2 * k + 1];
+        if (c < 0 || c >= n || t < 0 || t >= n || c == t) return fail(-4, "bad final CNOT bits");
+        A->cxc[k] = c;
+        A->cxt[k] = t;
+    }
+    A->ncx = ncx;
     for (int b = 0; b < 4; ++b) add(b);
     for (int d = 0; d < 4; ++d) add(inv[d]);
+    // CNOT targets inside the tile keep every tile's source addresses within
+    // that tile (a CNOT only flips its target bit), so the gather stays coalesced
+    for (int k = 0; k < ncx; ++k) add(cx[2 * k + 1]);
     for (int b = 0; b < n && cnt < TB; ++b) add(b);
     int x = 0;
     for (int b = 0; b < n; ++b)
@@ -628,9 +657,10 @@ static int permute_args(int n, const int *perm, PermArgs *A) {
     return 0;
 }
 
-static int launch_permute(const void *src, void *dst, int n, int dtype, const int *perm, hipStream_t st) {
+static int launch_permute(const void *src, void *dst, int n, int dtype, const int *perm, hipStream_t st,
+                          int ncx = 0, const int32_t *cx = nullptr) {
     PermArgs A;
-    if (int e = permute_args(n, perm, &A)) return e;
+    if (int e = permute_args(n, perm, &A, ncx, cx)) return e;
     const unsigned grid = (unsigned)(1ull << (n - A.tbits));
     if (dtype == SVH_C64)
         k_permute_bits<float><<<grid, 256, 0, st>>>((const cpx<float> *)src, (cpx<float> *)dst, A);
@@ -943,8 +973,11 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
         const int32_t *perm = reinterpret_cast<const int32_t *>(hb + h->off_perm);
         std::vector<int> back(n);
         for (int q = 0; q < n; ++q) back[perm[q]] = q;
-        if (last_in_tmp) return launch_permute(perm_tmp, psi, n, dtype, back.data(), st);
-        if (int e = launch_permute(psi, perm_tmp, n, dtype, back.data(), st)) return e;
+        // CNOTs the program left to the permutation (count, then control/target pairs)
+        const int32_t *cx = h->off_cx ? reinterpret_cast<const int32_t *>(hb + h->off_cx) : nullptr;
+        const int ncx = cx ? cx[0] : 0;
+        if (last_in_tmp) return launch_permute(perm_tmp, psi, n, dtype, back.data(), st, ncx, cx ? cx + 1 : nullptr);
+        if (int e = launch_permute(psi, perm_tmp, n, dtype, back.data(), st, ncx, cx ? cx + 1 : nullptr)) return e;
         return hip_status(hipMemcpyAsync(psi, perm_tmp, perm_bytes, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync");
     }
     return 0;

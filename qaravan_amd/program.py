@@ -48,7 +48,8 @@ HDR_DT = np.dtype([('magic', '<u4'), ('version', '<i4'), ('n_qubits', '<i4'), ('
                    ('n_stages', '<i4'), ('n_ops', '<i4'), ('n_grad_slots', '<i4'), ('n_pool', '<i4'),
                    ('off_passes', '<i8'), ('off_stages', '<i8'), ('off_ops', '<i8'),
                    ('off_grad', '<i8'), ('off_pool', '<i8'), ('n_params', '<i8'),
-                   ('off_perm', '<i8'), ('reserved', '<i8')])
+                   ('off_perm', '<i8'), ('off_cx', '<i8')])
+MAX_FINAL_CX = 8
 PASS_DT = np.dtype([('kind', '<i4'), ('L', '<i4'), ('tb', '<i4'), ('rb', '<i4'),
                     ('stage_begin', '<i4'), ('n_stages', '<i4'), ('grad_begin', '<i4'),
                     ('n_grad', '<i4'), ('pool_off', '<i8'), ('flags', '<i4'), ('pad', '<i4'),
@@ -528,6 +529,7 @@ def partition_remap(ops, n, cfg):
                 if q not in wanted:
                     wanted.append(q)
         p = Pass(PASS_TILE, Ts, pops, tb=cfg.tb, rb=cfg.rb)
+        pos_in = list(pos)
         _stage_ops(p, cfg.io_lane_bits, keep_shared=cfg.lds_split, free_store=True)
         good = {Ts.index(pos[q]) for q in wanted if pos[q] in T}
         lanes = _remap_last_stage(p, good, cfg.lds_split, low)
@@ -556,6 +558,7 @@ def partition_remap(ops, n, cfg):
             dest[x] = ph
         for i, x in enumerate(lanes):
             dest[x] = i
+        p.pos_in = pos_in             # the layout this pass starts from
         if any(dest[x] != Ts[x] for x in range(L)):
             p.store_bits = [dest[x] for x in range(L)]
             p.store_lanes = lanes
@@ -743,6 +746,7 @@ class Program:
         self._sp = {}
         self.specialize = False
         self.final_pos = None             # remapped programs: physical bit of logical bit q at the end
+        self.final_cx = []                # (control, target) physical bits of CNOTs the final permutation applies
 
     def specialized(self, dtype_code, dual=False):
         """Loaded straight-line kernels of this program (compiled on first use)."""
@@ -777,7 +781,7 @@ class Program:
                     ops=sum(len(p.ops) for p in self.passes))
 
 
-def encode(n, passes, grad=False, n_params=0, final_pos=None):
+def encode(n, passes, grad=False, n_params=0, final_pos=None, final_cx=()):
     stage_rows, op_rows, grad_params, pool = [], [], [], []
     pass_rows = np.zeros(len(passes), dtype=PASS_DT)
     for pi, p in enumerate(passes):
@@ -873,13 +877,20 @@ def encode(n, passes, grad=False, n_params=0, final_pos=None):
     poola = np.array(pool, dtype=np.float64)
     perm = (np.array(final_pos, dtype=np.int32) if final_pos is not None and list(final_pos) != list(range(n))
             else np.zeros(0, np.int32))
+    if final_cx and not perm.size:
+        raise ValueError("final CNOTs need a final permutation")
+    if len(final_cx) > MAX_FINAL_CX:
+        raise ValueError(f"at most {MAX_FINAL_CX} final CNOTs")
+    # int32 count, then (control, target) pairs
+    cxa = (np.array([len(final_cx)] + [b for cx in final_cx for b in cx], dtype=np.int32) if final_cx
+           else np.zeros(0, np.int32))
 
     def align(x):
         return (x + 63) & ~63
     off = align(HDR_DT.itemsize)
     offs = {}
     for name, arr in (('passes', pass_rows), ('stages', stages), ('ops', opsa), ('grad', grads),
-                      ('pool', poola), ('perm', perm)):
+                      ('pool', poola), ('perm', perm), ('cx', cxa)):
         offs[name] = off
         off = align(off + arr.nbytes)
     blob = np.zeros(off, dtype=np.uint8)
@@ -892,13 +903,16 @@ def encode(n, passes, grad=False, n_params=0, final_pos=None):
         h['off_' + name] = offs[name]
     if not perm.size:
         h['off_perm'] = 0
+    if not cxa.size:
+        h['off_cx'] = 0
     blob[:HDR_DT.itemsize] = np.frombuffer(h.tobytes(), dtype=np.uint8)
     for name, arr in (('passes', pass_rows), ('stages', stages), ('ops', opsa), ('grad', grads),
-                      ('pool', poola), ('perm', perm)):
+                      ('pool', poola), ('perm', perm), ('cx', cxa)):
         if arr.nbytes:
             blob[offs[name]:offs[name] + arr.nbytes] = np.frombuffer(arr.tobytes(), dtype=np.uint8)
     prog = Program(n, passes, blob, grads, n_params)
     prog.final_pos = list(final_pos) if perm.size else None
+    prog.final_cx = [tuple(int(b) for b in cx) for cx in final_cx]
     return prog
 
 
@@ -948,12 +962,15 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
                      io_lane_bits=cfg.io_lane_bits, rb=cfg.rb, lds_split=split, remap=remap,
                      remap_free=cfg.remap_free, generic=cfg.generic)
     final_pos = None
+    final_cx = []
     if remap:
         passes, final_pos = partition_remap(ops, n, cfg)
         if grad:
             # an adjoint sweep only accumulates the gradient: the ψ and λ it
             # leaves behind are scratch, so no permutation back at the end
             final_pos = None
+        else:
+            passes, final_pos, final_cx = _fold_final_cx(passes, final_pos, n)
     else:
         passes = partition(ops, n, cfg)
     for p in passes:
@@ -967,9 +984,24 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
                 p.flags |= PASSF_QUARTER_LDS
             elif split >= 1 and shared >= 1:
                 p.flags |= PASSF_HALF_LDS
-    prog = encode(n, passes, grad, n_params, final_pos)
+    prog = encode(n, passes, grad, n_params, final_pos, final_cx)
     prog.specialize = bool(cfg.specialize)
     return prog
+
+
+def _fold_final_cx(passes, final_pos, n):
+    """A remapped program whose last tile pass holds only CNOTs (at most
+    MAX_FINAL_CX): those CNOTs are linear maps of the address bits, so the
+    final permutation applies them while it gathers (one sweep fewer).  The
+    program then ends in the layout the dropped pass started from.  Returns
+    (passes, final layout, [(control, target)] physical bits in gate order)."""
+    last = passes[-1] if passes else None
+    if (len(passes) < 2 or last.kind != PASS_TILE or not last.ops or len(last.ops) > MAX_FINAL_CX
+            or any(o.kind != 'cx' for o in last.ops) or getattr(last, 'pos_in', None) is None
+            or list(last.pos_in) == list(range(n))):
+        return passes, final_pos, []
+    # EOp cx bits are (target, control)
+    return passes[:-1], list(last.pos_in), [(o.bits[1], o.bits[0]) for o in last.ops]
 
 
 def _dense_bits(op):

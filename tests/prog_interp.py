@@ -107,6 +107,14 @@ def run(blob, sv, theta=None, lam=None, grad=None):
                 sv = permute_bits(sv, n, move)
                 if lam is not None:
                     lam = permute_bits(lam, n, move)
+    if int(h['off_cx']):
+        # CNOTs (control, target) on physical bits, in gate order, before the permutation
+        off = int(h['off_cx'])
+        cnt = int(np.frombuffer(blob[off:off + 4].tobytes(), dtype='<i4')[0])
+        pairs = np.frombuffer(blob[off + 4:off + 4 + 8 * cnt].tobytes(), dtype='<i4').reshape(cnt, 2)
+        idx = np.arange(2 ** n, dtype=np.int64)
+        for c, t in pairs:
+            sv = np.asarray(sv)[idx ^ (((idx >> int(c)) & 1) << int(t))]
     if int(h['off_perm']):
         off = int(h['off_perm'])
         perm = np.frombuffer(blob[off:off + 4 * n].tobytes(), dtype='<i4')

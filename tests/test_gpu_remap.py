@@ -80,3 +80,24 @@ def test_remapped_brickwork_24q_c64_vs_c128_and_inverse(gpu):
     z = StatevectorSim(inv, backend="hip", device=gpu, dtype=torch.complex128, tile=cfg)
     z.run(progress_bar=False)
     assert abs(z.get_statevector()[0].item() - 1) < 1e-11
+
+
+@pytest.mark.parametrize("n,tb", [(14, 6), (16, 7)])
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
+def test_trailing_cnots_in_final_permutation_vs_oracle(gpu, n, tb, dtype):
+    """Remapped programs whose trailing CNOT pass is folded into the final
+    permutation (the permutation kernel applies the CNOTs while it gathers)
+    give the oracle's state."""
+    from qaravan_amd import StatevectorSim
+    from test_program import _trailing_cx_circuits
+    cfg = P.TileConfig(tb=tb, low_bits=4, max_cost=1e9, specialize=True, io_lane_bits=4, rb=5, lds_split=1,
+                       remap=True)
+    folded = 0
+    for circ in _trailing_cx_circuits(n, range(6)):
+        sim = StatevectorSim(circ, backend="hip", device=gpu, dtype=dtype, tile=cfg)
+        sim.run(progress_bar=False)
+        folded += bool(sim._prog.final_cx)
+        err = np.linalg.norm(sim.get_statevector().cpu().numpy() - O.run(circ.gate_list, n))
+        print(f"  trailing cnots n={n} final_cx={sim._prog.final_cx} err={err:.2e}", flush=True)
+        assert err < (1e-12 if dtype == torch.complex128 else 1e-5)
+    assert folded > 0

@@ -468,6 +468,50 @@ def test_remap_programs_match_oracle(n, layers, tb, cap, seed):
         assert np.linalg.norm(sv - O.run(circ.gate_list, n)) < 1e-12
 
 
+def _trailing_cx_circuits(n, seeds):
+    """Brickwork and random circuits followed by a chain of CNOTs on far-apart
+    qubits (non-commuting: each shares a qubit with the next), so that some
+    remapped programs end in a pass of CNOTs only."""
+    out = []
+    for seed in seeds:
+        rng = np.random.default_rng(seed)
+        base = brickwork_circuit(n, 4) if seed % 2 == 0 else _random_tile_circuit(n, 60, rng)
+        qs = [int(x) for x in rng.permutation(n)[:5]]
+        cxs = [CNOT([qs[i], qs[i + 1]], n) for i in range(4)]
+        out.append(Circuit(list(base.gate_list) + cxs, n=n))
+    return out
+
+
+def test_trailing_cnots_fold_into_final_permutation():
+    """A remapped program whose last pass holds only CNOTs drops that pass
+    and hands its CNOTs (physical bits, gate order) to the final
+    permutation; the result is the oracle's state, and at least one of the
+    circuits folds."""
+    n = 12
+    cfg = P.TileConfig(tb=5, low_bits=4, max_cost=1e9, specialize=True, io_lane_bits=4, rb=5, lds_split=1,
+                       remap=True)
+    folded = 0
+    for circ in _trailing_cx_circuits(n, range(12)):
+        prog = compile_circ(circ, cfg)
+        h, passes, _, _, _, _ = prog_interp.decode(prog.blob)
+        if prog.final_cx:
+            folded += 1
+            assert int(h['off_cx']) and int(h['off_perm'])
+            assert 1 <= len(prog.final_cx) <= P.MAX_FINAL_CX
+        else:
+            assert int(h['off_cx']) == 0
+        sv0 = np.zeros(2 ** n, complex)
+        sv0[0] = 1
+        sv, _ = prog_interp.run(prog.blob, sv0)
+        assert np.linalg.norm(sv - O.run(circ.gate_list, n)) < 1e-12
+    assert folded > 0
+
+
+def test_final_cnots_need_a_permutation():
+    with pytest.raises(ValueError):
+        P.encode(10, [], final_pos=None, final_cx=[(0, 1)])
+
+
 def _random_tile_circuit(n, ng, rng):
     gates = []
     for _ in range(ng):
