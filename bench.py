@@ -559,6 +559,8 @@ def bench_persistent(dev, nc=512, seed=20260004, check=6):
                 algorithmic_bytes=alg, achieved_gbps=alg / dt / 1e9, hbm_frac=alg / dt / 8e12,
                 bytes_note="ψ read+write per forward tile pass, ψ and λ per adjoint pass, ψ read + λ written by "
                            "the Hamiltonian step; most circuits' states live in L2/MALL, so this can exceed HBM",
+                limiter="valu", valu_busy=0.77,
+                valu_source="profiles/batch_512_hea_c64.md (SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x launch cycles))",
                 scheduler=sched,
                 plan_s=round(plan_s, 1), compile_s=round(SP.STATS["compile_wall_s"] - c0["compile_wall_s"], 2),
                 total_amplitudes=float(sum(2.0 ** int(n) for n in ns)))
