@@ -61,8 +61,12 @@ class ParamProgram:
                 raise NotImplementedError("gates on > 2 qubits and resets are not differentiable here")
             if op.kind in ('rot1', 'rot2') and not 0 <= op.param < self.n_params:
                 raise ValueError(f"param index {op.param} outside [0, {self.n_params})")
+        # a remapped forward program leaves ψ in its own layout (no permutation
+        # back): λ = Hψ is formed in that layout with relabelled Pauli masks,
+        # ⟨ψ|λ⟩ does not depend on it, and the adjoint program starts from it
         self.fwd = compile_ops(ops, self.n, replace(cfg, tb=min(cfg.tb, self.n - cfg.rb), prefetch=False),
-                               n_params=self.n_params)
+                               n_params=self.n_params, keep_layout=True)
+        self.layout = self.fwd.end_pos    # physical bit of logical bit q after the forward program (None: natural)
         back = []
         for op in reversed(ops):
             d = op.dagger()
@@ -70,7 +74,7 @@ class ParamProgram:
                 d.flags |= OPF_GRAD
             back.append(d)
         self.adj = compile_ops(back, self.n, _adjoint_cfg(self.n, dtype, adj_cfg), grad=True,
-                               n_params=self.n_params)
+                               n_params=self.n_params, pos0=self.layout)
         self.n_gates = sum(1 for _ in ops)
 
 
@@ -87,11 +91,24 @@ def real_coefficients(terms):
     return np.array(out, dtype=np.float64)
 
 
-def term_table(n, terms):
-    """Device-layout Pauli-sum table: (xmask, zmask, Re, Im of c·i^{ny}) per term."""
+def _relabel_mask(m, layout):
+    out = 0
+    for b in range(len(layout)):
+        if (int(m) >> b) & 1:
+            out |= 1 << layout[b]
+    return out
+
+
+def term_table(n, terms, layout=None):
+    """Device-layout Pauli-sum table: (xmask, zmask, Re, Im of c·i^{ny}) per term.
+    ``layout``: the state's address-bit layout (physical bit of logical bit b),
+    whose bits the masks are moved to (default: the natural order)."""
     dt = np.dtype([('x', '<u8'), ('z', '<u8'), ('cre', '<f8'), ('cim', '<f8')])
     from .statevector_sim import _pauli_masks
     xm, zm = _pauli_masks(n, [p for _, p in terms])
+    if layout is not None:
+        xm = np.array([_relabel_mask(x, layout) for x in xm], dtype=np.uint64)
+        zm = np.array([_relabel_mask(z, layout) for z in zm], dtype=np.uint64)
     cs = real_coefficients(terms)
     t = np.zeros(len(terms), dtype=dt)
     ph = [(1.0, 0.0), (0.0, 1.0), (-1.0, 0.0), (0.0, -1.0)]
@@ -106,7 +123,8 @@ def _theta64(theta, device):
 
 
 def forward_state(pp, theta, device, scratch):
-    """ψ(θ) and λ = Hψ (device tensors) for a ParamProgram."""
+    """ψ(θ) and λ = Hψ (device tensors) for a ParamProgram, both in the
+    forward program's final layout ``pp.layout`` (None: the natural order)."""
     lib = _lib.load()
     n = pp.n
     th = _theta64(theta, device)
@@ -117,7 +135,7 @@ def forward_state(pp, theta, device, scratch):
     lam = torch.empty_like(psi)
     key = ("terms", str(device))
     if key not in pp.__dict__:
-        pp.__dict__[key] = torch.from_numpy(term_table(n, pp.terms).view(np.uint8).copy()).to(device)
+        pp.__dict__[key] = torch.from_numpy(term_table(n, pp.terms, pp.layout).view(np.uint8).copy()).to(device)
     tt = pp.__dict__[key]
     _lib.check(lib.svh_pauli_apply_table(_lib.ptr(psi), _lib.ptr(lam), n, _lib.dtype_code(pp.dtype), len(pp.terms),
                                          _lib.ptr(tt), _lib.stream_ptr(device)), "svh_pauli_apply_table")

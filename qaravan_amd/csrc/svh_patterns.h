@@ -2129,6 +2129,68 @@ __device__ __forceinline__ void pat3d_65(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << 
     gd_rot1<R, RB, 0, 3, false>(v, w, ops[2], trig, gout, lane0);
     { double gv[2] = {gq1, gq2}; gd_grad_group<1>(gv, gout, ops[1].grad_slot, 2); }
 }
+template <typename R, int RB>
+__device__ __forceinline__ void pat3d_66(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
+    ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
+    const double gq1 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[4], trig, gout, lane0);
+    const double gq5 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[5], trig, gout, lane0);
+    { double gv[8] = {gq1, gq2, gq3, gq4, gq5, 0.0, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[1].grad_slot, 5); }
+}
+template <typename R, int RB>
+__device__ __forceinline__ void pat3d_67(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
+    ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
+    ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
+    const double gq2 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[4], trig, gout, lane0);
+    ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
+    const double gq6 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[7], trig, gout, lane0);
+    { double gv[8] = {gq2, gq3, gq4, gq6, gq7, 0.0, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[2].grad_slot, 5); }
+}
+template <typename R, int RB>
+__device__ __forceinline__ void pat3d_68(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
+    ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
+    const double gq1 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[1], trig, gout, lane0);
+    const double gq2 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[4], trig, gout, lane0);
+    { double gv[4] = {gq1, gq2, gq3, gq4}; gd_grad_group<2>(gv, gout, ops[1].grad_slot, 4); }
+}
+template <typename R, int RB>
+__device__ __forceinline__ void pat3d_69(cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
+    ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
+    ap_cz<R, RB, 2, 0>(v); ap_cz<R, RB, 2, 0>(w);
+    const double gq2 = gr_rot1<R, RB, 2>(v, w, 3);
+    gd_rot1<R, RB, 2, 3, false>(v, w, ops[2], trig, gout, lane0);
+    const double gq3 = gr_rot1<R, RB, 1>(v, w, 2);
+    gd_rot1<R, RB, 1, 2, false>(v, w, ops[3], trig, gout, lane0);
+    const double gq4 = gr_rot1<R, RB, 0>(v, w, 2);
+    gd_rot1<R, RB, 0, 2, false>(v, w, ops[4], trig, gout, lane0);
+    ap_cz<R, RB, 1, 0>(v); ap_cz<R, RB, 1, 0>(w);
+    const double gq6 = gr_rot1<R, RB, 1>(v, w, 3);
+    gd_rot1<R, RB, 1, 3, false>(v, w, ops[6], trig, gout, lane0);
+    const double gq7 = gr_rot1<R, RB, 0>(v, w, 3);
+    gd_rot1<R, RB, 0, 3, false>(v, w, ops[7], trig, gout, lane0);
+    { double gv[8] = {gq2, gq3, gq4, gq6, gq7, 0.0, 0.0, 0.0}; gd_grad_group<3>(gv, gout, ops[2].grad_slot, 5); }
+}
 // pattern 3d: returns false when pid is not in the library
 template <typename R, int RB>
 __device__ __forceinline__ bool run_pattern3d(int pid, cpx<R> (&v)[1 << RB], cpx<R> (&w)[1 << RB], const svh_op_t *__restrict__ ops, const cpx<R> *__restrict__ trig, double *__restrict__ gout, bool lane0) {
@@ -2199,13 +2261,17 @@ __device__ __forceinline__ bool run_pattern3d(int pid, cpx<R> (&v)[1 << RB], cpx
         case 63: pat3d_63<R, RB>(v, w, ops, trig, gout, lane0); return true;
         case 64: pat3d_64<R, RB>(v, w, ops, trig, gout, lane0); return true;
         case 65: pat3d_65<R, RB>(v, w, ops, trig, gout, lane0); return true;
+        case 66: pat3d_66<R, RB>(v, w, ops, trig, gout, lane0); return true;
+        case 67: pat3d_67<R, RB>(v, w, ops, trig, gout, lane0); return true;
+        case 68: pat3d_68<R, RB>(v, w, ops, trig, gout, lane0); return true;
+        case 69: pat3d_69<R, RB>(v, w, ops, trig, gout, lane0); return true;
         default: return false;
     }
 }
 
-constexpr int kPatCount3d = 66;
-constexpr int kPatOff3d[] = {0, 6, 13, 19, 25, 29, 36, 47, 59, 68, 79, 82, 87, 99, 112, 130, 139, 150, 172, 185, 197, 205, 228, 242, 251, 260, 272, 285, 299, 309, 326, 347, 369, 379, 396, 417, 439, 446, 460, 478, 497, 500, 506, 513, 515, 520, 526, 530, 539, 551, 560, 573, 587, 594, 602, 609, 617, 623, 636, 650, 652, 657, 663, 671, 680, 683, 686};
-constexpr signed char kPatOps3d[][5] = {{4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 2, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 2, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 1, 0, 3, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 1, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 2, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 1, 0, 3, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 1, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 2, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 2, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 2, 1}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 2, 1}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 2, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}};
+constexpr int kPatCount3d = 70;
+constexpr int kPatOff3d[] = {0, 6, 13, 19, 25, 29, 36, 47, 59, 68, 79, 82, 87, 99, 112, 130, 139, 150, 172, 185, 197, 205, 228, 242, 251, 260, 272, 285, 299, 309, 326, 347, 369, 379, 396, 417, 439, 446, 460, 478, 497, 500, 506, 513, 515, 520, 526, 530, 539, 551, 560, 573, 587, 594, 602, 609, 617, 623, 636, 650, 652, 657, 663, 671, 680, 683, 686, 692, 700, 705, 713};
+constexpr signed char kPatOps3d[][5] = {{4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 2, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 2, 1, 0, 0}, {4, 1, 0, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 2, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 1, 0, 3, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 1, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 2, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 1, 0, 3, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 1, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 2, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {4, 2, 1, 0, 0}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 2, 0, 2, 1}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 2, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 2, 1}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 2, 1}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 2, 1}, {5, 2, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 2, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 2, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}, {4, 1, 0, 0, 0}, {5, 0, 0, 3, 1}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 1, 0, 3, 1}, {4, 1, 0, 0, 0}, {4, 2, 0, 0, 0}, {5, 2, 0, 3, 1}, {5, 1, 0, 2, 1}, {5, 0, 0, 2, 1}, {4, 1, 0, 0, 0}, {5, 1, 0, 3, 1}, {5, 0, 0, 3, 1}};
 
 // pattern 4f: returns false when pid is not in the library
 template <typename R, int RB>

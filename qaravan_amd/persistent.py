@@ -109,9 +109,8 @@ class BatchPlan:
             cfg, adj_cfg = batch_config(self.ns[c], dtype, 3, rm, self.threads)
             pp = ParamProgram(None, terms_list[c], n_params_list[c], cfg=cfg, adj_cfg=adj_cfg, dtype=dtype,
                               ops=op_lists[c], n=self.ns[c])
-            for prog in (pp.fwd, pp.adj):
-                if prog.final_pos is not None and prog is pp.fwd:
-                    raise ValueError("batched forward programs keep the natural bit order")
+            if pp.layout is not None or pp.fwd.final_pos is not None:
+                raise ValueError("batched forward programs keep the natural bit order")
             if key is not None:
                 cache[key] = pp
             self.pps.append(pp)

@@ -467,7 +467,7 @@ def partition(ops, n, cfg):
     return passes
 
 
-def partition_remap(ops, n, cfg):
+def partition_remap(ops, n, cfg, pos0=None):
     """Greedy packing like ``partition``, with qubits moving between address
     bits.  ``pos[q]`` is the physical address bit of logical bit q; ops are
     placed in passes with physical bits.  After each pass the store permutes
@@ -477,12 +477,13 @@ def partition_remap(ops, n, cfg):
     coalesced, the moved qubits come from the tile's low + rb lowest bits:
     then the last layout can hold every lower non-moving tile bit in
     registers and put the moved ones on lanes 0..low-1 in order.  Returns
-    (passes, pos) with pos the final layout."""
+    (passes, pos) with pos the final layout.  ``pos0``: the layout the
+    program starts from (default: the natural order)."""
     L = cfg.tb + cfg.rb
     low = min(cfg.low_bits, L)
     if L > n:
         raise ValueError(f"tile of {L} bits exceeds n={n}")
-    pos = list(range(n))
+    pos = list(pos0) if pos0 is not None else list(range(n))
     rem = list(ops)
     passes = []
     while rem:
@@ -747,6 +748,7 @@ class Program:
         self.specialize = False
         self.final_pos = None             # remapped programs: physical bit of logical bit q at the end
         self.final_cx = []                # (control, target) physical bits of CNOTs the final permutation applies
+        self.end_pos = None               # keep_layout programs: the layout the state is left in (None: natural)
 
     def specialized(self, dtype_code, dual=False):
         """Loaded straight-line kernels of this program (compiled on first use)."""
@@ -920,14 +922,27 @@ def min_tile_n(grad=False):
     return 8
 
 
-def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
-    """EOps -> Program.  Small registers (n < 8) use one dense pass per op."""
+def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True, keep_layout=False, pos0=None):
+    """EOps -> Program.  Small registers (n < 8) use one dense pass per op.
+
+    keep_layout: a remapped program ends in its own layout (``prog.end_pos``:
+    physical bit of logical bit q) instead of permuting back to the natural
+    order.  pos0: the layout the input state is in (physical bit of logical
+    bit q; default natural), e.g. the end_pos of the program that made it."""
     cfg = cfg or TileConfig()
+    if pos0 is not None:
+        pos0 = [int(x) for x in pos0]
+        if sorted(pos0) != list(range(n)):
+            raise ValueError("pos0 is not a permutation of the address bits")
+        if pos0 == list(range(n)):
+            pos0 = None
     if fuse:
         ops = fuse_1q(ops)
         if cfg.fuse2q:
             ops = fuse_2q(ops)
     if n < min_tile_n():
+        if pos0 is not None:
+            raise NotImplementedError("a permuted input layout needs n >= 8")
         if any(op.kind in ('rot1', 'rot2') for op in ops):
             raise NotImplementedError("device-angle rotations need n >= 8")
         passes = []
@@ -963,15 +978,21 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
                      remap_free=cfg.remap_free, generic=cfg.generic)
     final_pos = None
     final_cx = []
+    end_pos = pos0
     if remap:
-        passes, final_pos = partition_remap(ops, n, cfg)
-        if grad:
+        passes, final_pos = partition_remap(ops, n, cfg, pos0)
+        end_pos = final_pos
+        if grad or keep_layout:
             # an adjoint sweep only accumulates the gradient: the ψ and λ it
-            # leaves behind are scratch, so no permutation back at the end
+            # leaves behind are scratch; a program that keeps its layout hands
+            # it to the next one: no permutation back at the end
             final_pos = None
         else:
             passes, final_pos, final_cx = _fold_final_cx(passes, final_pos, n)
     else:
+        if pos0 is not None:
+            # no remapping: the ops act on the physical bits of the input layout
+            ops = [EOp(o.kind, tuple(pos0[b] for b in o.bits), o.mat, o.param, o.pauli, o.flags) for o in ops]
         passes = partition(ops, n, cfg)
     for p in passes:
         if p.kind == PASS_TILE:
@@ -986,6 +1007,8 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True):
                 p.flags |= PASSF_HALF_LDS
     prog = encode(n, passes, grad, n_params, final_pos, final_cx)
     prog.specialize = bool(cfg.specialize)
+    if keep_layout and end_pos is not None and list(end_pos) != list(range(n)):
+        prog.end_pos = list(end_pos)
     return prog
 
 

@@ -61,7 +61,9 @@ def test_generic_adjoint_programs_interpret_to_oracle(n, layers, tb, rb, remap):
     used = _check_ids(pp.fwd, False) + _check_ids(pp.adj, True)
     assert (used > 0) == (rb == 3)
     theta = hea_theta(n, layers)
-    sv, _ = prog_interp.run(pp.fwd.blob, O.init_state(n), theta)      # ends in natural bit order
+    sv, _ = prog_interp.run(pp.fwd.blob, O.init_state(n), theta)      # ends in the program's layout
+    if pp.layout is not None:
+        sv = prog_interp.permute_bits(sv, n, {pp.layout[q]: q for q in range(n)})
     spec = []
     for l in range(layers):
         for k, w in enumerate("yz"):

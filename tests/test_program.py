@@ -217,7 +217,7 @@ def test_param_forward_and_adjoint_programs(tb, layers, cost, rb, remap):
     are diagonal, so the scheduler may reorder them).  With remap, both
     programs move qubits between address bits; the forward permutes back at
     the end, the adjoint sweep leaves ψ and λ permuted."""
-    from qaravan_amd.adjoint import ParamProgram
+    from qaravan_amd.adjoint import ParamProgram, term_table
     n = 10
     theta = hea_theta(n, layers)
     circ = hea_circuit(n, layers, theta)
@@ -228,16 +228,35 @@ def test_param_forward_and_adjoint_programs(tb, layers, cost, rb, remap):
     if remap:
         assert any(p.flags & P.PASSF_REMAP_STORE for p in pp.adj.passes)
         assert pp.adj.final_pos is None
+        # the forward leaves ψ in its own layout, with no permutation back
+        assert pp.layout is not None and pp.fwd.final_pos is None
+    else:
+        assert pp.layout is None
     assert all(p.rb == rb for p in pp.adj.passes if p.kind == P.PASS_TILE)
     psi, _ = prog_interp.run(pp.fwd.blob, O.init_state(n), theta=theta)
     ref = O.run(circ.gate_list, n)
-    assert np.linalg.norm(psi - ref) < 1e-12
-    lam = O.apply_pauli_sum(psi, terms)
+    to_nat = {} if pp.layout is None else {pp.layout[q]: q for q in range(n)}
+    assert np.linalg.norm(prog_interp.permute_bits(psi, n, to_nat) - ref) < 1e-12
+    # λ = Hψ in the same layout, as the device forms it from the relabelled term table
+    lam = prog_interp.permute_bits(O.apply_pauli_sum(ref, terms), n, {q: p for p, q in to_nat.items()})
+    assert np.allclose(lam, _apply_terms_table(term_table(n, terms, pp.layout), psi), atol=1e-12)
     grad = np.zeros(len(theta))
     prog_interp.run(pp.adj.blob, psi, theta=theta, lam=lam, grad=grad)
     from test_oracle import hea_spec
     _, gref = O.adjoint_gradient(hea_spec(n, layers), theta, n, terms)
     assert np.allclose(grad, gref, atol=1e-10)
+
+
+def _apply_terms_table(tab, psi):
+    """λ_i = Σ_t c_t i^{ny} (-1)^{pc((i^x)&z)} ψ[i^x] from a device term table
+    (the kernel's formula, restated), for checking relabelled tables."""
+    idx = np.arange(len(psi), dtype=np.int64)
+    lam = np.zeros_like(psi)
+    for t in tab:
+        j = idx ^ int(t['x'])
+        sg = 1 - 2 * (np.array([bin(v).count("1") for v in (j & int(t['z']))]) & 1)
+        lam += (t['cre'] + 1j * t['cim']) * sg * psi[j]
+    return lam
 
 
 def _diag_rich_circuit(n, n_gates, rng):
