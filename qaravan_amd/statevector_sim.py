@@ -197,7 +197,7 @@ def inner(a, b, n, scratch):
     buf = scratch.get(16 * 1024)
     _lib.check(lib.svh_inner(_lib.ptr(a), _lib.ptr(b), n, _lib.dtype_code(a.dtype), _lib.ptr(out),
                              _lib.ptr(buf), buf.numel(), _lib.stream_ptr(a.device)), "svh_inner")
-    return torch.complex(out[0], out[1])
+    return out.view(torch.complex128)[0]     # (re, im) read as one complex128: no kernel
 
 
 _BACKENDS = ("numpy", "torch", "hip")
