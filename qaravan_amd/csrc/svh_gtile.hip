@@ -138,7 +138,6 @@ extern "C" int svh_batch_persistent(const void *descs_dev, const void *tasks_dev
         return fail(-1, "NULL pointer");
     if (nq < 1 || ncounters < 0) return fail(-4, "bad queue or counter count");
     if (dtype != SVH_C64 && dtype != SVH_C128) return fail(-3, "bad dtype");
-    if (lds_bytes > 159 * 1024) return fail(-7, "persistent batch needs more than 159 KiB of LDS");
     // workgroup size: 512 threads (two resident per CU), or for complex64 256
     // (four per CU, tiles of at most 2^11 amplitudes)
     if (threads == 0) threads = 512;
@@ -151,11 +150,11 @@ extern "C" int svh_batch_persistent(const void *descs_dev, const void *tasks_dev
     const size_t lam_bytes = dtype == SVH_C64 ? (size_t)threads * 8 * 8 : (size_t)threads * 4 * 16;
     if (lds_bytes > 2 * lam_bytes)
         return fail(-7, "persistent batch: tiles larger than the workgroup's LDS (λ offset)");
-    const size_t need = 0;
+    const size_t dyn_lds = 0;
     const void *kern = dtype == SVH_C128 ? persist_kernel_d3() : small ? persist_kernel_f3_256() : persist_kernel_f3();
     int grid = 0;
     uint32_t xcc_mask = 0;
-    if (int e = persist_grid(kern, threads, need, 2 * lam_bytes, &grid, &xcc_mask)) return e;
+    if (int e = persist_grid(kern, threads, dyn_lds, 2 * lam_bytes, &grid, &xcc_mask)) return e;
     if (max_workgroups > 0 && grid > max_workgroups) grid = max_workgroups;
     // XCD-local synchronisation only when the full grid covers exactly the
     // XCDs 0..nq-1, so every queue is served from inside its own XCD
@@ -172,6 +171,6 @@ extern "C" int svh_batch_persistent(const void *descs_dev, const void *tasks_dev
     const auto *T = static_cast<const svh_ptask_t *>(tasks_dev);
     auto *S = (unsigned long long *)stats_dev;
     auto launch = dtype == SVH_C128 ? persist_launch_d3 : small ? persist_launch_f3_256 : persist_launch_f3;
-    return launch((unsigned)grid, (unsigned)threads, need, st, D, T, qbeg_dev, nq, heads_dev, counters_dev, err_dev,
-                  mode, S);
+    return launch((unsigned)grid, (unsigned)threads, dyn_lds, st, D, T, qbeg_dev, nq, heads_dev, counters_dev,
+                  err_dev, mode, S);
 }
