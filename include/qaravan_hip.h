@@ -23,6 +23,7 @@
  *   svh_pauli_expect      tensorQ/statevector_sim.py:217-234  pauli_expectation_sv, summed
  *                         over Hamiltonian terms (core/hamiltonians.py:19-88)
  *   svh_pauli_apply       .dev/test_hamiltonians.ipynb:709-712 ham_action (H·ψ via op_action)
+ *   svh_pauli_apply_inner the same H·ψ with ⟨ψ|H·ψ⟩ (the energy a gradient step needs) in one sweep
  *   svh_inner             tensorQ/statevector_sim.py:155-180 partial_overlap with no kept
  *                         sites, i.e. ⟨a|b⟩ (applications/compilation.py:86 torch.dot)
  *   svh_run_adjoint       applications/compilation.py:73-133 (cost.backward() through the
@@ -232,6 +233,14 @@ int svh_timestamp(uint64_t *out_dev, void *stream);
  * copies, so it can be recorded in a HIP graph. */
 int svh_pauli_apply_table(const void *psi, void *out, int n, int dtype, int n_terms, const void *terms_dev,
                           void *stream);
+
+/* svh_pauli_apply_table and ⟨ψ|Hψ⟩ in one sweep: out = H·ψ as above, and
+ * out_dev[0..1] = (re, im) of Σ_i conj(ψ_i)·(Hψ)_i, accumulated in FP64 from
+ * the rows before they are rounded to the state precision (the loss value of
+ * applications/compilation.py-style energies, tensorQ/statevector_sim.py:217-234
+ * summed over the Hamiltonian's terms).  scratch >= 16 KiB. */
+int svh_pauli_apply_inner(const void *psi, void *out, int n, int dtype, int n_terms, const void *terms_dev,
+                          double *out_dev, void *scratch, size_t scratch_bytes, void *stream);
 
 /* out_dev[0..1] = ⟨a|b⟩ (re, im) accumulated in FP64. */
 int svh_inner(const void *a, const void *b, int n, int dtype, double *out_dev,

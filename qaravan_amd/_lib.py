@@ -52,6 +52,8 @@ SIGNATURES = {
     "svh_pauli_apply": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_void_p,
                                  _c_void_p, _c_void_p, _c_size_t, _c_void_p]),
     "svh_pauli_apply_table": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_void_p]),
+    "svh_pauli_apply_inner": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_void_p,
+                                       _c_void_p, _c_size_t, _c_void_p]),
     "svh_inner": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_size_t,
                            _c_void_p]),
     "svh_batch_tile": (_c_int, [_c_void_p, _c_int, ctypes.c_int64, _c_int, _c_int, _c_int, _c_int, _c_void_p]),

@@ -23,7 +23,7 @@ import torch
 
 from . import _lib
 from .program import OPF_GRAD, TileConfig, compile_ops, lower_circuit
-from .statevector_sim import DeviceScratch, inner, run_program
+from .statevector_sim import DeviceScratch, run_program
 
 
 def _adjoint_cfg(n, dtype, cfg):
@@ -123,8 +123,9 @@ def _theta64(theta, device):
 
 
 def forward_state(pp, theta, device, scratch):
-    """ψ(θ) and λ = Hψ (device tensors) for a ParamProgram, both in the
-    forward program's final layout ``pp.layout`` (None: the natural order)."""
+    """ψ(θ), λ = Hψ (device tensors, both in the forward program's final layout
+    ``pp.layout``; None: the natural order), the float64 θ and the energy
+    ⟨ψ|H|ψ⟩ (0-d float64 device tensor, FP64 sums in the same sweep as λ)."""
     lib = _lib.load()
     n = pp.n
     th = _theta64(theta, device)
@@ -137,9 +138,12 @@ def forward_state(pp, theta, device, scratch):
     if key not in pp.__dict__:
         pp.__dict__[key] = torch.from_numpy(term_table(n, pp.terms, pp.layout).view(np.uint8).copy()).to(device)
     tt = pp.__dict__[key]
-    _lib.check(lib.svh_pauli_apply_table(_lib.ptr(psi), _lib.ptr(lam), n, _lib.dtype_code(pp.dtype), len(pp.terms),
-                                         _lib.ptr(tt), _lib.stream_ptr(device)), "svh_pauli_apply_table")
-    return psi, lam, th
+    out = torch.empty(2, dtype=torch.float64, device=device)
+    buf = scratch.get(16 * 1024)
+    _lib.check(lib.svh_pauli_apply_inner(_lib.ptr(psi), _lib.ptr(lam), n, _lib.dtype_code(pp.dtype), len(pp.terms),
+                                         _lib.ptr(tt), _lib.ptr(out), _lib.ptr(buf), buf.numel(),
+                                         _lib.stream_ptr(device)), "svh_pauli_apply_inner")
+    return psi, lam, th, out[0]
 
 
 def adjoint_sweep(pp, psi, lam, th, scratch):
@@ -171,8 +175,7 @@ class AdjointEnergy(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, theta, pp, scratch):
-        psi, lam, th = forward_state(pp, theta, theta.device, scratch)
-        e = inner(psi, lam, pp.n, scratch).real
+        psi, lam, th, e = forward_state(pp, theta, theta.device, scratch)
         ctx.pp, ctx.psi, ctx.lam, ctx.th, ctx.scratch = pp, psi, lam, th, scratch
         ctx.theta_dtype = theta.dtype
         return e.to(torch.float64)
@@ -202,8 +205,7 @@ class AdjointModel:
 
     def energy_and_grad(self, theta):
         """(E, dE/dθ) without building an autograd graph."""
-        psi, lam, th = forward_state(self.pp, theta, self.device, self.scratch)
-        e = inner(psi, lam, self.pp.n, self.scratch).real
+        psi, lam, th, e = forward_state(self.pp, theta, self.device, self.scratch)
         g = adjoint_sweep(self.pp, psi, lam, th, self.scratch)
         return e, g
 

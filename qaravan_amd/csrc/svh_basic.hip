@@ -237,6 +237,39 @@ __global__ void __launch_bounds__(256) k_pauli_expect(const cpx<R> *__restrict__
     block_reduce_store<1>(v, part);
 }
 
+// Row i of H·ψ: Σ_t c_t i^{ny} (-1)^{pc((i^x)&z)} ψ[i^x], accumulated in FP64
+// over the terms in table order, 4 terms at a time (their records and gathers
+// in flight together).
+template <typename R>
+__device__ __forceinline__ void pauli_row(const cpx<R> *__restrict__ psi, uint64_t i,
+                                          const PauliTerm *__restrict__ terms, int nt, double &ar, double &ai) {
+    ar = 0.0;
+    ai = 0.0;
+    int t = 0;
+    for (; t + 4 <= nt; t += 4) {
+        PauliTerm T[4];
+        cpx<R> b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) T[u] = terms[t + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) b[u] = psi[i ^ T[u].x];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double s = parity_sign((i ^ T[u].x) & T[u].z);
+            ar += s * (T[u].cre * b[u].x - T[u].cim * b[u].y);
+            ai += s * (T[u].cre * b[u].y + T[u].cim * b[u].x);
+        }
+    }
+    for (; t < nt; ++t) {
+        const PauliTerm T = terms[t];
+        const uint64_t j = i ^ T.x;
+        const cpx<R> b = psi[j];
+        const double s = parity_sign(j & T.z);
+        ar += s * (T.cre * b.x - T.cim * b.y);
+        ai += s * (T.cre * b.y + T.cim * b.x);
+    }
+}
+
 // out[i] = Σ_t c_t i^{ny} (-1)^{pc((i^x)&z)} ψ[i^x]
 template <typename R>
 __global__ void __launch_bounds__(256) k_pauli_apply(const cpx<R> *__restrict__ psi,
@@ -244,34 +277,30 @@ __global__ void __launch_bounds__(256) k_pauli_apply(const cpx<R> *__restrict__ 
                                                      const PauliTerm *__restrict__ terms, int nt) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
-        double ar = 0.0, ai = 0.0;
-        int t = 0;
-        // 4 terms at a time: their records and gathers are in flight together
-        // (the sum still runs over the terms in table order)
-        for (; t + 4 <= nt; t += 4) {
-            PauliTerm T[4];
-            cpx<R> b[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) T[u] = terms[t + u];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) b[u] = psi[i ^ T[u].x];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const double s = parity_sign((i ^ T[u].x) & T[u].z);
-                ar += s * (T[u].cre * b[u].x - T[u].cim * b[u].y);
-                ai += s * (T[u].cre * b[u].y + T[u].cim * b[u].x);
-            }
-        }
-        for (; t < nt; ++t) {
-            const PauliTerm T = terms[t];
-            const uint64_t j = i ^ T.x;
-            const cpx<R> b = psi[j];
-            const double s = parity_sign(j & T.z);
-            ar += s * (T.cre * b.x - T.cim * b.y);
-            ai += s * (T.cre * b.y + T.cim * b.x);
-        }
+        double ar, ai;
+        pauli_row<R>(psi, i, terms, nt, ar, ai);
         out[i] = cpx<R>{(R)ar, (R)ai};
     }
+}
+
+// The same, and the block's partial of ⟨ψ|Hψ⟩ (re, im) from the FP64 rows
+// before they are rounded to the state precision.
+template <typename R>
+__global__ void __launch_bounds__(256) k_pauli_apply_inner(const cpx<R> *__restrict__ psi,
+                                                           cpx<R> *__restrict__ out, uint64_t N,
+                                                           const PauliTerm *__restrict__ terms, int nt,
+                                                           double *__restrict__ part) {
+    double v[2] = {0.0, 0.0};
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+        double ar, ai;
+        pauli_row<R>(psi, i, terms, nt, ar, ai);
+        out[i] = cpx<R>{(R)ar, (R)ai};
+        const cpx<R> a = psi[i];
+        v[0] += (double)a.x * ar + (double)a.y * ai;  // Re(conj(ψ_i) λ_i)
+        v[1] += (double)a.x * ai - (double)a.y * ar;  // Im(conj(ψ_i) λ_i)
+    }
+    block_reduce_store<2>(v, part);
 }
 
 }  // namespace svh
@@ -470,6 +499,28 @@ int svh_pauli_apply_table(const void *psi, void *out, int n, int dtype, int n_te
     else
         k_pauli_apply<double><<<g, 256, 0, st>>>((const cpx<double> *)psi, (cpx<double> *)out, N, terms, n_terms);
     return checked_launch("svh_pauli_apply_table");
+}
+
+int svh_pauli_apply_inner(const void *psi, void *out, int n, int dtype, int n_terms, const void *terms_dev,
+                          double *out_dev, void *scratch, size_t scratch_bytes, void *stream) {
+    if (int e = check_state(psi, n, dtype)) return e;
+    if (!out || !out_dev || !scratch || (n_terms > 0 && !terms_dev)) return fail(-1, "NULL pointer");
+    if (out == psi) return fail(-4, "out must not alias psi");
+    if (n_terms < 0 || n_terms > 4096) return fail(-4, "n_terms must be in [0, 4096]");
+    if (scratch_bytes < 2 * kReduceBlocks * sizeof(double)) return fail(-5, "scratch < 16 KiB");
+    const uint64_t N = 1ull << n;
+    const unsigned g = grid_for(N, 256, kReduceBlocks);
+    hipStream_t st = (hipStream_t)stream;
+    const auto *terms = static_cast<const PauliTerm *>(terms_dev);
+    double *part = (double *)scratch;
+    if (dtype == SVH_C64)
+        k_pauli_apply_inner<float><<<g, 256, 0, st>>>((const cpx<float> *)psi, (cpx<float> *)out, N, terms,
+                                                      n_terms, part);
+    else
+        k_pauli_apply_inner<double><<<g, 256, 0, st>>>((const cpx<double> *)psi, (cpx<double> *)out, N, terms,
+                                                       n_terms, part);
+    k_final_sum<2><<<1, 256, 0, st>>>(part, (int)g, out_dev);
+    return checked_launch("svh_pauli_apply_inner");
 }
 
 }  // extern "C"

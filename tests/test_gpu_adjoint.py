@@ -126,8 +126,10 @@ def test_adjoint_20q_c64_vs_c128_and_fd(gpu):
     bias = np.sin(2 * eps) / (2 * eps)
 
     def energy(t):
-        psi, lam, _ = forward_state(m128.pp, t, gpu, m128.scratch)
-        return inner(psi, lam, n, m128.scratch).real.item()
+        psi, lam, _, e = forward_state(m128.pp, t, gpu, m128.scratch)
+        # the fused energy equals ⟨ψ|λ⟩ taken separately
+        assert abs(e.item() - inner(psi, lam, n, m128.scratch).real.item()) < 1e-12
+        return e.item()
     for i in (0, 33, 321, 639):
         tp, tm = th.clone(), th.clone()
         tp[i] += eps
