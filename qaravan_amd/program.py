@@ -936,6 +936,9 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True, keep_layout
             raise ValueError("pos0 is not a permutation of the address bits")
         if pos0 == list(range(n)):
             pos0 = None
+        elif not (grad or keep_layout):
+            # nothing would bring such a state back to the natural order
+            raise ValueError("a permuted input layout (pos0) needs keep_layout or an adjoint program")
     if fuse:
         ops = fuse_1q(ops)
         if cfg.fuse2q:
