@@ -549,3 +549,29 @@ def test_remap_falls_back_without_tile_ops():
     prog = compile_circ(circ, cfg)
     assert prog.final_pos is None
     assert not any(p.flags & P.PASSF_REMAP_STORE for p in prog.passes)
+
+
+@pytest.mark.parametrize("remap2", [True, False])
+def test_layout_handoff_between_programs(remap2):
+    """A remapped program compiled with keep_layout leaves the state in its
+    final layout (end_pos, no permutation back); a second program compiled
+    from that layout (pos0) continues the circuit; together they give the
+    oracle's state once the bits are moved back by the last layout."""
+    n = 12
+    c1 = brickwork_circuit(n, 3)
+    rng = np.random.default_rng(3)
+    c2 = _random_tile_circuit(n, 40, rng)
+    cfg = P.TileConfig(tb=5, low_bits=4, max_cost=1e9, specialize=True, io_lane_bits=4, rb=5, lds_split=1,
+                       remap=True)
+    p1 = P.compile_ops(P.lower_circuit(c1.build(), n), n, cfg, keep_layout=True)
+    assert p1.end_pos is not None and p1.final_pos is None
+    cfg2 = cfg if remap2 else P.TileConfig(tb=5, low_bits=4, max_cost=1e9, rb=4, io_lane_bits=4)
+    p2 = P.compile_ops(P.lower_circuit(c2.build(), n), n, cfg2, keep_layout=True, pos0=p1.end_pos)
+    end = p2.end_pos or list(range(n))
+    sv, _ = prog_interp.run(p1.blob, O.init_state(n))
+    sv, _ = prog_interp.run(p2.blob, sv)
+    sv = prog_interp.permute_bits(sv, n, {end[q]: q for q in range(n)})
+    ref = O.run(list(c1.gate_list) + list(c2.gate_list), n)
+    assert np.linalg.norm(sv - ref) < 1e-12
+    with pytest.raises(ValueError):
+        P.compile_ops(P.lower_circuit(c2.build(), n), n, cfg2, pos0=p1.end_pos)
