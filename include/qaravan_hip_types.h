@@ -108,7 +108,11 @@ enum {
      * pattern library or interpreted); 3 or 4 register bits, tb >= 6.  In an
      * adjoint sweep every wave writes its own row of gradient partials, so a
      * pass has 2^(n-L) * 2^(tb-6) rows of n_grad doubles. */
-    SVH_PASSF_GENERIC = 16
+    SVH_PASSF_GENERIC = 16,
+    /* TILE, first pass of a forward program, per-pass compiled kernel only:
+     * the pass does not read the state; it starts from |0…0⟩, generated in
+     * its registers (the caller skips the initialisation sweep). */
+    SVH_PASSF_GEN_ZERO = 32
 };
 
 /* A stage's layout: thread t holds registers j = 0..2^rb-1 at tile-local index

@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .program import OPF_GRAD, TileConfig, compile_ops, lower_circuit
+from .program import OPF_GRAD, TileConfig, compile_ops, lower_circuit, zero_start_variant
 from .statevector_sim import DeviceScratch, run_program
 
 
@@ -43,7 +43,8 @@ def _adjoint_cfg(n, dtype, cfg):
 class ParamProgram:
     """Forward and adjoint programs of a parametrised circuit plus its loss."""
 
-    def __init__(self, circ, terms, n_params, cfg=None, adj_cfg=None, dtype=torch.complex64, ops=None, n=None):
+    def __init__(self, circ, terms, n_params, cfg=None, adj_cfg=None, dtype=torch.complex64, ops=None, n=None,
+                 start_zero=False):
         """``circ``: a qaravan-style Circuit whose rotations carry ``param_index``;
         or ``circ=None`` with ``ops`` (elementary ops, e.g. workloads.hea_ops) and ``n``."""
         self.n = circ.num_sites if circ is not None else int(n)
@@ -67,6 +68,9 @@ class ParamProgram:
         self.fwd = compile_ops(ops, self.n, replace(cfg, tb=min(cfg.tb, self.n - cfg.rb), prefetch=False),
                                n_params=self.n_params, keep_layout=True)
         self.layout = self.fwd.end_pos    # physical bit of logical bit q after the forward program (None: natural)
+        # start_zero: the forward's first pass generates |0…0⟩ itself when it
+        # has a compiled kernel (fwd0; the caller skips the initialisation)
+        self.fwd0 = zero_start_variant(self.fwd, _lib.dtype_code(dtype)) if start_zero else None
         back = []
         for op in reversed(ops):
             d = op.dagger()
@@ -130,9 +134,12 @@ def forward_state(pp, theta, device, scratch):
     n = pp.n
     th = _theta64(theta, device)
     psi = torch.empty(2 ** n, dtype=pp.dtype, device=device)
-    _lib.check(lib.svh_init_basis(_lib.ptr(psi), n, _lib.dtype_code(pp.dtype), 0, _lib.stream_ptr(device)),
-               "svh_init_basis")
-    run_program(psi, pp.fwd, scratch, th)
+    if pp.fwd0 is not None:
+        run_program(psi, pp.fwd0, scratch, th)       # its first pass generates |0…0⟩
+    else:
+        _lib.check(lib.svh_init_basis(_lib.ptr(psi), n, _lib.dtype_code(pp.dtype), 0, _lib.stream_ptr(device)),
+                   "svh_init_basis")
+        run_program(psi, pp.fwd, scratch, th)
     lam = torch.empty_like(psi)
     key = ("terms", str(device))
     if key not in pp.__dict__:
@@ -197,7 +204,7 @@ class AdjointModel:
     def __init__(self, circ, terms, n_params, device="cuda", dtype=torch.complex64, cfg=None):
         _lib.require_gpu()
         self.device = torch.device(device)
-        self.pp = ParamProgram(circ, terms, n_params, cfg=cfg, dtype=dtype)
+        self.pp = ParamProgram(circ, terms, n_params, cfg=cfg, dtype=dtype, start_zero=True)
         self.scratch = DeviceScratch(self.device)
 
     def __call__(self, theta):

@@ -433,6 +433,9 @@ static int check_program(const void *prog_host, int n, const svh_prog_header_t *
                             (P.flags & (SVH_PASSF_HALF_LDS | SVH_PASSF_QUARTER_LDS | SVH_PASSF_PREFETCH))))
                 return fail(-8, "tile pass " + std::to_string(p) +
                                     ": generic passes need rb 3 or 4, 6 <= tb <= 9, no LDS split or prefetch");
+            if ((P.flags & SVH_PASSF_GEN_ZERO) && (p != 0 || mode == 1 || generic))
+                return fail(-8, "tile pass " + std::to_string(p) +
+                                    ": only the first pass of a forward program may generate |0...0>");
             const int extra = (P.flags & SVH_PASSF_REMAP_STORE) ? 1 : 0;  // the store record
             if (P.n_stages < 1 || P.stage_begin < 0 || P.stage_begin + P.n_stages + extra > h->n_stages)
                 return fail(-8, "tile pass " + std::to_string(p) + ": bad stage range");
@@ -910,6 +913,10 @@ static int run_impl(void *psi, void *lam, int n, int dtype, const void *prog_hos
             e = gtile_launch(psi, out, dual ? lam : nullptr, n, dtype, P, dpass + p, dstage + P.stage_begin, dop,
                              trig, gp, st);
             if (out != psi) last_in_tmp = true;
+        } else if ((P.flags & SVH_PASSF_GEN_ZERO) && !(P.kind == SVH_PASS_TILE && sp && sp->fn[p])) {
+            // only the pass's compiled kernel generates |0...0>; any other
+            // kernel would read a state the caller did not initialise
+            return fail(-8, "pass 0 generates |0...0> and runs only in its compiled kernel (svh_run_program_sp)");
         } else if (P.kind == SVH_PASS_TILE && sp && sp->fn[p]) {
             // a remapped program's last compiled pass stores into the scratch
             // tail, which the final permutation reads

@@ -458,6 +458,16 @@ __device__ __forceinline__ void sp_load(const cpx<R> *__restrict__ psi, uint64_t
     for (int j = 0; j < (1 << RB); ++j) v[j] = p[S.goff[j]];
 }
 
+// |0…0⟩ in the registers of the first layout, in place of sp_load: only
+// amplitude 0 (base, thread part and register offset all zero; their bits are
+// disjoint) is 1.
+template <typename R, int TB, int RB = 4>
+__device__ __forceinline__ void sp_gen_zero(uint64_t base, const svh_stage_t &S, cpx<R> (&v)[1 << RB]) {
+    const uint64_t t = base | thr_global<TB>(S, threadIdx.x);
+#pragma unroll
+    for (int j = 0; j < (1 << RB); ++j) v[j] = cpx<R>{(t | (uint64_t)S.goff[j]) == 0 ? (R)1 : (R)0, (R)0};
+}
+
 template <typename R, int TB, int RB = 4>
 __device__ __forceinline__ void sp_store(cpx<R> *__restrict__ psi, uint64_t base, const svh_stage_t &S,
                                          const cpx<R> (&v)[1 << RB]) {

@@ -14,6 +14,8 @@ Workloads covered:
   layers);
 * the 20-qubit, 16-layer hardware-efficient ansatz gradient (forward and
   adjoint programs of the tuned adjoint configuration), complex64;
+* for the ``StatevectorSim`` programs, also the variant whose first pass
+  generates |0…0⟩ (what ``run()`` executes from the default initial state);
 * the reference's published GPU workload, two_local_circ(brickwall_skeleton(n, 1))
   at n = 20 ... 26, where ``StatevectorSim`` switches to the tuned kernels.
 """
@@ -47,13 +49,20 @@ def _programs():
     n = 20
     # the program AdjointModel builds from the circuit (the benchmark's path)
     pp = ParamProgram(hea_circuit(n, 16, hea_theta(n, 16)), tfim_terms(n), 2 * n * 16,
-                      cfg=best_adjoint_config(n, c64), dtype=c64)
-    out.append(("HEA 20q x 16 forward", pp.fwd, False))
+                      cfg=best_adjoint_config(n, c64), dtype=c64, start_zero=True)
+    out.append(("HEA 20q x 16 forward", pp.fwd0 or pp.fwd, False))
     out.append(("HEA 20q x 16 adjoint", pp.adj, True))
     for n in (20, 22, 24, 26):
         circ = two_local_circ(brickwall_skeleton(n, 1)).build()
         prog = compile_ops(lower_circuit(circ, n), n, sim_tile_config(n, c64))
         out.append((f"two-local brickwall {n}q", prog, False))
+    # StatevectorSim.run() from |0…0⟩ runs the variant whose first pass
+    # generates the state (only that pass's kernel differs)
+    from .program import zero_start_variant
+    for name, prog, dual in list(out):
+        z = None if dual or name.startswith("HEA") else zero_start_variant(prog, 0)
+        if z is not None:
+            out.append((name + ", from |0...0>", z, False))
     return out
 
 

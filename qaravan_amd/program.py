@@ -39,6 +39,7 @@ PASSF_HALF_LDS = 2
 PASSF_QUARTER_LDS = 4
 PASSF_REMAP_STORE = 8
 PASSF_GENERIC = 16
+PASSF_GEN_ZERO = 32
 OP_MAT1, OP_MAT2, OP_CX, OP_CZ, OP_ROT1, OP_ROT2 = 1, 2, 3, 4, 5, 6
 OPF_NEG, OPF_GRAD = 1, 2
 RB = 4
@@ -1013,6 +1014,27 @@ def compile_ops(ops, n, cfg=None, grad=False, n_params=0, fuse=True, keep_layout
     if keep_layout and end_pos is not None and list(end_pos) != list(range(n)):
         prog.end_pos = list(end_pos)
     return prog
+
+
+def zero_start_variant(prog, dtype_code):
+    """The same program with its first pass generating |0…0⟩ in registers
+    instead of reading the state (the caller then skips the initialisation
+    sweep), or None when that pass does not run in a compiled kernel of its
+    own (the only kernels that generate)."""
+    from .specialize import eligible
+    if not getattr(prog, "specialize", False) or not prog.passes or prog.passes[0].kind != PASS_TILE:
+        return None
+    off = int(np.frombuffer(prog.blob[:HDR_DT.itemsize].tobytes(), dtype=HDR_DT)[0]['off_passes'])
+    row = np.frombuffer(prog.blob[off:off + PASS_DT.itemsize].tobytes(), dtype=PASS_DT)[0].copy()
+    if int(row['flags']) & (PASSF_GENERIC | PASSF_PREFETCH) or not eligible(row, dtype_code, False):
+        return None
+    row['flags'] = int(row['flags']) | PASSF_GEN_ZERO
+    blob = prog.blob.copy()
+    blob[off:off + PASS_DT.itemsize] = np.frombuffer(row.tobytes(), dtype=np.uint8)
+    out = Program(prog.n, prog.passes, blob, prog.grad_params, prog.n_params)
+    out.specialize = True
+    out.final_pos, out.final_cx, out.end_pos = prog.final_pos, prog.final_cx, prog.end_pos
+    return out
 
 
 def _fold_final_cx(passes, final_pos, n):

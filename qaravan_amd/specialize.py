@@ -152,9 +152,12 @@ def _body(R, p, stages, ops, dual, store_to="psi"):
                 f"    const uint64_t base = sp_base_wg<{R}, {tb}, {rb}>(pass, n, wg);",
                 f"    sp_load2<{R}, {tb}, {rb}>(psi, lam, base, st[0], v, w);"]
     else:
+        # the first pass of a program that starts from |0…0⟩ generates it
+        load = "sp_gen_zero" if int(p['flags']) & P.PASSF_GEN_ZERO else "sp_load"
+        src = "" if load == "sp_gen_zero" else "psi, "
         out += [f"    cpx<{R}> v[{1 << rb}];",
                 f"    const uint64_t base = sp_base_wg<{R}, {tb}, {rb}>(pass, n, wg);",
-                f"    sp_load<{R}, {tb}, {rb}>(psi, base, st[0], v);"]
+                f"    {load}<{R}, {tb}, {rb}>({src}base, st[0], v);"]
     W = min(1 << tb, 64)
     grads = []          # (variable, op index) of this stage's gradient terms not yet stored
 

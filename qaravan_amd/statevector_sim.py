@@ -34,7 +34,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .program import compile_ops, lower_circuit, lower_gate
+from .program import compile_ops, lower_circuit, lower_gate, zero_start_variant
 from .utils import pretty_print_sv
 
 
@@ -236,6 +236,7 @@ class StatevectorSim:
         self._host = None
         self.ran = False
         self._prog = None
+        self._prog0 = None     # the program with a first pass that generates |0…0⟩ (False: none)
         self._scratch = DeviceScratch(self.compute_device)
 
     # ------------------------------------------------------------ state
@@ -294,6 +295,13 @@ class StatevectorSim:
             self._prog = compile_ops(ops, self.num_sites, self.tile)
         return self._prog
 
+    def _program_from_zero(self):
+        """program() with its first pass generating |0…0⟩ in registers (no
+        initialisation sweep), or None when that pass has no compiled kernel."""
+        if self._prog0 is None:
+            self._prog0 = zero_start_variant(self.program(), _lib.dtype_code(self.dtype)) or False
+        return self._prog0 or None
+
     def _run_gates(self, gates):
         """Apply ``gates`` (application order) to the current state."""
         from .state_autograd import differentiable_run
@@ -324,8 +332,16 @@ class StatevectorSim:
     def run(self, progress_bar=True, debug=0):
         """Simulate the whole circuit; returns the state as shape (2,)*n
         (base_sim.py:30-53).  debug=1/2 print the state after every layer/gate."""
-        self.initialize_state()
         circ = self.circ.build() if not self.circ.built else self.circ
+        prog0 = None
+        if not debug and self.backend != "torch" and self.init_state is None:
+            self._check_no_grad(circ.gate_list)
+            prog0 = self._program_from_zero()
+        if prog0 is not None:
+            # the first pass generates |0…0⟩: the buffer needs no initialisation
+            self._set_flat(torch.empty(2 ** self.num_sites, dtype=self.dtype, device=self.compute_device))
+        else:
+            self.initialize_state()
         if debug:
             for layer in circ.layers:
                 for op in layer:
@@ -340,7 +356,7 @@ class StatevectorSim:
         else:
             self._check_no_grad(circ.gate_list)
             with torch.no_grad():
-                prog = self.program()
+                prog = prog0 if prog0 is not None else self.program()
                 if progress_bar:
                     from tqdm import tqdm
                     with tqdm(total=1, desc=f"{prog.n_passes} fused passes") as bar:

@@ -69,6 +69,10 @@ def run(blob, sv, theta=None, lam=None, grad=None):
         else:
             L = int(p['L'])
             tile_bits = [int(b) for b in p['bits'][:L]]
+            if int(p['flags']) & P.PASSF_GEN_ZERO:
+                # the pass starts from |0…0⟩ whatever the buffer holds
+                sv = np.zeros_like(sv)
+                sv[0] = 1.0
             for s in range(int(p['stage_begin']), int(p['stage_begin']) + int(p['n_stages'])):
                 st = stages[s]
                 rb = [int(x) for x in st['rbits'][:int(p['rb'])]]

@@ -575,3 +575,33 @@ def test_layout_handoff_between_programs(remap2):
     assert np.linalg.norm(sv - ref) < 1e-12
     with pytest.raises(ValueError):
         P.compile_ops(P.lower_circuit(c2.build(), n), n, cfg2, pos0=p1.end_pos)
+
+
+def test_zero_start_variant_ignores_the_buffer():
+    """The variant whose first pass generates |0…0⟩ gives the oracle's state
+    whatever the state buffer held; the flag is refused anywhere but on the
+    first pass of a forward program (host check, no GPU)."""
+    import ctypes
+
+    from qaravan_amd import _lib
+    n = 12
+    circ = brickwork_circuit(n, 3)
+    cfg = P.TileConfig(tb=5, low_bits=4, max_cost=1e9, specialize=True, io_lane_bits=4, rb=5, lds_split=1,
+                       remap=True)
+    prog = P.compile_ops(P.lower_circuit(circ.build(), n), n, cfg)
+    z = P.zero_start_variant(prog, 0)
+    assert z is not None and z.final_pos == prog.final_pos
+    junk = np.random.default_rng(0).standard_normal(2 ** n) + 0j
+    sv, _ = prog_interp.run(z.blob, junk)
+    assert np.linalg.norm(sv - O.run(circ.gate_list, n)) < 1e-12
+    # the same flag on the second pass is refused by the library's program check
+    h = np.frombuffer(z.blob[:P.HDR_DT.itemsize].tobytes(), dtype=P.HDR_DT)[0]
+    bad = z.blob.copy()
+    off = int(h['off_passes']) + P.PASS_DT.itemsize
+    row = np.frombuffer(bad[off:off + P.PASS_DT.itemsize].tobytes(), dtype=P.PASS_DT)[0].copy()
+    row['flags'] = int(row['flags']) | P.PASSF_GEN_ZERO
+    bad[off:off + P.PASS_DT.itemsize] = np.frombuffer(row.tobytes(), dtype=np.uint8)
+    lib = _lib.load()
+    need = ctypes.c_size_t()
+    assert lib.svh_program_scratch_bytes(z.blob.ctypes.data, 0, ctypes.byref(need)) == 0
+    assert lib.svh_program_scratch_bytes(bad.ctypes.data, 0, ctypes.byref(need)) == -8
